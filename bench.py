@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the visibility simulator on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload hera350] [--gemm split_bf16]
+
+A step is one whole simulation of the workload with its inputs resident in
+HBM: for the default ``hera350`` workload, 350 antennas x 2^20 sources x 32
+times x 64 frequencies (2048 (time, frequency) units).  The headline value is
+the useful Hermitian GEMM rate 4 N (N+1) K T F / step time in TFLOP/s of the
+split-BF16 path; the plain FP32 path is timed on the same workload beside it,
+and both are compared with the FP64 path on a sample of units.  One JSON line
+is printed on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "vis GEMM TFLOP/s & p99 job latency, split-BF16 vs FP32 at ≤1e-5 rel err"
+PEAK_TFLOPS = {"split_bf16": 2500.0, "split_fp16": 2500.0, "fp32": 157.3, "fp64": 78.6}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def herk_flops(N, K, units):
+    return 4.0 * N * (N + 1) * K * units
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def run_timed(stages, sky, beam, w, gemm, steps, warmup):
+    for _ in range(warmup):
+        stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops, gemm=gemm, want_output=False,
+                        dtype=np.float32)
+    t0 = time.perf_counter()
+    _, st = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops, gemm=gemm, reps=steps, timing=True,
+                            want_output=False, dtype=np.float32)
+    wall = time.perf_counter() - t0
+    return st, wall
+
+
+def cpu_baseline(w, nsrc, gpu_unit):
+    """vis_cpu restated (float32, one call per frequency) on one unit of the workload."""
+    from oracle.cpu_ref import vis_cpu
+    from oracle.vis_oracle import vis_oracle_units
+
+    N = w.antpos.shape[0]
+    crd = w.crd_eq[:, :nsrc]
+    I = w.I_sky[:nsrc]
+    t0 = time.perf_counter()
+    v = vis_cpu(w.antpos, float(w.freqs[-1]), w.eq2tops[:1], crd, I, w.bm_cube, precision=1)
+    dt = time.perf_counter() - t0
+    # the FP64 oracle on the same unit checks the GPU result for it
+    ref = vis_oracle_units(w.antpos, w.freqs[-1:], w.eq2tops[:1], crd, I, w.bm_cube, [(0, 0)])[0]
+    cores = len(os.sched_getaffinity(0))
+    res = {
+        "value": herk_flops(N, nsrc, 1) / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+        "sample": f"1 unit (time 0, {w.freqs[-1] / 1e6:.0f} MHz), N={N}, first {nsrc} sources; "
+                  f"{dt:.2f} s; vis_cpu restated in float32 (scipy linear spline beam, BLAS "
+                  f"dot products on {cores} threads)",
+        "seconds": dt,
+        "cpu_fp32_rel_err_vs_oracle": rel_err(v[0], ref),
+    }
+    for gemm, vg in gpu_unit.items():
+        res[f"gpu_{gemm}_rel_err_vs_oracle"] = rel_err(vg, ref)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="hera350", choices=["hera350", "disk128", "diffuse", "hex37"])
+    ap.add_argument("--gemm", default="split_bf16", choices=["split_bf16", "split_fp16", "fp32"])
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 sample comparison")
+    ap.add_argument("--cpu-nsrc", type=int, default=2 ** 16)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
+              "multi-GPU path (time/frequency units would shard without communication); "
+              "run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from hera_gpu_amd import configs, stages
+
+    t0 = time.perf_counter()
+    w = configs.WORKLOADS[args.workload]()
+    s = w.shape
+    N, K, T, F = s["N"], s["K"], s["T"], s["F"]
+    units = T * F
+    log(f"[bench] workload {w.name}: N={N} K={K} T={T} F={F} P={s['P']} ({time.perf_counter() - t0:.1f}s)")
+    sky = stages.Sky(w.crd_eq, w.I_sky)
+    beam = stages.Beam(w.bm_cube)
+    flops_job = herk_flops(N, K, units)
+
+    log(f"[bench] {args.gemm}: warmup {args.warmup}, timed {args.steps}")
+    st, wall = run_timed(stages, sky, beam, w, args.gemm, args.steps, args.warmup)
+    ms_step = st["total_ms"] / args.steps
+    value = flops_job / (ms_step / 1e3) / 1e12
+    herk_avg_ms = st["herk_ms"] / st["herk_n"]
+    flops_per_launch = flops_job * args.steps / st["herk_n"]
+    herk_tflops = flops_per_launch / (herk_avg_ms / 1e3) / 1e12
+    log(f"[bench] {args.gemm}: {ms_step:.1f} ms/step, {value:.1f} TFLOP/s whole job, "
+        f"herk {herk_tflops:.1f} TFLOP/s ({st['herk_n']} launches, {herk_avg_ms:.3f} ms avg)")
+    phasor_avg_ms = st["phasor_ms"] / st["phasor_n"]
+    elem = 16 if args.gemm == "fp64" else 8
+    # phasor launch writes units_per_launch * N * K phasors of `elem` bytes
+    phasor_bytes = elem * N * K * units * args.steps / st["phasor_n"]
+    stage_ms = {k: st[k] / args.steps for k in ("coords_ms", "phasor_ms", "herk_ms", "reduce_ms")}
+
+    fp32 = None
+    if not args.no_fp32 and args.gemm != "fp32":
+        log("[bench] fp32 comparison: warmup 0, timed 1")
+        st32, _ = run_timed(stages, sky, beam, w, "fp32", 1, 0)
+        v32 = flops_job / (st32["total_ms"] / 1e3) / 1e12
+        fp32 = {"value": v32, "ms_per_step": st32["total_ms"],
+                "herk_tflops": flops_job / st32["herk_n"] / (st32["herk_ms"] / st32["herk_n"] / 1e3) / 1e12}
+        log(f"[bench] fp32: {st32['total_ms']:.1f} ms/step, {v32:.1f} TFLOP/s")
+
+    accuracy = None
+    if not args.no_accuracy:
+        # FP64 path (pinned to the oracle by the tests) on a sample of units
+        tsel = sorted({0, T - 1})
+        fsel = sorted({0, F // 3, (2 * F) // 3, F - 1})
+        beam64 = stages.Beam(w.bm_cube.astype(np.float64), dtype=np.float64)
+        ref, _ = stages.simulate(sky, beam64, w.antpos, w.freqs[fsel], w.eq2tops[tsel], gemm="fp64",
+                                 dtype=np.float32)
+        accuracy = {"sample": {"times": tsel, "freqs_mhz": [round(float(w.freqs[i]) / 1e6, 3) for i in fsel]},
+                    "reference": "FP64 GPU path (pinned to the FP64 oracle by tests)"}
+        for gemm in ("split_bf16", "split_fp16", "fp32"):
+            v, _ = stages.simulate(sky, beam, w.antpos, w.freqs[fsel], w.eq2tops[tsel], gemm=gemm,
+                                   dtype=np.float32)
+            per_f = [rel_err(v[i], ref[i]) for i in range(len(fsel))]
+            accuracy[gemm] = {"max_rel_err": max(per_f), "per_freq": per_f}
+            log(f"[bench] rel err {gemm}: max {max(per_f):.2e}")
+        beam64.close()
+
+    cpu = None
+    if not args.no_cpu:
+        nsrc = min(args.cpu_nsrc, K)
+        sub = stages.Sky(w.crd_eq[:, :nsrc], w.I_sky[:nsrc])
+        gpu_unit = {}
+        for gemm in ("split_bf16", "fp32"):
+            v, _ = stages.simulate(sub, beam, w.antpos, w.freqs[-1:], w.eq2tops[:1], gemm=gemm, dtype=np.float32)
+            gpu_unit[gemm] = v[0, 0]
+        sub.close()
+        log(f"[bench] cpu baseline on {nsrc} sources")
+        cpu = cpu_baseline(w, nsrc, gpu_unit)
+        log(f"[bench] cpu: {cpu['seconds']:.1f} s, {cpu['value']:.4f} TFLOP/s")
+
+    peak = PEAK_TFLOPS[args.gemm]
+    out = {
+        "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16" if args.gemm == "split_bf16" else args.gemm,
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: N={N} antennas, K={K} sources, T={T} times, F={F} freqs, "
+                               f"beam {s['P']}x{s['P']}; one step = the whole {units}-unit simulation",
+                   "gemm": args.gemm, "flops_per_step": flops_job,
+                   "flop_count": "useful Hermitian 4 N (N+1) K per unit"},
+        "roofline": {"bound": "mfma", "achieved": herk_tflops, "peak": peak, "unit": "TFLOP/s",
+                     "frac": herk_tflops / peak, "traffic": None, "kernel": f"herk_kernel<{args.gemm}>",
+                     "avg_launch_ms": herk_avg_ms, "launches": st["herk_n"],
+                     "flops_per_launch": flops_per_launch},
+        "stages_ms_per_step": stage_ms,
+        "phasor": {"avg_launch_ms": phasor_avg_ms, "bytes_per_launch": phasor_bytes,
+                   "GBps": phasor_bytes / (phasor_avg_ms / 1e3) / 1e9, "hbm_frac": phasor_bytes /
+                   (phasor_avg_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "fp32": fp32,
+        "speedup_vs_fp32": (value / fp32["value"]) if fp32 else None,
+        "accuracy": accuracy,
+        "cpu_baseline": cpu,
+        "serving": None,
+        "wall_s_timed": wall,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

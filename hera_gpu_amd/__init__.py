@@ -1,0 +1,10 @@
+"""MI355X-native (gfx950) visibility simulator: a drop-in for
+``hera_gpu.vis.vis_gpu`` (hera_gpu/__init__.py:3, hera_gpu/vis.py:144).
+
+Importing the package does not touch the GPU; the HIP library is loaded on
+the first call.
+"""
+from .vis import vis_gpu  # noqa: F401
+
+__all__ = ["vis_gpu"]
+__version__ = "0.1.0"

@@ -1,0 +1,160 @@
+"""ctypes binding of ``libheravis.so`` (declarations in include/heravis.h).
+
+The library is built in-tree by ``hera_gpu_amd.build``.  There is no CPU
+fallback: if the library or a GPU is missing, every call raises.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as C
+import os
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libheravis.so"
+
+HV_OK = 0
+HV_ERR_ARG = -1
+HV_ERR_HIP = -2
+HV_ERR_NOMEM = -3
+HV_ERR_STATE = -4
+HV_ERR_BUSY = -5
+
+HV_F32 = 0
+HV_F64 = 1
+
+MODES = {"fp32": 0, "split_bf16": 1, "split_fp16": 2, "fp64": 3}
+MODE_NAMES = {v: k for k, v in MODES.items()}
+
+# Every symbol the header declares, with its ctypes signature.
+_vp = C.c_void_p
+_i = C.c_int
+_i64 = C.c_int64
+_d = C.c_double
+_sz = C.c_size_t
+
+
+class StageTimes(C.Structure):
+    _fields_ = [("coords_ms", _d), ("phasor_ms", _d), ("herk_ms", _d), ("reduce_ms", _d),
+                ("total_ms", _d), ("coords_n", _i64), ("phasor_n", _i64), ("herk_n", _i64),
+                ("reduce_n", _i64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SIGNATURES = {
+    "hv_version": (_i, []),
+    "hv_last_error": (C.c_char_p, []),
+    "hv_device_count": (_i, [C.POINTER(_i)]),
+    "hv_ctx_create": (_i, [_i, _sz, C.POINTER(_vp)]),
+    "hv_ctx_destroy": (_i, [_vp]),
+    "hv_vis": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _vp]),
+    "hv_k_rotate": (_i, [_vp, _i, _vp, _vp, _i64, _vp]),
+    "hv_k_beam": (_i, [_vp, _i, _vp, _i, _i, _vp, _i64, _vp]),
+    "hv_k_phasor": (_i, [_vp, _i, _i, _vp, _i, _i, _vp, _vp, _i64, _vp, _d, _vp]),
+    "hv_k_herk": (_i, [_vp, _i, _vp, _i, _i64, _vp]),
+    "hv_sky_upload": (_i, [_vp, _i, _vp, _vp, _i64, C.POINTER(_vp)]),
+    "hv_sky_free": (_i, [_vp]),
+    "hv_beam_upload": (_i, [_vp, _i, _vp, _i, _i, C.POINTER(_vp)]),
+    "hv_beam_free": (_i, [_vp]),
+    "hv_simulate": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i,
+                         C.POINTER(StageTimes)]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(f"{LIB_PATH} is not built; run `python -m hera_gpu_amd.build`")
+            lib = C.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(lib, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = lib
+    return _lib
+
+
+class HeravisError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc == HV_OK:
+        return
+    msg = load().hv_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == HV_ERR_ARG:
+        raise AssertionError(text)          # the reference raises AssertionError (vis.py:147,162-169)
+    if rc == HV_ERR_NOMEM:
+        raise MemoryError(text)
+    raise HeravisError(f"[{rc}] {text}")
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device, stream and workspace (wraps hv_ctx)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.hv_ctx_create(device, hbm_budget, C.byref(h)), "hv_ctx_create")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            load().hv_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts: dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    """Process-wide context of a device, created on first use."""
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        budget = int(os.environ.get("HERAVIS_HBM_BUDGET", "0"))
+        ctx = Context(device, budget)
+        with _lock:
+            _contexts.setdefault(device, ctx)
+            ctx = _contexts[device]
+    return ctx
+
+
+@atexit.register
+def _close_all():
+    for c in list(_contexts.values()):
+        try:
+            c.close()
+        except Exception:
+            pass
+    _contexts.clear()
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(load().hv_device_count(C.byref(n)))
+    return n.value

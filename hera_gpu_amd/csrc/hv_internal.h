@@ -1,0 +1,125 @@
+// Internal declarations shared by the HIP translation units of libheravis.
+//
+// Device data layout (one simulation "job"):
+//   antpos     double[N][3]            metres
+//   inv_lambda double[F]               nu / c  (1/m)
+//   eq2tops    double[T][9]            row-major 3x3
+//   crd_eq     double[3][Kpad]         zero beyond K
+//   isqrt      double[Kpad]            sqrt(I_sky), zero beyond K
+//   beam       real[N][P][P]           [ant][y][x]
+//   crd_top    double[T][3][Kpad]      rotated source directions
+//   W          operand[units][N][row]  phasor rows, one per antenna (below)
+//   partials   acc[units][nsplit][ntiles][2][TILE][TILE]   (Re, Im) per split
+//   vis        complex[F][T][N][N]
+//
+// Phasor operand rows interleave (re, im) per source so a row is the
+// complex phasor v[a][0..Kpad):
+//   FP32 / FP64 : complex64 / complex128 per source;
+//   split modes : 32-byte groups of 4 sources, [8 hi halves | 8 lo halves],
+//                 each half ordered (x0 y0 x1 y1 x2 y2 x3 y3).
+// Row bytes: 8*Kpad (FP32 and split), 16*Kpad (FP64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hv {
+
+constexpr int KALIGN = 64;          // Kpad multiple (sources)
+constexpr int KSPLIT = 8192;        // sources per K-split (fixed: results do not
+                                    // depend on padding or batching)
+constexpr int FOLD_SRC = 512;       // sources per FP32 accumulator chain
+constexpr double C_LIGHT = 299792458.0;
+
+enum Mode { MODE_FP32 = 0, MODE_SPLIT_BF16 = 1, MODE_SPLIT_FP16 = 2, MODE_FP64 = 3 };
+
+inline bool mode_is_f64(int mode) { return mode == MODE_FP64; }
+inline size_t operand_row_bytes(int mode, int64_t kpad) {
+    return (size_t)kpad * (mode_is_f64(mode) ? 16 : 8);
+}
+
+struct CoordsArgs {
+    const double* eq2tops;   // [T][9]
+    const double* crd_eq;    // [3][Kpad]
+    double* crd_top;         // [T][3][Kpad]
+    int64_t kpad;
+    int ntimes;
+};
+
+struct PhasorArgs {
+    const double* crd_top;   // [3][Kpad] for the first time of the launch;
+                             // time tb of the launch at crd_top + tb*3*Kpad
+    const double* isqrt;     // [Kpad]
+    const void* beam;        // [N][P][P] float or double (in_f64)
+    const double* antpos;    // [N][3]
+    const double* inv_lambda;// [nfreq] nu/c of the frequencies of this launch
+    void* W;                 // operand base of unit 0
+    size_t unit_stride;      // bytes between units
+    size_t row_stride;       // bytes between antenna rows
+    int64_t kpad;
+    int nant;
+    int beam_px;
+    int nfreq;               // frequencies per time in this launch
+    int ntime;               // times in this launch (grid z); unit = tb*nfreq + f
+    float amp_scale;         // multiplies A*sqrt(I) (power of two; FP16 range)
+    int in_f64;              // beam element type is double
+};
+
+struct HerkArgs {
+    const void* W;           // operand base of unit 0
+    size_t unit_stride;
+    size_t row_stride;
+    void* partials;          // [units][nsplit][ntiles][2][TILE][TILE]
+    int nant;                // valid rows
+    int nb;                  // tile blocks per dim
+    int ntiles;              // nb (nb+1) / 2
+    int64_t kpad;
+    int nsplit;
+    int nunits;
+};
+
+struct ReduceArgs {
+    const void* partials;
+    void* vis;               // complex[F][T][N][N]
+    int t0, f0;              // first time / frequency of the launch
+    int nfreq_launch;        // unit u of the launch = (t0 + u / nfreq_launch,
+                             //                        f0 + u % nfreq_launch)
+    int ntimes_total;        // T of the output array
+    int nant;
+    int nb;
+    int ntiles;
+    int nsplit;
+    int nunits;
+    int tile;
+    double scale;            // 1 / amp_scale^2
+};
+
+struct BeamArgs {
+    const void* beam;
+    const void* crd_top;     // [3][K] in dtype
+    void* A;                 // [N][K] in dtype
+    int64_t nsrc;
+    int nant;
+    int beam_px;
+    int f64;
+};
+
+struct SplitArgs {           // complex phasors -> operand rows (hv_k_herk)
+    const void* v;           // [N][K] complex64 or complex128
+    void* W;
+    size_t row_stride;
+    int64_t nsrc;
+    int64_t kpad;
+    int nant;
+    float scale;
+};
+
+int herk_tile(int mode);
+
+hipError_t launch_coords(const CoordsArgs& a, hipStream_t s);
+hipError_t launch_phasor(int mode, const PhasorArgs& a, hipStream_t s);
+hipError_t launch_herk(int mode, const HerkArgs& a, hipStream_t s);
+hipError_t launch_reduce(int mode, const ReduceArgs& a, hipStream_t s);
+hipError_t launch_beam(const BeamArgs& a, hipStream_t s);
+hipError_t launch_split(int mode, const SplitArgs& a, hipStream_t s);
+
+}  // namespace hv

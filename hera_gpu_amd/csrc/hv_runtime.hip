@@ -1,0 +1,806 @@
+// Host runtime and C ABI of libheravis (see include/heravis.h).
+//
+// One hv_ctx owns a device, a stream and a grow-only workspace.  A
+// simulation is planned as: one rotation launch for all times, then for each
+// batch of (time, frequency) units a phasor build, the Hermitian MFMA
+// contraction and the K-split reduction, all on the context stream.  The
+// plan replaces the reference's per-call JIT, texture upload, per-chunk
+// streams and host-side chunk sum (hera_gpu/vis.py:176-258).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "heravis.h"
+#include "hv_internal.h"
+
+using namespace hv;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HV_HIP(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(HV_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define HV_CHECK(expr)            \
+    do {                          \
+        int _r = (expr);          \
+        if (_r != HV_OK) return _r; \
+    } while (0)
+
+extern "C" int hv_version(void) { return 100; }
+extern "C" const char* hv_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hv_device_count(int* count) {
+    if (!count) return fail(HV_ERR_ARG, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return HV_OK;
+}
+
+// --------------------------------------------------------------- workspace
+namespace {
+
+enum Buf {
+    B_ANTPOS, B_INVLAM, B_EQ2TOPS, B_CRD_EQ, B_ISQRT, B_BEAM, B_CRD_TOP, B_W, B_PART,
+    B_OUT, B_AUX, B_COUNT
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct hv_ctx {
+    int device = 0;
+    size_t budget = 0;
+    size_t used = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevBuf buf[B_COUNT];
+};
+
+struct hv_sky {
+    hv_ctx* ctx;
+    double* crd_eq;   // [3][Kpad]
+    double* isqrt;    // [Kpad]
+    int64_t nsrc, kpad;
+    double max_isqrt;
+};
+
+struct hv_beam {
+    hv_ctx* ctx;
+    void* data;       // [N][P][P] float or double
+    int nant, beam_px, f64;
+    double max_abs;
+};
+
+static int set_device(hv_ctx* c) {
+    HV_HIP(hipSetDevice(c->device));
+    return HV_OK;
+}
+
+static int dev_alloc(hv_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (c->used + bytes > c->budget)
+        return fail(HV_ERR_NOMEM, "device memory request of " + std::to_string(bytes) +
+                                      " bytes exceeds the HBM budget (" + std::to_string(c->budget - c->used) +
+                                      " of " + std::to_string(c->budget) + " bytes left)");
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(HV_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->used += bytes;
+    return HV_OK;
+}
+
+static void dev_free(hv_ctx* c, void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    c->used -= bytes;
+}
+
+static int ensure(hv_ctx* c, Buf b, size_t bytes) {
+    DevBuf& d = c->buf[b];
+    if (d.cap >= bytes && d.p) return HV_OK;
+    if (d.p) {
+        HV_HIP(hipStreamSynchronize(c->stream));
+        dev_free(c, d.p, d.cap);
+        d.p = nullptr;
+        d.cap = 0;
+    }
+    size_t want = std::max(bytes, (size_t)256);
+    HV_CHECK(dev_alloc(c, &d.p, want));
+    d.cap = want;
+    return HV_OK;
+}
+
+extern "C" int hv_ctx_create(int device, size_t hbm_budget, hv_ctx** out) {
+    if (!out) return fail(HV_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(HV_ERR_HIP, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(HV_ERR_ARG, "device index out of range");
+    hv_ctx* c = new hv_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return fail(HV_ERR_HIP, "hipSetDevice failed");
+    }
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        delete c;
+        return fail(HV_ERR_HIP, "hipMemGetInfo failed");
+    }
+    c->budget = hbm_budget ? std::min(hbm_budget, fr) : (size_t)(0.9 * (double)fr);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(HV_ERR_HIP, "hipStreamCreate failed");
+    }
+    *out = c;
+    return HV_OK;
+}
+
+extern "C" int hv_ctx_destroy(hv_ctx* c) {
+    if (!c) return HV_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& d : c->buf) dev_free(c, d.p, d.cap);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HV_OK;
+}
+
+// ------------------------------------------------------- host conversions
+static void to_f64(const void* src, int dtype, size_t n, std::vector<double>& dst) {
+    dst.resize(n);
+    if (dtype == HV_F64) std::memcpy(dst.data(), src, n * sizeof(double));
+    else {
+        const float* f = static_cast<const float*>(src);
+        for (size_t i = 0; i < n; ++i) dst[i] = (double)f[i];
+    }
+}
+
+static int check_dtype(int dtype) {
+    if (dtype != HV_F32 && dtype != HV_F64) return fail(HV_ERR_ARG, "dtype must be HV_F32 or HV_F64");
+    return HV_OK;
+}
+static int check_mode(int mode) {
+    if (mode < HV_MODE_FP32 || mode > HV_MODE_FP64) return fail(HV_ERR_ARG, "unknown mode");
+    return HV_OK;
+}
+
+// Power-of-two scale that puts the largest |A sqrt(I)| near 2^14 (fp16 range).
+static float fp16_scale(double max_amp) {
+    if (!(max_amp > 0.0) || !std::isfinite(max_amp)) return 1.0f;
+    int e = (int)std::floor(std::log2(16384.0 / max_amp));
+    e = std::max(-60, std::min(60, e));
+    return std::ldexp(1.0f, e);
+}
+
+// -------------------------------------------------------------- the job
+namespace {
+
+struct JobDev {
+    int mode;
+    int nant, ntimes, nfreq, beam_px, beam_f64;
+    int64_t kpad;
+    const double* antpos;
+    const double* inv_lambda;   // [F]
+    const double* eq2tops;
+    const double* crd_eq;       // [3][kpad]
+    const double* isqrt;        // [kpad]
+    const void* beam;
+    void* vis;                  // [F][T][N][N] complex, device
+    float amp_scale;
+};
+
+struct StageTimer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> ev[4][2];
+    int begin(int stage) {
+        if (!on) return HV_OK;
+        hipEvent_t e;
+        HV_HIP(hipEventCreate(&e));
+        HV_HIP(hipEventRecord(e, s));
+        ev[stage][0].push_back(e);
+        return HV_OK;
+    }
+    int end(int stage) {
+        if (!on) return HV_OK;
+        hipEvent_t e;
+        HV_HIP(hipEventCreate(&e));
+        HV_HIP(hipEventRecord(e, s));
+        ev[stage][1].push_back(e);
+        return HV_OK;
+    }
+    int collect(hv_stage_times* t) {
+        double ms[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) {
+            for (size_t i = 0; i < ev[k][0].size(); ++i) {
+                float m = 0;
+                HV_HIP(hipEventElapsedTime(&m, ev[k][0][i], ev[k][1][i]));
+                ms[k] += m;
+            }
+        }
+        if (t) {
+            t->coords_ms = ms[0]; t->phasor_ms = ms[1]; t->herk_ms = ms[2]; t->reduce_ms = ms[3];
+            t->coords_n = (int64_t)ev[0][0].size(); t->phasor_n = (int64_t)ev[1][0].size();
+            t->herk_n = (int64_t)ev[2][0].size(); t->reduce_n = (int64_t)ev[3][0].size();
+        }
+        return HV_OK;
+    }
+    ~StageTimer() {
+        for (auto& st : ev)
+            for (auto& v : st)
+                for (auto e : v) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+// Upper bound on operand bytes held at once (one batch of units).
+static size_t operand_budget(hv_ctx* c) {
+    return std::min((size_t)24 << 30, c->budget / 3);
+}
+
+static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
+    const int N = j.nant, T = j.ntimes, F = j.nfreq;
+    const int64_t kpad = j.kpad;
+    const int tile = herk_tile(j.mode);
+    const int nb = (N + tile - 1) / tile;
+    const int ntiles = nb * (nb + 1) / 2;
+    const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
+    const size_t row_bytes = operand_row_bytes(j.mode, kpad);
+    const size_t unit_bytes = (size_t)N * row_bytes;
+    const size_t acc_bytes = mode_is_f64(j.mode) ? 8 : 4;
+    const size_t part_unit = (size_t)nsplit * ntiles * 2 * tile * tile * acc_bytes;
+
+    // batch: all frequencies of a time if they fit, then as many times as fit
+    const size_t wb = operand_budget(c);
+    int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)F, wb / std::max<size_t>(unit_bytes, 1)));
+    int tb = 1;
+    if (fb == F) tb = (int)std::max<size_t>(1, std::min<size_t>((size_t)T, wb / (unit_bytes * F)));
+    // keep the contraction grid within ~64k workgroups per launch
+    while (tb > 1 && (int64_t)tb * fb * nsplit * ntiles > 65536) --tb;
+    const int units = fb * tb;
+
+    HV_CHECK(ensure(c, B_CRD_TOP, (size_t)T * 3 * kpad * sizeof(double)));
+    HV_CHECK(ensure(c, B_W, unit_bytes * units));
+    HV_CHECK(ensure(c, B_PART, part_unit * units));
+    double* crd_top = static_cast<double*>(c->buf[B_CRD_TOP].p);
+
+    CoordsArgs ca{j.eq2tops, j.crd_eq, crd_top, kpad, T};
+    HV_CHECK(tm.begin(0));
+    HV_HIP(launch_coords(ca, c->stream));
+    HV_CHECK(tm.end(0));
+
+    for (int t0 = 0; t0 < T; t0 += tb) {
+        const int nt = std::min(tb, T - t0);
+        for (int f0 = 0; f0 < F; f0 += fb) {
+            const int nf = std::min(fb, F - f0);
+            PhasorArgs pa{};
+            pa.crd_top = crd_top + (size_t)t0 * 3 * kpad;
+            pa.isqrt = j.isqrt;
+            pa.beam = j.beam;
+            pa.antpos = j.antpos;
+            pa.inv_lambda = j.inv_lambda + f0;
+            pa.W = c->buf[B_W].p;
+            pa.unit_stride = unit_bytes;
+            pa.row_stride = row_bytes;
+            pa.kpad = kpad;
+            pa.nant = N;
+            pa.beam_px = j.beam_px;
+            pa.nfreq = nf;
+            pa.ntime = nt;
+            pa.amp_scale = j.amp_scale;
+            pa.in_f64 = j.beam_f64;
+            HV_CHECK(tm.begin(1));
+            HV_HIP(launch_phasor(j.mode, pa, c->stream));
+            HV_CHECK(tm.end(1));
+
+            HerkArgs ha{};
+            ha.W = c->buf[B_W].p;
+            ha.unit_stride = unit_bytes;
+            ha.row_stride = row_bytes;
+            ha.partials = c->buf[B_PART].p;
+            ha.nant = N;
+            ha.nb = nb;
+            ha.ntiles = ntiles;
+            ha.kpad = kpad;
+            ha.nsplit = nsplit;
+            ha.nunits = nt * nf;
+            HV_CHECK(tm.begin(2));
+            HV_HIP(launch_herk(j.mode, ha, c->stream));
+            HV_CHECK(tm.end(2));
+
+            ReduceArgs ra{};
+            ra.partials = c->buf[B_PART].p;
+            ra.vis = j.vis;
+            ra.t0 = t0;
+            ra.f0 = f0;
+            ra.nfreq_launch = nf;
+            ra.ntimes_total = T;
+            ra.nant = N;
+            ra.nb = nb;
+            ra.ntiles = ntiles;
+            ra.nsplit = nsplit;
+            ra.nunits = nt * nf;
+            ra.tile = tile;
+            ra.scale = 1.0 / ((double)j.amp_scale * (double)j.amp_scale);
+            HV_CHECK(tm.begin(3));
+            HV_HIP(launch_reduce(j.mode, ra, c->stream));
+            HV_CHECK(tm.end(3));
+        }
+    }
+    return HV_OK;
+}
+
+// ---------------------------------------------------------- argument checks
+static int check_job_args(int mode, int dtype, int nant, int nfreq, int ntimes, int64_t nsrc, int beam_px) {
+    HV_CHECK(check_mode(mode));
+    HV_CHECK(check_dtype(dtype));
+    if (nant <= 0) return fail(HV_ERR_ARG, "nant must be positive");
+    if (nfreq <= 0) return fail(HV_ERR_ARG, "nfreq must be positive");
+    if (ntimes <= 0) return fail(HV_ERR_ARG, "ntimes must be positive");
+    if (nsrc < 0) return fail(HV_ERR_ARG, "nsrc must be non-negative");
+    if (beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "beam_px must be in [1, 128]");
+    if (nsrc > ((int64_t)1 << 36)) return fail(HV_ERR_ARG, "nsrc too large");
+    return HV_OK;
+}
+
+static void inv_lambdas(const double* freqs, int nfreq, std::vector<double>& out) {
+    out.resize(nfreq);
+    for (int i = 0; i < nfreq; ++i) out[i] = freqs[i] / C_LIGHT;
+}
+
+static int upload(hv_ctx* c, Buf b, const void* host, size_t bytes) {
+    HV_CHECK(ensure(c, b, bytes));
+    HV_HIP(hipMemcpyAsync(c->buf[b].p, host, bytes, hipMemcpyHostToDevice, c->stream));
+    return HV_OK;
+}
+
+// Pad (3,K) directions and K sqrt-intensities to kpad with zeros (a zero
+// direction has z = 0, i.e. not above the horizon, so it contributes 0).
+static void pad_sky(const void* crd_eq, const void* isqrt, int dtype, int64_t nsrc, int64_t kpad,
+                    std::vector<double>& crd, std::vector<double>& isq) {
+    std::vector<double> tmp;
+    to_f64(crd_eq, dtype, (size_t)3 * nsrc, tmp);
+    crd.assign((size_t)3 * kpad, 0.0);
+    for (int d = 0; d < 3; ++d)
+        std::memcpy(crd.data() + d * kpad, tmp.data() + d * nsrc, nsrc * sizeof(double));
+    to_f64(isqrt, dtype, (size_t)nsrc, tmp);
+    isq.assign((size_t)kpad, 0.0);
+    std::memcpy(isq.data(), tmp.data(), nsrc * sizeof(double));
+}
+
+static double max_abs(const void* p, int dtype, size_t n) {
+    double m = 0;
+    if (dtype == HV_F64) {
+        const double* d = static_cast<const double*>(p);
+        for (size_t i = 0; i < n; ++i) m = std::max(m, std::fabs(d[i]));
+    } else {
+        const float* f = static_cast<const float*>(p);
+        for (size_t i = 0; i < n; ++i) m = std::max(m, (double)std::fabs(f[i]));
+    }
+    return m;
+}
+
+// ------------------------------------------------------------ hv_vis
+extern "C" int hv_vis(hv_ctx* c, int mode, int dtype, const void* antpos, int nant, const double* freqs,
+                      int nfreq, const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt,
+                      int64_t nsrc, const void* bm_cube, int beam_px, void* vis_out) {
+    if (!c) return fail(HV_ERR_ARG, "ctx is NULL");
+    HV_CHECK(check_job_args(mode, dtype, nant, nfreq, ntimes, nsrc, beam_px));
+    if (!antpos || !freqs || !eq2tops || !bm_cube || !vis_out || (nsrc > 0 && (!crd_eq || !isqrt)))
+        return fail(HV_ERR_ARG, "NULL array argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    const int64_t kpad = round_up(std::max<int64_t>(nsrc, 1), KALIGN);
+
+    std::vector<double> ap, eq, il, crd, isq;
+    to_f64(antpos, dtype, (size_t)nant * 3, ap);
+    to_f64(eq2tops, dtype, (size_t)ntimes * 9, eq);
+    inv_lambdas(freqs, nfreq, il);
+    pad_sky(crd_eq, isqrt, dtype, nsrc, kpad, crd, isq);
+    const size_t beam_bytes = (size_t)nant * beam_px * beam_px * (dtype == HV_F64 ? 8 : 4);
+
+    HV_CHECK(upload(c, B_ANTPOS, ap.data(), ap.size() * 8));
+    HV_CHECK(upload(c, B_EQ2TOPS, eq.data(), eq.size() * 8));
+    HV_CHECK(upload(c, B_INVLAM, il.data(), il.size() * 8));
+    HV_CHECK(upload(c, B_CRD_EQ, crd.data(), crd.size() * 8));
+    HV_CHECK(upload(c, B_ISQRT, isq.data(), isq.size() * 8));
+    HV_CHECK(upload(c, B_BEAM, bm_cube, beam_bytes));
+    const size_t cplx = mode_is_f64(mode) ? 16 : 8;
+    const size_t out_bytes = (size_t)nfreq * ntimes * nant * nant * cplx;
+    HV_CHECK(ensure(c, B_OUT, out_bytes));
+
+    JobDev j{};
+    j.mode = mode;
+    j.nant = nant; j.ntimes = ntimes; j.nfreq = nfreq; j.beam_px = beam_px; j.beam_f64 = dtype == HV_F64;
+    j.kpad = kpad;
+    j.antpos = (const double*)c->buf[B_ANTPOS].p;
+    j.inv_lambda = (const double*)c->buf[B_INVLAM].p;
+    j.eq2tops = (const double*)c->buf[B_EQ2TOPS].p;
+    j.crd_eq = (const double*)c->buf[B_CRD_EQ].p;
+    j.isqrt = (const double*)c->buf[B_ISQRT].p;
+    j.beam = c->buf[B_BEAM].p;
+    j.vis = c->buf[B_OUT].p;
+    j.amp_scale = (mode == HV_MODE_SPLIT_FP16)
+                      ? fp16_scale(max_abs(bm_cube, dtype, beam_bytes / (dtype == HV_F64 ? 8 : 4)) *
+                                   max_abs(isq.data(), HV_F64, isq.size()))
+                      : 1.0f;
+    StageTimer tm;
+    HV_CHECK(run_job(c, j, tm));
+    HV_HIP(hipMemcpyAsync(vis_out, j.vis, out_bytes, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipStreamSynchronize(c->stream));
+    return HV_OK;
+}
+
+// --------------------------------------------------------- resident inputs
+extern "C" int hv_sky_upload(hv_ctx* c, int dtype, const void* crd_eq, const void* isqrt, int64_t nsrc,
+                             hv_sky** out) {
+    if (!c || !out) return fail(HV_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    HV_CHECK(check_dtype(dtype));
+    if (nsrc < 0 || (nsrc > 0 && (!crd_eq || !isqrt))) return fail(HV_ERR_ARG, "bad sky arrays");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    const int64_t kpad = round_up(std::max<int64_t>(nsrc, 1), KALIGN);
+    std::vector<double> crd, isq;
+    pad_sky(crd_eq, isqrt, dtype, nsrc, kpad, crd, isq);
+    hv_sky* s = new hv_sky();
+    s->ctx = c;
+    s->nsrc = nsrc;
+    s->kpad = kpad;
+    s->max_isqrt = max_abs(isq.data(), HV_F64, isq.size());
+    int r = dev_alloc(c, (void**)&s->crd_eq, crd.size() * 8);
+    if (r == HV_OK) r = dev_alloc(c, (void**)&s->isqrt, isq.size() * 8);
+    if (r != HV_OK) {
+        if (s->crd_eq) dev_free(c, s->crd_eq, crd.size() * 8);
+        delete s;
+        return r;
+    }
+    HV_HIP(hipMemcpy(s->crd_eq, crd.data(), crd.size() * 8, hipMemcpyHostToDevice));
+    HV_HIP(hipMemcpy(s->isqrt, isq.data(), isq.size() * 8, hipMemcpyHostToDevice));
+    *out = s;
+    return HV_OK;
+}
+
+extern "C" int hv_sky_free(hv_sky* s) {
+    if (!s) return HV_OK;
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    dev_free(s->ctx, s->crd_eq, (size_t)3 * s->kpad * 8);
+    dev_free(s->ctx, s->isqrt, (size_t)s->kpad * 8);
+    delete s;
+    return HV_OK;
+}
+
+extern "C" int hv_beam_upload(hv_ctx* c, int dtype, const void* bm_cube, int nant, int beam_px, hv_beam** out) {
+    if (!c || !out || !bm_cube) return fail(HV_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    HV_CHECK(check_dtype(dtype));
+    if (nant <= 0 || beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "bad beam shape");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    const size_t n = (size_t)nant * beam_px * beam_px;
+    const size_t bytes = n * (dtype == HV_F64 ? 8 : 4);
+    hv_beam* b = new hv_beam();
+    b->ctx = c;
+    b->nant = nant;
+    b->beam_px = beam_px;
+    b->f64 = dtype == HV_F64;
+    b->max_abs = max_abs(bm_cube, dtype, n);
+    int r = dev_alloc(c, &b->data, bytes);
+    if (r != HV_OK) {
+        delete b;
+        return r;
+    }
+    HV_HIP(hipMemcpy(b->data, bm_cube, bytes, hipMemcpyHostToDevice));
+    *out = b;
+    return HV_OK;
+}
+
+extern "C" int hv_beam_free(hv_beam* b) {
+    if (!b) return HV_OK;
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    dev_free(b->ctx, b->data, (size_t)b->nant * b->beam_px * b->beam_px * (b->f64 ? 8 : 4));
+    delete b;
+    return HV_OK;
+}
+
+extern "C" int hv_simulate(hv_ctx* c, int mode, const hv_sky* sky, int64_t nsrc_used, const hv_beam* beam,
+                           int dtype, const void* antpos, int nant, const double* freqs, int nfreq,
+                           const void* eq2tops, int ntimes, void* vis_out, int reps, hv_stage_times* times) {
+    if (!c || !sky || !beam || !antpos || !freqs || !eq2tops) return fail(HV_ERR_ARG, "NULL argument");
+    if (sky->ctx != c || beam->ctx != c) return fail(HV_ERR_ARG, "sky/beam belong to another context");
+    HV_CHECK(check_job_args(mode, dtype, nant, nfreq, ntimes, nsrc_used, beam->beam_px));
+    if (beam->nant != nant) return fail(HV_ERR_ARG, "beam antenna count does not match antpos");
+    if (nsrc_used > sky->nsrc) return fail(HV_ERR_ARG, "nsrc_used exceeds the sky catalog");
+    if (mode == HV_MODE_FP64 && !beam->f64) return fail(HV_ERR_ARG, "FP64 mode needs a float64 beam");
+    if (reps < 1) reps = 1;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    // Sources beyond nsrc_used must read as zero: use a zero-padded view when
+    // the catalog is truncated at a non-multiple of KALIGN.
+    const int64_t kpad = round_up(std::max<int64_t>(nsrc_used, 1), KALIGN);
+    const double* crd_eq = sky->crd_eq;
+    const double* isq = sky->isqrt;
+    if (nsrc_used != sky->nsrc) {
+        HV_CHECK(ensure(c, B_CRD_EQ, (size_t)3 * kpad * 8));
+        HV_CHECK(ensure(c, B_ISQRT, (size_t)kpad * 8));
+        double* dc = (double*)c->buf[B_CRD_EQ].p;
+        double* di = (double*)c->buf[B_ISQRT].p;
+        HV_HIP(hipMemsetAsync(dc, 0, (size_t)3 * kpad * 8, c->stream));
+        HV_HIP(hipMemsetAsync(di, 0, (size_t)kpad * 8, c->stream));
+        for (int d = 0; d < 3; ++d)
+            HV_HIP(hipMemcpyAsync(dc + d * kpad, sky->crd_eq + d * sky->kpad, nsrc_used * 8,
+                                  hipMemcpyDeviceToDevice, c->stream));
+        HV_HIP(hipMemcpyAsync(di, sky->isqrt, nsrc_used * 8, hipMemcpyDeviceToDevice, c->stream));
+        crd_eq = dc;
+        isq = di;
+    }
+    std::vector<double> ap, eq, il;
+    to_f64(antpos, dtype, (size_t)nant * 3, ap);
+    to_f64(eq2tops, dtype, (size_t)ntimes * 9, eq);
+    inv_lambdas(freqs, nfreq, il);
+    HV_CHECK(upload(c, B_ANTPOS, ap.data(), ap.size() * 8));
+    HV_CHECK(upload(c, B_EQ2TOPS, eq.data(), eq.size() * 8));
+    HV_CHECK(upload(c, B_INVLAM, il.data(), il.size() * 8));
+    const size_t cplx = mode_is_f64(mode) ? 16 : 8;
+    const size_t out_bytes = (size_t)nfreq * ntimes * nant * nant * cplx;
+    HV_CHECK(ensure(c, B_OUT, out_bytes));
+
+    JobDev j{};
+    j.mode = mode;
+    j.nant = nant; j.ntimes = ntimes; j.nfreq = nfreq; j.beam_px = beam->beam_px; j.beam_f64 = beam->f64;
+    j.kpad = kpad;
+    j.antpos = (const double*)c->buf[B_ANTPOS].p;
+    j.inv_lambda = (const double*)c->buf[B_INVLAM].p;
+    j.eq2tops = (const double*)c->buf[B_EQ2TOPS].p;
+    j.crd_eq = crd_eq;
+    j.isqrt = isq;
+    j.beam = beam->data;
+    j.vis = c->buf[B_OUT].p;
+    j.amp_scale = (mode == HV_MODE_SPLIT_FP16) ? fp16_scale(beam->max_abs * sky->max_isqrt) : 1.0f;
+
+    // Warm allocation outside the timed loop.
+    HV_HIP(hipStreamSynchronize(c->stream));
+    hipEvent_t e0, e1;
+    HV_HIP(hipEventCreate(&e0));
+    HV_HIP(hipEventCreate(&e1));
+    StageTimer tm;
+    tm.on = times != nullptr;
+    tm.s = c->stream;
+    HV_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) HV_CHECK(run_job(c, j, tm));
+    HV_HIP(hipEventRecord(e1, c->stream));
+    HV_HIP(hipEventSynchronize(e1));
+    float total = 0;
+    HV_HIP(hipEventElapsedTime(&total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (times) {
+        HV_CHECK(tm.collect(times));
+        times->total_ms = total;
+    }
+    if (vis_out) {
+        HV_HIP(hipMemcpyAsync(vis_out, j.vis, out_bytes, hipMemcpyDeviceToHost, c->stream));
+        HV_HIP(hipStreamSynchronize(c->stream));
+    }
+    return HV_OK;
+}
+
+// ------------------------------------------------------ kernel-level entries
+extern "C" int hv_k_rotate(hv_ctx* c, int dtype, const void* eq2top, const void* crd_eq, int64_t nsrc,
+                           void* crd_top_out) {
+    if (!c || !eq2top || !crd_top_out || (nsrc > 0 && !crd_eq)) return fail(HV_ERR_ARG, "NULL argument");
+    HV_CHECK(check_dtype(dtype));
+    if (nsrc <= 0) return fail(HV_ERR_ARG, "nsrc must be positive");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    std::vector<double> eq, crd;
+    to_f64(eq2top, dtype, 9, eq);
+    to_f64(crd_eq, dtype, (size_t)3 * nsrc, crd);
+    HV_CHECK(upload(c, B_EQ2TOPS, eq.data(), 72));
+    HV_CHECK(upload(c, B_CRD_EQ, crd.data(), crd.size() * 8));
+    HV_CHECK(ensure(c, B_CRD_TOP, crd.size() * 8));
+    CoordsArgs a{(const double*)c->buf[B_EQ2TOPS].p, (const double*)c->buf[B_CRD_EQ].p,
+                 (double*)c->buf[B_CRD_TOP].p, nsrc, 1};
+    HV_HIP(launch_coords(a, c->stream));
+    std::vector<double> out(crd.size());
+    HV_HIP(hipMemcpyAsync(out.data(), c->buf[B_CRD_TOP].p, out.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipStreamSynchronize(c->stream));
+    if (dtype == HV_F64) std::memcpy(crd_top_out, out.data(), out.size() * 8);
+    else {
+        float* f = static_cast<float*>(crd_top_out);
+        for (size_t i = 0; i < out.size(); ++i) f[i] = (float)out[i];
+    }
+    return HV_OK;
+}
+
+extern "C" int hv_k_beam(hv_ctx* c, int dtype, const void* bm_cube, int nant, int beam_px, const void* crd_top,
+                         int64_t nsrc, void* A_out) {
+    if (!c || !bm_cube || !crd_top || !A_out) return fail(HV_ERR_ARG, "NULL argument");
+    HV_CHECK(check_dtype(dtype));
+    if (nant <= 0 || nsrc <= 0 || beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "bad shape");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    const size_t es = dtype == HV_F64 ? 8 : 4;
+    HV_CHECK(upload(c, B_BEAM, bm_cube, (size_t)nant * beam_px * beam_px * es));
+    HV_CHECK(upload(c, B_CRD_TOP, crd_top, (size_t)3 * nsrc * es));
+    HV_CHECK(ensure(c, B_AUX, (size_t)nant * nsrc * es));
+    BeamArgs a{c->buf[B_BEAM].p, c->buf[B_CRD_TOP].p, c->buf[B_AUX].p, nsrc, nant, beam_px, dtype == HV_F64};
+    HV_HIP(launch_beam(a, c->stream));
+    HV_HIP(hipMemcpyAsync(A_out, c->buf[B_AUX].p, (size_t)nant * nsrc * es, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipStreamSynchronize(c->stream));
+    return HV_OK;
+}
+
+// Decode one operand row of `mode` into complex values (float or double).
+template <typename R>
+static void decode_rows(int mode, const std::vector<uint8_t>& W, size_t row_bytes, int nant, int64_t nsrc,
+                        double inv_scale, R* out) {
+    for (int a = 0; a < nant; ++a) {
+        const uint8_t* row = W.data() + (size_t)a * row_bytes;
+        for (int64_t s = 0; s < nsrc; ++s) {
+            double re, im;
+            if (mode == HV_MODE_FP64) {
+                const double* d = reinterpret_cast<const double*>(row) + 2 * s;
+                re = d[0];
+                im = d[1];
+            } else if (mode == HV_MODE_FP32) {
+                const float* f = reinterpret_cast<const float*>(row) + 2 * s;
+                re = f[0];
+                im = f[1];
+            } else {
+                const uint16_t* g = reinterpret_cast<const uint16_t*>(row + (s / 4) * 32);
+                const int e = (int)(s % 4);
+                auto dec = [&](uint16_t h) -> double {
+                    if (mode == HV_MODE_SPLIT_BF16) {
+                        uint32_t u = (uint32_t)h << 16;
+                        float f;
+                        std::memcpy(&f, &u, 4);
+                        return f;
+                    }
+                    _Float16 x;
+                    std::memcpy(&x, &h, 2);
+                    return (double)(float)x;
+                };
+                re = dec(g[2 * e]) + dec(g[8 + 2 * e]);
+                im = dec(g[2 * e + 1]) + dec(g[8 + 2 * e + 1]);
+            }
+            out[2 * ((size_t)a * nsrc + s)] = (R)(re * inv_scale);
+            out[2 * ((size_t)a * nsrc + s) + 1] = (R)(im * inv_scale);
+        }
+    }
+}
+
+extern "C" int hv_k_phasor(hv_ctx* c, int mode, int dtype, const void* bm_cube, int nant, int beam_px,
+                           const void* crd_top, const void* isqrt, int64_t nsrc, const void* antpos, double freq,
+                           void* v_out) {
+    if (!c || !bm_cube || !crd_top || !isqrt || !antpos || !v_out) return fail(HV_ERR_ARG, "NULL argument");
+    HV_CHECK(check_job_args(mode, dtype, nant, 1, 1, nsrc, beam_px));
+    if (nsrc <= 0) return fail(HV_ERR_ARG, "nsrc must be positive");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    const int64_t kpad = round_up(nsrc, KALIGN);
+    std::vector<double> ct, isq, ap, il{freq / C_LIGHT};
+    pad_sky(crd_top, isqrt, dtype, nsrc, kpad, ct, isq);
+    to_f64(antpos, dtype, (size_t)nant * 3, ap);
+    const size_t es = dtype == HV_F64 ? 8 : 4;
+    HV_CHECK(upload(c, B_BEAM, bm_cube, (size_t)nant * beam_px * beam_px * es));
+    HV_CHECK(upload(c, B_CRD_TOP, ct.data(), ct.size() * 8));
+    HV_CHECK(upload(c, B_ISQRT, isq.data(), isq.size() * 8));
+    HV_CHECK(upload(c, B_ANTPOS, ap.data(), ap.size() * 8));
+    HV_CHECK(upload(c, B_INVLAM, il.data(), 8));
+    const size_t row_bytes = operand_row_bytes(mode, kpad);
+    HV_CHECK(ensure(c, B_W, row_bytes * nant));
+    const float scale = (mode == HV_MODE_SPLIT_FP16)
+                            ? fp16_scale(max_abs(bm_cube, dtype, (size_t)nant * beam_px * beam_px) *
+                                         max_abs(isq.data(), HV_F64, isq.size()))
+                            : 1.0f;
+    PhasorArgs pa{};
+    pa.crd_top = (const double*)c->buf[B_CRD_TOP].p;
+    pa.isqrt = (const double*)c->buf[B_ISQRT].p;
+    pa.beam = c->buf[B_BEAM].p;
+    pa.antpos = (const double*)c->buf[B_ANTPOS].p;
+    pa.inv_lambda = (const double*)c->buf[B_INVLAM].p;
+    pa.W = c->buf[B_W].p;
+    pa.unit_stride = row_bytes * nant;
+    pa.row_stride = row_bytes;
+    pa.kpad = kpad;
+    pa.nant = nant;
+    pa.beam_px = beam_px;
+    pa.nfreq = 1;
+    pa.ntime = 1;
+    pa.amp_scale = scale;
+    pa.in_f64 = dtype == HV_F64;
+    HV_HIP(launch_phasor(mode, pa, c->stream));
+    std::vector<uint8_t> W(row_bytes * nant);
+    HV_HIP(hipMemcpyAsync(W.data(), c->buf[B_W].p, W.size(), hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipStreamSynchronize(c->stream));
+    if (mode == HV_MODE_FP64) decode_rows<double>(mode, W, row_bytes, nant, nsrc, 1.0 / scale, (double*)v_out);
+    else decode_rows<float>(mode, W, row_bytes, nant, nsrc, 1.0 / scale, (float*)v_out);
+    return HV_OK;
+}
+
+extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t nsrc, void* vis_out) {
+    if (!c || !v || !vis_out) return fail(HV_ERR_ARG, "NULL argument");
+    HV_CHECK(check_mode(mode));
+    if (nant <= 0 || nsrc <= 0) return fail(HV_ERR_ARG, "bad shape");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    const bool f64 = mode == HV_MODE_FP64;
+    const int64_t kpad = round_up(nsrc, KALIGN);
+    const size_t cplx = f64 ? 16 : 8;
+    HV_CHECK(upload(c, B_AUX, v, (size_t)nant * nsrc * cplx));
+    const size_t row_bytes = operand_row_bytes(mode, kpad);
+    HV_CHECK(ensure(c, B_W, row_bytes * nant));
+    float scale = 1.0f;
+    if (mode == HV_MODE_SPLIT_FP16) {
+        const float* f = static_cast<const float*>(v);
+        double m = 0;
+        for (size_t i = 0; i < (size_t)nant * nsrc * 2; ++i) m = std::max(m, (double)std::fabs(f[i]));
+        scale = fp16_scale(m);
+    }
+    SplitArgs sa{c->buf[B_AUX].p, c->buf[B_W].p, row_bytes, nsrc, kpad, nant, scale};
+    HV_HIP(launch_split(mode, sa, c->stream));
+    const int tile = herk_tile(mode);
+    const int nb = (nant + tile - 1) / tile;
+    const int ntiles = nb * (nb + 1) / 2;
+    const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
+    HV_CHECK(ensure(c, B_PART, (size_t)nsplit * ntiles * 2 * tile * tile * (f64 ? 8 : 4)));
+    HV_CHECK(ensure(c, B_OUT, (size_t)nant * nant * cplx));
+    HerkArgs ha{};
+    ha.W = c->buf[B_W].p;
+    ha.unit_stride = row_bytes * nant;
+    ha.row_stride = row_bytes;
+    ha.partials = c->buf[B_PART].p;
+    ha.nant = nant;
+    ha.nb = nb;
+    ha.ntiles = ntiles;
+    ha.kpad = kpad;
+    ha.nsplit = nsplit;
+    ha.nunits = 1;
+    HV_HIP(launch_herk(mode, ha, c->stream));
+    ReduceArgs ra{};
+    ra.partials = c->buf[B_PART].p;
+    ra.vis = c->buf[B_OUT].p;
+    ra.t0 = 0;
+    ra.f0 = 0;
+    ra.nfreq_launch = 1;
+    ra.ntimes_total = 1;
+    ra.nant = nant;
+    ra.nb = nb;
+    ra.ntiles = ntiles;
+    ra.nsplit = nsplit;
+    ra.nunits = 1;
+    ra.tile = tile;
+    ra.scale = 1.0 / ((double)scale * scale);
+    HV_HIP(launch_reduce(mode, ra, c->stream));
+    HV_HIP(hipMemcpyAsync(vis_out, c->buf[B_OUT].p, (size_t)nant * nant * cplx, hipMemcpyDeviceToHost,
+                          c->stream));
+    HV_HIP(hipStreamSynchronize(c->stream));
+    return HV_OK;
+}

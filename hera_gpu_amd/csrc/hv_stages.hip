@@ -1,0 +1,332 @@
+// Memory-bound stages of the visibility pipeline: rotation, beam, phasor
+// build, operand split for caller phasors, and the K-split reduction.
+//
+// Reference semantics (hera_gpu/vis.py):
+//   rotation   crd_top = eq2top . crd_eq                       vis.py:228-232
+//   beam       bilinear at texel (P-1)(0.5 x + 0.5), clamp,   vis.py:46-89
+//              A = 0 unless z > 0
+//   delay      tau = antpos . crd_top / c                      vis.py:233-236
+//   phasor     v = A sqrt(I) exp(+i 2 pi nu tau)               vis.py:92-110,159
+// Here the delay is never materialised: each phasor thread forms the path
+// difference d = antpos . crd_top in FP64, converts it to turns with
+// nu / c, keeps only the fractional turn (exact in FP64 for the |turns| < 2^52
+// that occur) and evaluates sin/cos of that small angle.
+#include "hv_internal.h"
+
+namespace hv {
+
+// --------------------------------------------------------------- helpers
+// fmin/fmax return the non-NaN operand, so a NaN coordinate clamps to 0.
+template <typename R>
+__device__ inline R clampr(R v, R lo, R hi) { return fmin(fmax(v, lo), hi); }
+
+// Clamp-to-edge bilinear interpolation of beam[y][x] (P x P) at direction
+// cosines (x, y).  The texel coordinate is formed in FP64 and rounded to R
+// (the reference forms it in double and stores it in DTYPE, vis.py:63,69).
+template <typename R>
+__device__ inline R beam_bilinear(const R* __restrict__ b, int P, double x, double y) {
+    const R bx = (R)((double)(P - 1) * (0.5 * x + 0.5));
+    const R by = (R)((double)(P - 1) * (0.5 * y + 0.5));
+    const R px = floor(bx), py = floor(by);
+    const R fx = bx - px, fy = by - py;
+    const R pm = (R)(P - 1);
+    const int ix0 = (int)clampr(px, (R)0, pm), ix1 = (int)clampr(px + (R)1, (R)0, pm);
+    const int iy0 = (int)clampr(py, (R)0, pm), iy1 = (int)clampr(py + (R)1, (R)0, pm);
+    const R v00 = b[iy0 * P + ix0], v01 = b[iy0 * P + ix1];
+    const R v10 = b[iy1 * P + ix0], v11 = b[iy1 * P + ix1];
+    const R top = fma(fx, v01 - v00, v00);
+    const R bot = fma(fx, v11 - v10, v10);
+    return fma(fy, bot - top, top);
+}
+
+__device__ inline void sincos_turns(double turns, float* s, float* c) {
+    const double r = turns - rint(turns);          // [-0.5, 0.5] turns
+    sincospif((float)(2.0 * r), s, c);
+}
+__device__ inline void sincos_turns(double turns, double* s, double* c) {
+    const double r = turns - rint(turns);
+    sincospi(2.0 * r, s, c);
+}
+
+// Round-to-nearest-even float -> bf16 / fp16 bit patterns and back.
+__device__ inline uint16_t to_bf16(float x) {
+    __bf16 h = (__bf16)x;
+    return __builtin_bit_cast(uint16_t, h);
+}
+__device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ inline uint16_t to_f16(float x) {
+    _Float16 h = (_Float16)x;
+    return __builtin_bit_cast(uint16_t, h);
+}
+__device__ inline float from_f16(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+// Split 4 complex values into a 32-byte group [8 hi | 8 lo] of 16-bit halves.
+template <int MODE>
+__device__ inline void split_group(const float* re, const float* im, uint4* out) {
+    uint16_t hi[8], lo[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const float x = c ? im[j] : re[j];
+            uint16_t h;
+            float back;
+            if constexpr (MODE == MODE_SPLIT_BF16) { h = to_bf16(x); back = from_bf16(h); }
+            else { h = to_f16(x); back = from_f16(h); }
+            hi[2 * j + c] = h;
+            lo[2 * j + c] = (MODE == MODE_SPLIT_BF16) ? to_bf16(x - back) : to_f16(x - back);
+        }
+    }
+    uint32_t w[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        w[i] = (uint32_t)hi[2 * i] | ((uint32_t)hi[2 * i + 1] << 16);
+        w[4 + i] = (uint32_t)lo[2 * i] | ((uint32_t)lo[2 * i + 1] << 16);
+    }
+    out[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    out[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// Write 8 consecutive phasors (one thread's share) of one antenna row.
+template <int MODE, typename R>
+__device__ inline void store_phasors8(char* dst, const R* re, const R* im) {
+    if constexpr (MODE == MODE_FP64) {
+        double2* d = reinterpret_cast<double2*>(dst);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[j] = make_double2(re[j], im[j]);
+    } else if constexpr (MODE == MODE_FP32) {
+        float4* d = reinterpret_cast<float4*>(dst);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            d[j] = make_float4(re[2 * j], im[2 * j], re[2 * j + 1], im[2 * j + 1]);
+    } else {
+        uint4* d = reinterpret_cast<uint4*>(dst);
+        uint4 g[2];
+        split_group<MODE>(re, im, g);
+        d[0] = g[0];
+        d[1] = g[1];
+        split_group<MODE>(re + 4, im + 4, g);
+        d[2] = g[0];
+        d[3] = g[1];
+    }
+}
+
+// ------------------------------------------------------------- rotation
+// One thread per (time, source): crd_top[t][:, s] = eq2top[t] . crd_eq[:, s].
+__global__ __launch_bounds__(256) void coords_kernel(CoordsArgs a) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.y;
+    if (s >= a.kpad) return;
+    const double* R = a.eq2tops + 9 * t;
+    const double x = a.crd_eq[s], y = a.crd_eq[a.kpad + s], z = a.crd_eq[2 * a.kpad + s];
+    double* o = a.crd_top + (size_t)t * 3 * a.kpad;
+    o[s] = fma(R[0], x, fma(R[1], y, R[2] * z));
+    o[a.kpad + s] = fma(R[3], x, fma(R[4], y, R[5] * z));
+    o[2 * a.kpad + s] = fma(R[6], x, fma(R[7], y, R[8] * z));
+}
+
+hipError_t launch_coords(const CoordsArgs& a, hipStream_t s) {
+    dim3 grid((unsigned)((a.kpad + 255) / 256), (unsigned)a.ntimes);
+    hipLaunchKernelGGL(coords_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ fused phasor build
+// Grid (source chunks, antennas).  The antenna's beam is staged in LDS once
+// per workgroup; each thread then handles 8 consecutive sources: beam
+// amplitude A sqrt(I) (zero below the horizon) and FP64 path difference are
+// formed once and reused for every frequency of the launch.
+constexpr int PH_THREADS = 256;
+constexpr int PH_SRC_PER_THREAD = 8;
+constexpr int PH_SRC_PER_WG = 8192;
+
+template <int MODE, typename BT>
+__global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
+    using R = typename std::conditional<MODE == MODE_FP64, double, float>::type;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    BT* sb = reinterpret_cast<BT*>(smem);
+    const int ant = blockIdx.y;
+    const int tb = blockIdx.z;
+    const int P = a.beam_px;
+    const BT* gb = reinterpret_cast<const BT*>(a.beam) + (size_t)ant * P * P;
+    for (int i = threadIdx.x; i < P * P; i += PH_THREADS) sb[i] = gb[i];
+    __syncthreads();
+
+    const double ax = a.antpos[3 * ant], ay = a.antpos[3 * ant + 1], az = a.antpos[3 * ant + 2];
+    const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
+    const double* cy = cx + a.kpad;
+    const double* cz = cx + 2 * a.kpad;
+    const double* isq = a.isqrt;
+    const int64_t wg0 = (int64_t)blockIdx.x * PH_SRC_PER_WG;
+    const int64_t wg1 = min(wg0 + (int64_t)PH_SRC_PER_WG, a.kpad);
+    char* wrow = reinterpret_cast<char*>(a.W) + (size_t)tb * a.nfreq * a.unit_stride +
+                 (size_t)ant * a.row_stride;
+    const size_t elem = (MODE == MODE_FP64) ? 16 : 8;
+
+    for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
+         s0 += PH_THREADS * PH_SRC_PER_THREAD) {
+        R amp[8];
+        double d[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t s = s0 + j;
+            const double x = cx[s], y = cy[s], z = cz[s];
+            R A = (R)0;
+            if (z > 0.0) A = (R)beam_bilinear<BT>(sb, P, x, y);
+            amp[j] = A * (R)isq[s] * (R)a.amp_scale;
+            d[j] = fma(ax, x, fma(ay, y, az * z));
+        }
+        for (int u = 0; u < a.nfreq; ++u) {
+            const double il = a.inv_lambda[u];
+            R re[8], im[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                R sn, cs;
+                sincos_turns(d[j] * il, &sn, &cs);
+                re[j] = amp[j] * cs;
+                im[j] = amp[j] * sn;
+            }
+            store_phasors8<MODE, R>(wrow + (size_t)u * a.unit_stride + (size_t)s0 * elem, re, im);
+        }
+    }
+}
+
+template <int MODE, typename BT>
+static hipError_t launch_phasor_t(const PhasorArgs& a, hipStream_t s) {
+    const size_t lds = (size_t)a.beam_px * a.beam_px * sizeof(BT);
+    dim3 grid((unsigned)((a.kpad + PH_SRC_PER_WG - 1) / PH_SRC_PER_WG), (unsigned)a.nant, (unsigned)a.ntime);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)phasor_kernel<MODE, BT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((phasor_kernel<MODE, BT>), grid, dim3(PH_THREADS), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_phasor(int mode, const PhasorArgs& a, hipStream_t s) {
+    const bool f64 = a.in_f64 != 0;
+    switch (mode) {
+        case MODE_FP32:
+            return f64 ? launch_phasor_t<MODE_FP32, double>(a, s) : launch_phasor_t<MODE_FP32, float>(a, s);
+        case MODE_SPLIT_BF16:
+            return f64 ? launch_phasor_t<MODE_SPLIT_BF16, double>(a, s)
+                       : launch_phasor_t<MODE_SPLIT_BF16, float>(a, s);
+        case MODE_SPLIT_FP16:
+            return f64 ? launch_phasor_t<MODE_SPLIT_FP16, double>(a, s)
+                       : launch_phasor_t<MODE_SPLIT_FP16, float>(a, s);
+        case MODE_FP64:
+            return f64 ? launch_phasor_t<MODE_FP64, double>(a, s) : launch_phasor_t<MODE_FP64, float>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ----------------------------------------------------- standalone beam stage
+// Kernel-level beam entry (one workgroup per (source chunk, antenna)), same
+// device function as the fused phasor build.
+template <typename R>
+__global__ __launch_bounds__(256) void beam_kernel(BeamArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    R* sb = reinterpret_cast<R*>(smem);
+    const int ant = blockIdx.y;
+    const int P = a.beam_px;
+    const R* gb = reinterpret_cast<const R*>(a.beam) + (size_t)ant * P * P;
+    for (int i = threadIdx.x; i < P * P; i += 256) sb[i] = gb[i];
+    __syncthreads();
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= a.nsrc) return;
+    const R* c = reinterpret_cast<const R*>(a.crd_top);
+    const double x = c[s], y = c[a.nsrc + s], z = c[2 * a.nsrc + s];
+    R A = (R)0;
+    if (z > 0.0) A = beam_bilinear<R>(sb, P, x, y);
+    reinterpret_cast<R*>(a.A)[(size_t)ant * a.nsrc + s] = A;
+}
+
+hipError_t launch_beam(const BeamArgs& a, hipStream_t s) {
+    dim3 grid((unsigned)((a.nsrc + 255) / 256), (unsigned)a.nant);
+    if (a.f64) {
+        hipLaunchKernelGGL(beam_kernel<double>, grid, dim3(256), (size_t)a.beam_px * a.beam_px * 8, s, a);
+    } else {
+        hipLaunchKernelGGL(beam_kernel<float>, grid, dim3(256), (size_t)a.beam_px * a.beam_px * 4, s, a);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------- caller phasors -> operand rows
+// One thread per (antenna, 8 sources); zero beyond nsrc up to kpad.
+template <int MODE>
+__global__ __launch_bounds__(256) void split_kernel(SplitArgs a) {
+    using R = typename std::conditional<MODE == MODE_FP64, double, float>::type;
+    const int64_t s0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 8;
+    const int ant = blockIdx.y;
+    if (s0 >= a.kpad) return;
+    R re[8], im[8];
+    const R* v = reinterpret_cast<const R*>(a.v) + (size_t)ant * a.nsrc * 2;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int64_t s = s0 + j;
+        re[j] = s < a.nsrc ? v[2 * s] * (R)a.scale : (R)0;
+        im[j] = s < a.nsrc ? v[2 * s + 1] * (R)a.scale : (R)0;
+    }
+    const size_t elem = (MODE == MODE_FP64) ? 16 : 8;
+    store_phasors8<MODE, R>(reinterpret_cast<char*>(a.W) + (size_t)ant * a.row_stride + s0 * elem, re, im);
+}
+
+hipError_t launch_split(int mode, const SplitArgs& a, hipStream_t s) {
+    dim3 grid((unsigned)((a.kpad / 8 + 255) / 256), (unsigned)a.nant);
+    switch (mode) {
+        case MODE_FP32: hipLaunchKernelGGL(split_kernel<MODE_FP32>, grid, dim3(256), 0, s, a); break;
+        case MODE_SPLIT_BF16: hipLaunchKernelGGL(split_kernel<MODE_SPLIT_BF16>, grid, dim3(256), 0, s, a); break;
+        case MODE_SPLIT_FP16: hipLaunchKernelGGL(split_kernel<MODE_SPLIT_FP16>, grid, dim3(256), 0, s, a); break;
+        case MODE_FP64: hipLaunchKernelGGL(split_kernel<MODE_FP64>, grid, dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- K-split reduce
+// V[a][b] = sum over splits, in split order and in FP64, of the (Re, Im)
+// partial tiles; entries of the lower triangle are read from the mirrored
+// upper-triangle entry and conjugated (the Hermitian fill of
+// test_vis_gpu.py:354-355).  The diagonal is real.
+__device__ inline int tile_id(int bi, int bj, int nb) { return bi * nb - (bi * (bi - 1)) / 2 + (bj - bi); }
+
+template <typename ACC, typename OUT>
+__global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
+    const int b = blockIdx.x * 32 + (threadIdx.x & 31);
+    const int a = blockIdx.y * 8 + (threadIdx.x >> 5);
+    const int u = blockIdx.z;
+    if (a >= r.nant || b >= r.nant) return;
+    const int T = r.tile;
+    const int ta = a / T, tb = b / T;
+    const bool mirror = (ta > tb) || (ta == tb && a > b);
+    const int row = mirror ? b : a, col = mirror ? a : b;
+    const int tid = mirror ? tile_id(tb, ta, r.nb) : tile_id(ta, tb, r.nb);
+    const size_t tsz = (size_t)T * T;
+    const size_t off = (size_t)(row % T) * T + (col % T);
+    const ACC* p = reinterpret_cast<const ACC*>(r.partials) + ((size_t)u * r.nsplit * r.ntiles + tid) * 2 * tsz + off;
+    const size_t split_stride = (size_t)r.ntiles * 2 * tsz;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < r.nsplit; ++s) {
+        re += (double)p[(size_t)s * split_stride];
+        im += (double)p[(size_t)s * split_stride + tsz];
+    }
+    if (mirror) im = -im;
+    if (a == b) im = 0.0;
+    const int64_t t = r.t0 + u / r.nfreq_launch, f = r.f0 + u % r.nfreq_launch;
+    const int64_t unit_off = (f * r.ntimes_total + t) * (int64_t)r.nant * r.nant;
+    OUT* o = reinterpret_cast<OUT*>(r.vis) + 2 * (unit_off + (int64_t)a * r.nant + b);
+    o[0] = (OUT)(re * r.scale);
+    o[1] = (OUT)(im * r.scale);
+}
+
+hipError_t launch_reduce(int mode, const ReduceArgs& r, hipStream_t s) {
+    dim3 grid((unsigned)((r.nant + 31) / 32), (unsigned)((r.nant + 7) / 8), (unsigned)r.nunits);
+    if (mode == MODE_FP64)
+        hipLaunchKernelGGL((reduce_kernel<double, double>), grid, dim3(256), 0, s, r);
+    else
+        hipLaunchKernelGGL((reduce_kernel<float, float>), grid, dim3(256), 0, s, r);
+    return hipGetLastError();
+}
+
+}  // namespace hv

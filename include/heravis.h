@@ -1,0 +1,154 @@
+/*
+ * heravis.h -- C ABI of the MI355X (gfx950) visibility simulator.
+ *
+ * Drop-in target: hera_gpu.vis.vis_gpu (hera_gpu/vis.py:144-262), the only
+ * public entry of the reference's visibility path (re-exported at
+ * hera_gpu/__init__.py:3).  The reference is Python calling PyCUDA and
+ * cuBLAS; it has no C ABI of its own, so the entry points below are what its
+ * ctypes-style FFI for that path binds: one call for the whole simulation
+ * plus one call per device stage that its kernel-level tests
+ * (hera_gpu/tests/test_vis_gpu.py:17-169) launch on their own.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative HV_ERR_* code on failure; the message
+ *     is available from hv_last_error() on the calling thread;
+ *   - host pointers are caller-owned, read (or written) only during the call;
+ *   - arrays are C-contiguous; complex outputs are interleaved (re, im);
+ *   - "dtype" selects float32 (HV_F32) or float64 (HV_F64) element type for
+ *     real inputs, and complex64 / complex128 for complex outputs;
+ *   - shapes follow the reference: antpos (N,3), eq2tops (T,3,3),
+ *     crd_eq (3,K), I_sky / isqrt (K,), bm_cube (N,P,P) indexed [ant,y,x],
+ *     visibilities (F,T,N,N) with V[a,b] = sum_s v_a(s) conj(v_b(s)).
+ */
+#ifndef HERAVIS_H
+#define HERAVIS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HV_OK 0
+#define HV_ERR_ARG (-1)     /* bad shape / argument: Python raises AssertionError */
+#define HV_ERR_HIP (-2)     /* HIP runtime error */
+#define HV_ERR_NOMEM (-3)   /* request exceeds the HBM budget */
+#define HV_ERR_STATE (-4)   /* wrong object state (e.g. closed server) */
+#define HV_ERR_BUSY (-5)    /* non-blocking call would block */
+
+#define HV_F32 0
+#define HV_F64 1
+
+/* Arithmetic of the Hermitian contraction.  The delay phase is FP64 in all. */
+#define HV_MODE_FP32 0        /* complex64 phasors, FP32 MFMA               */
+#define HV_MODE_SPLIT_BF16 1  /* bf16 hi+lo phasors, 3 bf16 MFMA products   */
+#define HV_MODE_SPLIT_FP16 2  /* fp16 hi+lo phasors (power-of-2 scaled)     */
+#define HV_MODE_FP64 3        /* complex128 phasors, FP64 MFMA (precision=2) */
+
+typedef struct hv_ctx hv_ctx;
+
+/* Library version as an integer (major*10000 + minor*100 + patch). */
+int hv_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* hv_last_error(void);
+
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int hv_device_count(int* count);
+
+/* Context: one device, a stream, a grow-only device workspace bounded by
+ * hbm_budget bytes (0 = 90% of free device memory).  Replaces the import-time
+ * pycuda.autoinit context and per-call cublasCreate (vis.py:1,196). */
+int hv_ctx_create(int device, size_t hbm_budget, hv_ctx** out);
+int hv_ctx_destroy(hv_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Whole simulation, host arrays in and out (vis.py:144-262).
+ *   dtype  : element type of antpos, eq2tops, crd_eq, isqrt and bm_cube
+ *            (HV_F32 for precision=1, HV_F64 for precision=2);
+ *   freqs  : F frequencies in Hz (the reference takes one scalar, vis.py:159);
+ *   isqrt  : sqrt(I_sky), taken by the caller as the reference does
+ *            (vis.py:174);
+ *   vis_out: (F,T,N,N) complex64 when mode != HV_MODE_FP64, complex128 else.
+ * Unlike the reference, every source is summed (vis.py:176-177 drops the
+ * K mod chunk tail).
+ * ------------------------------------------------------------------------- */
+int hv_vis(hv_ctx* ctx, int mode, int dtype,
+           const void* antpos, int nant,
+           const double* freqs, int nfreq,
+           const void* eq2tops, int ntimes,
+           const void* crd_eq, const void* isqrt, int64_t nsrc,
+           const void* bm_cube, int beam_px,
+           void* vis_out);
+
+/* ---------------------------------------------------------------------------
+ * Kernel-level entry points, one device stage each, host arrays in and out.
+ * ------------------------------------------------------------------------- */
+
+/* Rotation: crd_top (3,K) = eq2top (3,3) . crd_eq (3,K), computed in FP64,
+ * written in dtype.  Replaces the rotation Sgemm/Dgemm (vis.py:228-232);
+ * pinned like test_cublas (test_vis_gpu.py:140-169). */
+int hv_k_rotate(hv_ctx* ctx, int dtype, const void* eq2top, const void* crd_eq,
+                int64_t nsrc, void* crd_top_out);
+
+/* Beam: A (N,K) = clamp-to-edge bilinear of bm_cube[ant] at crd_top (x,y),
+ * zero unless z > 0, from an LDS-staged copy of each antenna's beam.
+ * Replaces InterpolateBeam (vis.py:46-89); pinned like test_bm_interp_flat /
+ * test_bm_interp_middle (test_vis_gpu.py:31-102). */
+int hv_k_beam(hv_ctx* ctx, int dtype, const void* bm_cube, int nant, int beam_px,
+              const void* crd_top, int64_t nsrc, void* A_out);
+
+/* Phasor: v (N,K) = A * isqrt * exp(+i 2 pi nu (antpos . crd_top) / c) with
+ * the beam A as in hv_k_beam and the delay phase reduced in FP64 turns;
+ * written in the operand format of `mode` and returned as complex64
+ * (complex128 for HV_MODE_FP64; split modes return hi+lo).  Replaces the
+ * delay Sgemm (vis.py:233-236) and MeasEq (vis.py:92-110); pinned like
+ * test_meas_eq (test_vis_gpu.py:103-139). */
+int hv_k_phasor(hv_ctx* ctx, int mode, int dtype,
+                const void* bm_cube, int nant, int beam_px,
+                const void* crd_top, const void* isqrt, int64_t nsrc,
+                const void* antpos, double freq, void* v_out);
+
+/* Hermitian product: V (N,N) = v . v^H of caller phasors v (N,K) (complex64,
+ * or complex128 for HV_MODE_FP64), split to the operand format of `mode`.
+ * Replaces the visibility Cgemm/Zgemm (vis.py:246-250) and the host chunk
+ * sum (vis.py:258). */
+int hv_k_herk(hv_ctx* ctx, int mode, const void* v, int nant, int64_t nsrc,
+              void* vis_out);
+
+/* ---------------------------------------------------------------------------
+ * Resident inputs and timed runs (no reference counterpart: the reference
+ * re-uploads everything per call, vis.py:198-206).  Used by the serving loop
+ * and by measurement, so timed regions start with inputs already in HBM.
+ * ------------------------------------------------------------------------- */
+typedef struct hv_sky hv_sky;     /* crd_eq (3,K) + isqrt (K), FP64 in HBM */
+typedef struct hv_beam hv_beam;   /* bm_cube (N,P,P) in HBM                */
+
+int hv_sky_upload(hv_ctx* ctx, int dtype, const void* crd_eq, const void* isqrt,
+                  int64_t nsrc, hv_sky** out);
+int hv_sky_free(hv_sky* sky);
+int hv_beam_upload(hv_ctx* ctx, int dtype, const void* bm_cube, int nant,
+                   int beam_px, hv_beam** out);
+int hv_beam_free(hv_beam* beam);
+
+/* Per-stage device times of the last hv_simulate call with timing on, in
+ * milliseconds, summed over all launches of that stage, plus launch counts. */
+typedef struct hv_stage_times {
+    double coords_ms, phasor_ms, herk_ms, reduce_ms, total_ms;
+    int64_t coords_n, phasor_n, herk_n, reduce_n;
+} hv_stage_times;
+
+/* Run a simulation on resident sky and beam, using the first nsrc_used sky
+ * sources.  antpos/eq2tops are host arrays in `dtype`.  If vis_out is not
+ * NULL the (F,T,N,N) result is copied to it; reps >= 1 repeats the run (for
+ * timing) and, if times != NULL, records per-stage HIP events. */
+int hv_simulate(hv_ctx* ctx, int mode, const hv_sky* sky, int64_t nsrc_used,
+                const hv_beam* beam, int dtype, const void* antpos, int nant,
+                const double* freqs, int nfreq, const void* eq2tops, int ntimes,
+                void* vis_out, int reps, hv_stage_times* times);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HERAVIS_H */

@@ -1,0 +1,73 @@
+"""The C-ABI library loads and exports every symbol include/heravis.h declares.
+
+No compute calls here: this runs without a GPU.
+"""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from hera_gpu_amd import _lib, vis_gpu
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "heravis.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hv_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_surface():
+    names = declared_functions()
+    for n in ("hv_vis", "hv_k_rotate", "hv_k_beam", "hv_k_phasor", "hv_k_herk", "hv_simulate",
+              "hv_ctx_create", "hv_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of sync with the header"
+
+
+def test_version_and_device_count_without_compute():
+    lib = _lib.load()
+    assert lib.hv_version() >= 100
+    assert _lib.device_count() >= 0
+
+
+def test_null_context_is_an_argument_error():
+    lib = _lib.load()
+    rc = lib.hv_ctx_destroy(None)
+    assert rc == _lib.HV_OK
+    out = np.zeros(4, dtype=np.complex64)
+    rc = lib.hv_vis(None, 0, 0, None, 1, None, 1, None, 1, None, None, 1, None, 1, _lib.ptr(out))
+    assert rc == _lib.HV_ERR_ARG
+    assert b"ctx" in lib.hv_last_error()
+
+
+# Shape checks of vis.py:161-169 fire before any device work (test_vis_gpu.py:172-191).
+NANT, NTIMES, BM_PIX, NPIX = 16, 10, 31, 3072
+
+
+@pytest.mark.parametrize("which", ["antpos", "eq2tops", "crd_eq", "bm_cube"])
+def test_transposed_inputs_raise_assertion(which):
+    args = dict(antpos=np.zeros((NANT, 3)), eq2tops=np.zeros((NTIMES, 3, 3)), crd_eq=np.zeros((3, NPIX)),
+                I_sky=np.zeros(NPIX), bm_cube=np.zeros((NANT, BM_PIX, BM_PIX)))
+    args[which] = args[which].T
+    with pytest.raises(AssertionError):
+        vis_gpu(args["antpos"], 0.15, args["eq2tops"], args["crd_eq"], args["I_sky"], args["bm_cube"])
+
+
+def test_bad_precision_raises_assertion():
+    with pytest.raises(AssertionError):
+        vis_gpu(np.zeros((2, 3)), 0.15, np.zeros((1, 3, 3)), np.zeros((3, 4)), np.zeros(4),
+                np.zeros((2, 3, 3)), precision=3)
+    with pytest.raises(AssertionError):
+        vis_gpu(np.zeros((2, 3)), 0.15, np.zeros((1, 3, 3)), np.zeros((3, 4)), np.zeros(4),
+                np.zeros((2, 3, 3)), precision=2, gemm="split_bf16")
