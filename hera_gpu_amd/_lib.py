@@ -45,6 +45,29 @@ class StageTimes(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class JobInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("slot", C.c_int), ("graph_hit", C.c_int),
+                ("reserved", C.c_int), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("footprint", C.c_int64), ("flops", C.c_double),
+                ("error", C.c_char * 160)]
+
+
+class Job(C.Structure):
+    _fields_ = [("mode", C.c_int), ("flags", C.c_int), ("sky", C.c_void_p), ("nsrc", C.c_int64),
+                ("beam", C.c_void_p), ("antpos", C.c_void_p), ("nant", C.c_int), ("freqs", C.c_void_p),
+                ("nfreq", C.c_int), ("eq2tops", C.c_void_p), ("ntimes", C.c_int), ("vis_out", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("graph_captures", C.c_int64), ("graph_launches", C.c_int64), ("direct_launches", C.c_int64),
+                ("peak_running", C.c_int64), ("pending", C.c_int64), ("running", C.c_int64),
+                ("workspace_bytes", C.c_size_t), ("budget_bytes", C.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 SIGNATURES = {
     "hv_version": (_i, []),
     "hv_last_error": (C.c_char_p, []),
@@ -62,6 +85,12 @@ SIGNATURES = {
     "hv_beam_free": (_i, [_vp]),
     "hv_simulate": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i,
                          C.POINTER(StageTimes)]),
+    "hv_server_create": (_i, [_vp, _i, _sz, _i, C.POINTER(_vp)]),
+    "hv_server_submit": (_i, [_vp, C.POINTER(Job), C.POINTER(_i64)]),
+    "hv_server_poll": (_i, [_vp, C.POINTER(JobInfo), _i, C.POINTER(_i)]),
+    "hv_server_wait": (_i, [_vp, _i64, _d, C.POINTER(JobInfo)]),
+    "hv_server_stats": (_i, [_vp, C.POINTER(ServerStats)]),
+    "hv_server_destroy": (_i, [_vp]),
 }
 
 _lib = None

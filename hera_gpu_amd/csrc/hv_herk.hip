@@ -118,9 +118,10 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
     decode_tile(tile, a.nb, bi, bj);
     const bool diag = (bi == bj);
 
-    const int kst = (int)(a.kpad / SRC_STAGE);
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / SRC_STAGE);
     const int s_begin = split * STAGES_SPLIT;
     const int s_end = min(s_begin + STAGES_SPLIT, kst);
+    if (s_begin >= s_end) return;   // split beyond the job's sources (uniform per workgroup)
 
     const char* Wu = reinterpret_cast<const char*>(a.W) + (size_t)unit * a.unit_stride;
 

@@ -1,0 +1,136 @@
+// Host-side internals shared by the runtime (hv_runtime.hip) and the
+// serving executor (hv_serve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "heravis.h"
+#include "hv_internal.h"
+
+namespace hv {
+
+int fail(int code, const std::string& msg);
+
+#define HV_HIP(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess)                                                                           \
+            return ::hv::fail(HV_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(_e));    \
+    } while (0)
+
+#define HV_CHECK(expr)              \
+    do {                            \
+        int _r = (expr);            \
+        if (_r != HV_OK) return _r; \
+    } while (0)
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+enum Buf {
+    B_ANTPOS, B_INVLAM, B_EQ2TOPS, B_CRD_EQ, B_ISQRT, B_BEAM, B_CRD_TOP, B_W, B_PART,
+    B_OUT, B_AUX, B_DYN, B_COUNT
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+// A set of grow-only device buffers.  The context owns one; every serving
+// slot owns its own, so concurrent jobs never share scratch memory.
+struct Workspace {
+    DevBuf buf[B_COUNT];
+    size_t bytes() const {
+        size_t s = 0;
+        for (auto& b : buf) s += b.cap;
+        return s;
+    }
+};
+
+}  // namespace hv
+
+struct hv_ctx {
+    int device = 0;
+    size_t budget = 0;
+    size_t used = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;       // guards `ws` and calls on `stream`
+    std::mutex mem_mu;   // guards `used`
+    hv::Workspace ws;
+};
+
+struct hv_sky {
+    hv_ctx* ctx;
+    double* crd_eq;   // [3][Kpad]
+    double* isqrt;    // [Kpad]
+    int64_t nsrc, kpad;
+    double max_isqrt;
+};
+
+struct hv_beam {
+    hv_ctx* ctx;
+    void* data;       // [N][P][P] float or double
+    int nant, beam_px, f64;
+    double max_abs;
+};
+
+namespace hv {
+
+int dev_alloc(hv_ctx* c, void** p, size_t bytes);
+void dev_free(hv_ctx* c, void* p, size_t bytes);
+// Grow buffer b of ws to at least `bytes` (synchronises `stream` before
+// releasing an old buffer).  Returns whether the buffer moved in *moved.
+int ensure(hv_ctx* c, Workspace& ws, Buf b, size_t bytes, hipStream_t stream, bool* moved = nullptr);
+void release(hv_ctx* c, Workspace& ws);
+
+float fp16_scale(double max_amp);
+
+// Launch plan of one job.
+struct Plan {
+    int mode = 0, nant = 0, ntimes = 0, nfreq = 0;
+    int64_t kpad = 0;              // padded sources (buffers and grids)
+    int tile = 0, nb = 0, ntiles = 0, nsplit = 0;
+    size_t row_bytes = 0, unit_bytes = 0, part_unit = 0;
+    int fb = 1, tb = 1;            // frequencies / times per batch
+    // device bytes per workspace buffer
+    size_t crd_top_bytes = 0, w_bytes = 0, part_bytes = 0, out_bytes = 0;
+    size_t footprint() const { return crd_top_bytes + w_bytes + part_bytes + out_bytes; }
+};
+
+Plan plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t operand_budget);
+
+struct JobDev {
+    int mode;
+    int nant, ntimes, nfreq, beam_px, beam_f64;
+    int64_t kpad;
+    const double* antpos;
+    const double* inv_lambda;   // [F]
+    const double* eq2tops;
+    const double* crd_eq;       // [3][kpad]
+    const double* isqrt;        // [kpad]
+    const void* beam;
+    void* vis;                  // [F][T][N][N] complex, device
+    float amp_scale;
+    const int64_t* dyn_kpad;    // optional device-side source count
+};
+
+struct StageTimer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> ev[4][2];
+    int begin(int stage);
+    int end(int stage);
+    int collect(hv_stage_times* t);
+    ~StageTimer();
+};
+
+// Allocate the plan's buffers in ws (outside any graph capture).
+int prepare_workspace(hv_ctx* c, Workspace& ws, const Plan& p, hipStream_t s, bool* moved = nullptr);
+// Enqueue the kernels of one job on stream s (capturable: no allocation,
+// no memset, no synchronisation).
+int enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t s, StageTimer& tm);
+
+}  // namespace hv

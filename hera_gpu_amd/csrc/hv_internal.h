@@ -37,12 +37,28 @@ inline size_t operand_row_bytes(int mode, int64_t kpad) {
     return (size_t)kpad * (mode_is_f64(mode) ? 16 : 8);
 }
 
+// Every stage takes the padded source count `kpad` that sizes its buffers
+// and grid, and optionally `dyn_kpad`, a device pointer to the number of
+// sources actually present (a multiple of KALIGN, <= kpad).  Work beyond
+// *dyn_kpad is skipped, so a graph captured for a bucket of source counts
+// serves every job of the bucket with the same arithmetic as an exact-size
+// run.
+__device__ __host__ inline int64_t eff_kpad(int64_t kpad, const int64_t* dyn) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return dyn ? *dyn : kpad;
+#else
+    (void)dyn;
+    return kpad;
+#endif
+}
+
 struct CoordsArgs {
     const double* eq2tops;   // [T][9]
     const double* crd_eq;    // [3][Kpad]
     double* crd_top;         // [T][3][Kpad]
     int64_t kpad;
     int ntimes;
+    const int64_t* dyn_kpad;
 };
 
 struct PhasorArgs {
@@ -62,6 +78,7 @@ struct PhasorArgs {
     int ntime;               // times in this launch (grid z); unit = tb*nfreq + f
     float amp_scale;         // multiplies A*sqrt(I) (power of two; FP16 range)
     int in_f64;              // beam element type is double
+    const int64_t* dyn_kpad;
 };
 
 struct HerkArgs {
@@ -75,6 +92,7 @@ struct HerkArgs {
     int64_t kpad;
     int nsplit;
     int nunits;
+    const int64_t* dyn_kpad;
 };
 
 struct ReduceArgs {
@@ -91,6 +109,7 @@ struct ReduceArgs {
     int nunits;
     int tile;
     double scale;            // 1 / amp_scale^2
+    const int64_t* dyn_kpad; // splits summed = ceil(*dyn_kpad / KSPLIT)
 };
 
 struct BeamArgs {

@@ -13,31 +13,17 @@
 #include <string>
 #include <vector>
 
-#include "heravis.h"
-#include "hv_internal.h"
+#include "hv_host.h"
 
 using namespace hv;
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+int hv::fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-
-#define HV_HIP(expr)                                                                        \
-    do {                                                                                    \
-        hipError_t _e = (expr);                                                             \
-        if (_e != hipSuccess)                                                               \
-            return fail(HV_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(_e)); \
-    } while (0)
-
-#define HV_CHECK(expr)            \
-    do {                          \
-        int _r = (expr);          \
-        if (_r != HV_OK) return _r; \
-    } while (0)
 
 extern "C" int hv_version(void) { return 100; }
 extern "C" const char* hv_last_error(void) { return g_err.c_str(); }
@@ -52,74 +38,44 @@ extern "C" int hv_device_count(int* count) {
 }
 
 // --------------------------------------------------------------- workspace
-namespace {
-
-enum Buf {
-    B_ANTPOS, B_INVLAM, B_EQ2TOPS, B_CRD_EQ, B_ISQRT, B_BEAM, B_CRD_TOP, B_W, B_PART,
-    B_OUT, B_AUX, B_COUNT
-};
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
-
-}  // namespace
-
-struct hv_ctx {
-    int device = 0;
-    size_t budget = 0;
-    size_t used = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    DevBuf buf[B_COUNT];
-};
-
-struct hv_sky {
-    hv_ctx* ctx;
-    double* crd_eq;   // [3][Kpad]
-    double* isqrt;    // [Kpad]
-    int64_t nsrc, kpad;
-    double max_isqrt;
-};
-
-struct hv_beam {
-    hv_ctx* ctx;
-    void* data;       // [N][P][P] float or double
-    int nant, beam_px, f64;
-    double max_abs;
-};
-
 static int set_device(hv_ctx* c) {
     HV_HIP(hipSetDevice(c->device));
     return HV_OK;
 }
 
-static int dev_alloc(hv_ctx* c, void** p, size_t bytes) {
+int hv::dev_alloc(hv_ctx* c, void** p, size_t bytes) {
     if (bytes == 0) bytes = 256;
-    if (c->used + bytes > c->budget)
-        return fail(HV_ERR_NOMEM, "device memory request of " + std::to_string(bytes) +
-                                      " bytes exceeds the HBM budget (" + std::to_string(c->budget - c->used) +
-                                      " of " + std::to_string(c->budget) + " bytes left)");
+    {
+        std::lock_guard<std::mutex> lk(c->mem_mu);
+        if (c->used + bytes > c->budget)
+            return fail(HV_ERR_NOMEM, "device memory request of " + std::to_string(bytes) +
+                                          " bytes exceeds the HBM budget (" + std::to_string(c->budget - c->used) +
+                                          " of " + std::to_string(c->budget) + " bytes left)");
+        c->used += bytes;
+    }
     hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return fail(HV_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    c->used += bytes;
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lk(c->mem_mu);
+        c->used -= bytes;
+        *p = nullptr;
+        return fail(HV_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     return HV_OK;
 }
 
-static void dev_free(hv_ctx* c, void* p, size_t bytes) {
+void hv::dev_free(hv_ctx* c, void* p, size_t bytes) {
     if (!p) return;
     (void)hipFree(p);
+    std::lock_guard<std::mutex> lk(c->mem_mu);
     c->used -= bytes;
 }
 
-static int ensure(hv_ctx* c, Buf b, size_t bytes) {
-    DevBuf& d = c->buf[b];
+int hv::ensure(hv_ctx* c, Workspace& ws, Buf b, size_t bytes, hipStream_t stream, bool* moved) {
+    DevBuf& d = ws.buf[b];
     if (d.cap >= bytes && d.p) return HV_OK;
+    if (moved) *moved = true;
     if (d.p) {
-        HV_HIP(hipStreamSynchronize(c->stream));
+        HV_HIP(hipStreamSynchronize(stream));
         dev_free(c, d.p, d.cap);
         d.p = nullptr;
         d.cap = 0;
@@ -129,6 +85,16 @@ static int ensure(hv_ctx* c, Buf b, size_t bytes) {
     d.cap = want;
     return HV_OK;
 }
+
+void hv::release(hv_ctx* c, Workspace& ws) {
+    for (auto& d : ws.buf) {
+        dev_free(c, d.p, d.cap);
+        d.p = nullptr;
+        d.cap = 0;
+    }
+}
+
+static int ensure(hv_ctx* c, Buf b, size_t bytes) { return hv::ensure(c, c->ws, b, bytes, c->stream); }
 
 extern "C" int hv_ctx_create(int device, size_t hbm_budget, hv_ctx** out) {
     if (!out) return fail(HV_ERR_ARG, "out is NULL");
@@ -160,7 +126,7 @@ extern "C" int hv_ctx_destroy(hv_ctx* c) {
     if (!c) return HV_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (auto& d : c->buf) dev_free(c, d.p, d.cap);
+    release(c, c->ws);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HV_OK;
@@ -186,7 +152,7 @@ static int check_mode(int mode) {
 }
 
 // Power-of-two scale that puts the largest |A sqrt(I)| near 2^14 (fp16 range).
-static float fp16_scale(double max_amp) {
+float hv::fp16_scale(double max_amp) {
     if (!(max_amp > 0.0) || !std::isfinite(max_amp)) return 1.0f;
     int e = (int)std::floor(std::log2(16384.0 / max_amp));
     e = std::max(-60, std::min(60, e));
@@ -194,116 +160,109 @@ static float fp16_scale(double max_amp) {
 }
 
 // -------------------------------------------------------------- the job
-namespace {
-
-struct JobDev {
-    int mode;
-    int nant, ntimes, nfreq, beam_px, beam_f64;
-    int64_t kpad;
-    const double* antpos;
-    const double* inv_lambda;   // [F]
-    const double* eq2tops;
-    const double* crd_eq;       // [3][kpad]
-    const double* isqrt;        // [kpad]
-    const void* beam;
-    void* vis;                  // [F][T][N][N] complex, device
-    float amp_scale;
-};
-
-struct StageTimer {
-    bool on = false;
-    hipStream_t s = nullptr;
-    std::vector<hipEvent_t> ev[4][2];
-    int begin(int stage) {
-        if (!on) return HV_OK;
-        hipEvent_t e;
-        HV_HIP(hipEventCreate(&e));
-        HV_HIP(hipEventRecord(e, s));
-        ev[stage][0].push_back(e);
-        return HV_OK;
-    }
-    int end(int stage) {
-        if (!on) return HV_OK;
-        hipEvent_t e;
-        HV_HIP(hipEventCreate(&e));
-        HV_HIP(hipEventRecord(e, s));
-        ev[stage][1].push_back(e);
-        return HV_OK;
-    }
-    int collect(hv_stage_times* t) {
-        double ms[4] = {0, 0, 0, 0};
-        for (int k = 0; k < 4; ++k) {
-            for (size_t i = 0; i < ev[k][0].size(); ++i) {
-                float m = 0;
-                HV_HIP(hipEventElapsedTime(&m, ev[k][0][i], ev[k][1][i]));
-                ms[k] += m;
-            }
+int StageTimer::begin(int stage) {
+    if (!on) return HV_OK;
+    hipEvent_t e;
+    HV_HIP(hipEventCreate(&e));
+    HV_HIP(hipEventRecord(e, s));
+    ev[stage][0].push_back(e);
+    return HV_OK;
+}
+int StageTimer::end(int stage) {
+    if (!on) return HV_OK;
+    hipEvent_t e;
+    HV_HIP(hipEventCreate(&e));
+    HV_HIP(hipEventRecord(e, s));
+    ev[stage][1].push_back(e);
+    return HV_OK;
+}
+int StageTimer::collect(hv_stage_times* t) {
+    double ms[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        for (size_t i = 0; i < ev[k][0].size(); ++i) {
+            float m = 0;
+            HV_HIP(hipEventElapsedTime(&m, ev[k][0][i], ev[k][1][i]));
+            ms[k] += m;
         }
-        if (t) {
-            t->coords_ms = ms[0]; t->phasor_ms = ms[1]; t->herk_ms = ms[2]; t->reduce_ms = ms[3];
-            t->coords_n = (int64_t)ev[0][0].size(); t->phasor_n = (int64_t)ev[1][0].size();
-            t->herk_n = (int64_t)ev[2][0].size(); t->reduce_n = (int64_t)ev[3][0].size();
-        }
-        return HV_OK;
     }
-    ~StageTimer() {
-        for (auto& st : ev)
-            for (auto& v : st)
-                for (auto e : v) (void)hipEventDestroy(e);
+    if (t) {
+        t->coords_ms = ms[0]; t->phasor_ms = ms[1]; t->herk_ms = ms[2]; t->reduce_ms = ms[3];
+        t->coords_n = (int64_t)ev[0][0].size(); t->phasor_n = (int64_t)ev[1][0].size();
+        t->herk_n = (int64_t)ev[2][0].size(); t->reduce_n = (int64_t)ev[3][0].size();
     }
-};
+    return HV_OK;
+}
+StageTimer::~StageTimer() {
+    for (auto& st : ev)
+        for (auto& v : st)
+            for (auto e : v) (void)hipEventDestroy(e);
+}
 
-}  // namespace
-
-// Upper bound on operand bytes held at once (one batch of units).
+// Upper bound on operand bytes held at once by a direct (non-serving) call.
 static size_t operand_budget(hv_ctx* c) {
     return std::min((size_t)24 << 30, c->budget / 3);
 }
 
-static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
-    const int N = j.nant, T = j.ntimes, F = j.nfreq;
-    const int64_t kpad = j.kpad;
-    const int tile = herk_tile(j.mode);
-    const int nb = (N + tile - 1) / tile;
-    const int ntiles = nb * (nb + 1) / 2;
-    const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
-    const size_t row_bytes = operand_row_bytes(j.mode, kpad);
-    const size_t unit_bytes = (size_t)N * row_bytes;
-    const size_t acc_bytes = mode_is_f64(j.mode) ? 8 : 4;
-    const size_t part_unit = (size_t)nsplit * ntiles * 2 * tile * tile * acc_bytes;
-
+Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t wbudget) {
+    Plan p;
+    p.mode = mode;
+    p.nant = nant;
+    p.ntimes = ntimes;
+    p.nfreq = nfreq;
+    p.kpad = kpad;
+    p.tile = herk_tile(mode);
+    p.nb = (nant + p.tile - 1) / p.tile;
+    p.ntiles = p.nb * (p.nb + 1) / 2;
+    p.nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
+    p.row_bytes = operand_row_bytes(mode, kpad);
+    p.unit_bytes = (size_t)nant * p.row_bytes;
+    const size_t acc_bytes = mode_is_f64(mode) ? 8 : 4;
+    p.part_unit = (size_t)p.nsplit * p.ntiles * 2 * p.tile * p.tile * acc_bytes;
     // batch: all frequencies of a time if they fit, then as many times as fit
-    const size_t wb = operand_budget(c);
-    int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)F, wb / std::max<size_t>(unit_bytes, 1)));
-    int tb = 1;
-    if (fb == F) tb = (int)std::max<size_t>(1, std::min<size_t>((size_t)T, wb / (unit_bytes * F)));
-    // keep the contraction grid within ~64k workgroups per launch
-    while (tb > 1 && (int64_t)tb * fb * nsplit * ntiles > 65536) --tb;
-    const int units = fb * tb;
+    p.fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nfreq, wbudget / std::max<size_t>(p.unit_bytes, 1)));
+    p.tb = 1;
+    if (p.fb == nfreq)
+        p.tb = (int)std::max<size_t>(1, std::min<size_t>((size_t)ntimes, wbudget / (p.unit_bytes * nfreq)));
+    // keep one contraction launch within ~64k workgroups
+    while (p.tb > 1 && (int64_t)p.tb * p.fb * p.nsplit * p.ntiles > 65536) --p.tb;
+    const int units = p.fb * p.tb;
+    p.crd_top_bytes = (size_t)ntimes * 3 * kpad * sizeof(double);
+    p.w_bytes = p.unit_bytes * units;
+    p.part_bytes = p.part_unit * units;
+    p.out_bytes = (size_t)nfreq * ntimes * nant * nant * (mode_is_f64(mode) ? 16 : 8);
+    return p;
+}
 
-    HV_CHECK(ensure(c, B_CRD_TOP, (size_t)T * 3 * kpad * sizeof(double)));
-    HV_CHECK(ensure(c, B_W, unit_bytes * units));
-    HV_CHECK(ensure(c, B_PART, part_unit * units));
-    double* crd_top = static_cast<double*>(c->buf[B_CRD_TOP].p);
+int hv::prepare_workspace(hv_ctx* c, Workspace& ws, const Plan& p, hipStream_t s, bool* moved) {
+    HV_CHECK(ensure(c, ws, B_CRD_TOP, p.crd_top_bytes, s, moved));
+    HV_CHECK(ensure(c, ws, B_W, p.w_bytes, s, moved));
+    HV_CHECK(ensure(c, ws, B_PART, p.part_bytes, s, moved));
+    return HV_OK;
+}
 
-    CoordsArgs ca{j.eq2tops, j.crd_eq, crd_top, kpad, T};
+int hv::enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t stream, StageTimer& tm) {
+    const int T = j.ntimes, F = j.nfreq, N = j.nant;
+    const int64_t kpad = p.kpad;
+    double* crd_top = static_cast<double*>(ws.buf[B_CRD_TOP].p);
+
+    CoordsArgs ca{j.eq2tops, j.crd_eq, crd_top, kpad, T, j.dyn_kpad};
     HV_CHECK(tm.begin(0));
-    HV_HIP(launch_coords(ca, c->stream));
+    HV_HIP(launch_coords(ca, stream));
     HV_CHECK(tm.end(0));
 
-    for (int t0 = 0; t0 < T; t0 += tb) {
-        const int nt = std::min(tb, T - t0);
-        for (int f0 = 0; f0 < F; f0 += fb) {
-            const int nf = std::min(fb, F - f0);
+    for (int t0 = 0; t0 < T; t0 += p.tb) {
+        const int nt = std::min(p.tb, T - t0);
+        for (int f0 = 0; f0 < F; f0 += p.fb) {
+            const int nf = std::min(p.fb, F - f0);
             PhasorArgs pa{};
             pa.crd_top = crd_top + (size_t)t0 * 3 * kpad;
             pa.isqrt = j.isqrt;
             pa.beam = j.beam;
             pa.antpos = j.antpos;
             pa.inv_lambda = j.inv_lambda + f0;
-            pa.W = c->buf[B_W].p;
-            pa.unit_stride = unit_bytes;
-            pa.row_stride = row_bytes;
+            pa.W = ws.buf[B_W].p;
+            pa.unit_stride = p.unit_bytes;
+            pa.row_stride = p.row_bytes;
             pa.kpad = kpad;
             pa.nant = N;
             pa.beam_px = j.beam_px;
@@ -311,45 +270,54 @@ static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
             pa.ntime = nt;
             pa.amp_scale = j.amp_scale;
             pa.in_f64 = j.beam_f64;
+            pa.dyn_kpad = j.dyn_kpad;
             HV_CHECK(tm.begin(1));
-            HV_HIP(launch_phasor(j.mode, pa, c->stream));
+            HV_HIP(launch_phasor(j.mode, pa, stream));
             HV_CHECK(tm.end(1));
 
             HerkArgs ha{};
-            ha.W = c->buf[B_W].p;
-            ha.unit_stride = unit_bytes;
-            ha.row_stride = row_bytes;
-            ha.partials = c->buf[B_PART].p;
+            ha.W = ws.buf[B_W].p;
+            ha.unit_stride = p.unit_bytes;
+            ha.row_stride = p.row_bytes;
+            ha.partials = ws.buf[B_PART].p;
             ha.nant = N;
-            ha.nb = nb;
-            ha.ntiles = ntiles;
+            ha.nb = p.nb;
+            ha.ntiles = p.ntiles;
             ha.kpad = kpad;
-            ha.nsplit = nsplit;
+            ha.nsplit = p.nsplit;
             ha.nunits = nt * nf;
+            ha.dyn_kpad = j.dyn_kpad;
             HV_CHECK(tm.begin(2));
-            HV_HIP(launch_herk(j.mode, ha, c->stream));
+            HV_HIP(launch_herk(j.mode, ha, stream));
             HV_CHECK(tm.end(2));
 
             ReduceArgs ra{};
-            ra.partials = c->buf[B_PART].p;
+            ra.partials = ws.buf[B_PART].p;
             ra.vis = j.vis;
             ra.t0 = t0;
             ra.f0 = f0;
             ra.nfreq_launch = nf;
             ra.ntimes_total = T;
             ra.nant = N;
-            ra.nb = nb;
-            ra.ntiles = ntiles;
-            ra.nsplit = nsplit;
+            ra.nb = p.nb;
+            ra.ntiles = p.ntiles;
+            ra.nsplit = p.nsplit;
             ra.nunits = nt * nf;
-            ra.tile = tile;
+            ra.tile = p.tile;
             ra.scale = 1.0 / ((double)j.amp_scale * (double)j.amp_scale);
+            ra.dyn_kpad = j.dyn_kpad;
             HV_CHECK(tm.begin(3));
-            HV_HIP(launch_reduce(j.mode, ra, c->stream));
+            HV_HIP(launch_reduce(j.mode, ra, stream));
             HV_CHECK(tm.end(3));
         }
     }
     return HV_OK;
+}
+
+static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
+    Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c));
+    HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
+    return enqueue_job(j, p, c->ws, c->stream, tm);
 }
 
 // ---------------------------------------------------------- argument checks
@@ -372,7 +340,7 @@ static void inv_lambdas(const double* freqs, int nfreq, std::vector<double>& out
 
 static int upload(hv_ctx* c, Buf b, const void* host, size_t bytes) {
     HV_CHECK(ensure(c, b, bytes));
-    HV_HIP(hipMemcpyAsync(c->buf[b].p, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HV_HIP(hipMemcpyAsync(c->ws.buf[b].p, host, bytes, hipMemcpyHostToDevice, c->stream));
     return HV_OK;
 }
 
@@ -435,13 +403,13 @@ extern "C" int hv_vis(hv_ctx* c, int mode, int dtype, const void* antpos, int na
     j.mode = mode;
     j.nant = nant; j.ntimes = ntimes; j.nfreq = nfreq; j.beam_px = beam_px; j.beam_f64 = dtype == HV_F64;
     j.kpad = kpad;
-    j.antpos = (const double*)c->buf[B_ANTPOS].p;
-    j.inv_lambda = (const double*)c->buf[B_INVLAM].p;
-    j.eq2tops = (const double*)c->buf[B_EQ2TOPS].p;
-    j.crd_eq = (const double*)c->buf[B_CRD_EQ].p;
-    j.isqrt = (const double*)c->buf[B_ISQRT].p;
-    j.beam = c->buf[B_BEAM].p;
-    j.vis = c->buf[B_OUT].p;
+    j.antpos = (const double*)c->ws.buf[B_ANTPOS].p;
+    j.inv_lambda = (const double*)c->ws.buf[B_INVLAM].p;
+    j.eq2tops = (const double*)c->ws.buf[B_EQ2TOPS].p;
+    j.crd_eq = (const double*)c->ws.buf[B_CRD_EQ].p;
+    j.isqrt = (const double*)c->ws.buf[B_ISQRT].p;
+    j.beam = c->ws.buf[B_BEAM].p;
+    j.vis = c->ws.buf[B_OUT].p;
     j.amp_scale = (mode == HV_MODE_SPLIT_FP16)
                       ? fp16_scale(max_abs(bm_cube, dtype, beam_bytes / (dtype == HV_F64 ? 8 : 4)) *
                                    max_abs(isq.data(), HV_F64, isq.size()))
@@ -549,8 +517,8 @@ extern "C" int hv_simulate(hv_ctx* c, int mode, const hv_sky* sky, int64_t nsrc_
     if (nsrc_used != sky->nsrc) {
         HV_CHECK(ensure(c, B_CRD_EQ, (size_t)3 * kpad * 8));
         HV_CHECK(ensure(c, B_ISQRT, (size_t)kpad * 8));
-        double* dc = (double*)c->buf[B_CRD_EQ].p;
-        double* di = (double*)c->buf[B_ISQRT].p;
+        double* dc = (double*)c->ws.buf[B_CRD_EQ].p;
+        double* di = (double*)c->ws.buf[B_ISQRT].p;
         HV_HIP(hipMemsetAsync(dc, 0, (size_t)3 * kpad * 8, c->stream));
         HV_HIP(hipMemsetAsync(di, 0, (size_t)kpad * 8, c->stream));
         for (int d = 0; d < 3; ++d)
@@ -575,13 +543,13 @@ extern "C" int hv_simulate(hv_ctx* c, int mode, const hv_sky* sky, int64_t nsrc_
     j.mode = mode;
     j.nant = nant; j.ntimes = ntimes; j.nfreq = nfreq; j.beam_px = beam->beam_px; j.beam_f64 = beam->f64;
     j.kpad = kpad;
-    j.antpos = (const double*)c->buf[B_ANTPOS].p;
-    j.inv_lambda = (const double*)c->buf[B_INVLAM].p;
-    j.eq2tops = (const double*)c->buf[B_EQ2TOPS].p;
+    j.antpos = (const double*)c->ws.buf[B_ANTPOS].p;
+    j.inv_lambda = (const double*)c->ws.buf[B_INVLAM].p;
+    j.eq2tops = (const double*)c->ws.buf[B_EQ2TOPS].p;
     j.crd_eq = crd_eq;
     j.isqrt = isq;
     j.beam = beam->data;
-    j.vis = c->buf[B_OUT].p;
+    j.vis = c->ws.buf[B_OUT].p;
     j.amp_scale = (mode == HV_MODE_SPLIT_FP16) ? fp16_scale(beam->max_abs * sky->max_isqrt) : 1.0f;
 
     // Warm allocation outside the timed loop.
@@ -625,11 +593,11 @@ extern "C" int hv_k_rotate(hv_ctx* c, int dtype, const void* eq2top, const void*
     HV_CHECK(upload(c, B_EQ2TOPS, eq.data(), 72));
     HV_CHECK(upload(c, B_CRD_EQ, crd.data(), crd.size() * 8));
     HV_CHECK(ensure(c, B_CRD_TOP, crd.size() * 8));
-    CoordsArgs a{(const double*)c->buf[B_EQ2TOPS].p, (const double*)c->buf[B_CRD_EQ].p,
-                 (double*)c->buf[B_CRD_TOP].p, nsrc, 1};
+    CoordsArgs a{(const double*)c->ws.buf[B_EQ2TOPS].p, (const double*)c->ws.buf[B_CRD_EQ].p,
+                 (double*)c->ws.buf[B_CRD_TOP].p, nsrc, 1};
     HV_HIP(launch_coords(a, c->stream));
     std::vector<double> out(crd.size());
-    HV_HIP(hipMemcpyAsync(out.data(), c->buf[B_CRD_TOP].p, out.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipMemcpyAsync(out.data(), c->ws.buf[B_CRD_TOP].p, out.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HV_HIP(hipStreamSynchronize(c->stream));
     if (dtype == HV_F64) std::memcpy(crd_top_out, out.data(), out.size() * 8);
     else {
@@ -650,9 +618,9 @@ extern "C" int hv_k_beam(hv_ctx* c, int dtype, const void* bm_cube, int nant, in
     HV_CHECK(upload(c, B_BEAM, bm_cube, (size_t)nant * beam_px * beam_px * es));
     HV_CHECK(upload(c, B_CRD_TOP, crd_top, (size_t)3 * nsrc * es));
     HV_CHECK(ensure(c, B_AUX, (size_t)nant * nsrc * es));
-    BeamArgs a{c->buf[B_BEAM].p, c->buf[B_CRD_TOP].p, c->buf[B_AUX].p, nsrc, nant, beam_px, dtype == HV_F64};
+    BeamArgs a{c->ws.buf[B_BEAM].p, c->ws.buf[B_CRD_TOP].p, c->ws.buf[B_AUX].p, nsrc, nant, beam_px, dtype == HV_F64};
     HV_HIP(launch_beam(a, c->stream));
-    HV_HIP(hipMemcpyAsync(A_out, c->buf[B_AUX].p, (size_t)nant * nsrc * es, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipMemcpyAsync(A_out, c->ws.buf[B_AUX].p, (size_t)nant * nsrc * es, hipMemcpyDeviceToHost, c->stream));
     HV_HIP(hipStreamSynchronize(c->stream));
     return HV_OK;
 }
@@ -721,12 +689,12 @@ extern "C" int hv_k_phasor(hv_ctx* c, int mode, int dtype, const void* bm_cube, 
                                          max_abs(isq.data(), HV_F64, isq.size()))
                             : 1.0f;
     PhasorArgs pa{};
-    pa.crd_top = (const double*)c->buf[B_CRD_TOP].p;
-    pa.isqrt = (const double*)c->buf[B_ISQRT].p;
-    pa.beam = c->buf[B_BEAM].p;
-    pa.antpos = (const double*)c->buf[B_ANTPOS].p;
-    pa.inv_lambda = (const double*)c->buf[B_INVLAM].p;
-    pa.W = c->buf[B_W].p;
+    pa.crd_top = (const double*)c->ws.buf[B_CRD_TOP].p;
+    pa.isqrt = (const double*)c->ws.buf[B_ISQRT].p;
+    pa.beam = c->ws.buf[B_BEAM].p;
+    pa.antpos = (const double*)c->ws.buf[B_ANTPOS].p;
+    pa.inv_lambda = (const double*)c->ws.buf[B_INVLAM].p;
+    pa.W = c->ws.buf[B_W].p;
     pa.unit_stride = row_bytes * nant;
     pa.row_stride = row_bytes;
     pa.kpad = kpad;
@@ -738,7 +706,7 @@ extern "C" int hv_k_phasor(hv_ctx* c, int mode, int dtype, const void* bm_cube, 
     pa.in_f64 = dtype == HV_F64;
     HV_HIP(launch_phasor(mode, pa, c->stream));
     std::vector<uint8_t> W(row_bytes * nant);
-    HV_HIP(hipMemcpyAsync(W.data(), c->buf[B_W].p, W.size(), hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipMemcpyAsync(W.data(), c->ws.buf[B_W].p, W.size(), hipMemcpyDeviceToHost, c->stream));
     HV_HIP(hipStreamSynchronize(c->stream));
     if (mode == HV_MODE_FP64) decode_rows<double>(mode, W, row_bytes, nant, nsrc, 1.0 / scale, (double*)v_out);
     else decode_rows<float>(mode, W, row_bytes, nant, nsrc, 1.0 / scale, (float*)v_out);
@@ -764,7 +732,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
         for (size_t i = 0; i < (size_t)nant * nsrc * 2; ++i) m = std::max(m, (double)std::fabs(f[i]));
         scale = fp16_scale(m);
     }
-    SplitArgs sa{c->buf[B_AUX].p, c->buf[B_W].p, row_bytes, nsrc, kpad, nant, scale};
+    SplitArgs sa{c->ws.buf[B_AUX].p, c->ws.buf[B_W].p, row_bytes, nsrc, kpad, nant, scale};
     HV_HIP(launch_split(mode, sa, c->stream));
     const int tile = herk_tile(mode);
     const int nb = (nant + tile - 1) / tile;
@@ -773,10 +741,10 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     HV_CHECK(ensure(c, B_PART, (size_t)nsplit * ntiles * 2 * tile * tile * (f64 ? 8 : 4)));
     HV_CHECK(ensure(c, B_OUT, (size_t)nant * nant * cplx));
     HerkArgs ha{};
-    ha.W = c->buf[B_W].p;
+    ha.W = c->ws.buf[B_W].p;
     ha.unit_stride = row_bytes * nant;
     ha.row_stride = row_bytes;
-    ha.partials = c->buf[B_PART].p;
+    ha.partials = c->ws.buf[B_PART].p;
     ha.nant = nant;
     ha.nb = nb;
     ha.ntiles = ntiles;
@@ -785,8 +753,8 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     ha.nunits = 1;
     HV_HIP(launch_herk(mode, ha, c->stream));
     ReduceArgs ra{};
-    ra.partials = c->buf[B_PART].p;
-    ra.vis = c->buf[B_OUT].p;
+    ra.partials = c->ws.buf[B_PART].p;
+    ra.vis = c->ws.buf[B_OUT].p;
     ra.t0 = 0;
     ra.f0 = 0;
     ra.nfreq_launch = 1;
@@ -799,7 +767,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     ra.tile = tile;
     ra.scale = 1.0 / ((double)scale * scale);
     HV_HIP(launch_reduce(mode, ra, c->stream));
-    HV_HIP(hipMemcpyAsync(vis_out, c->buf[B_OUT].p, (size_t)nant * nant * cplx, hipMemcpyDeviceToHost,
+    HV_HIP(hipMemcpyAsync(vis_out, c->ws.buf[B_OUT].p, (size_t)nant * nant * cplx, hipMemcpyDeviceToHost,
                           c->stream));
     HV_HIP(hipStreamSynchronize(c->stream));
     return HV_OK;

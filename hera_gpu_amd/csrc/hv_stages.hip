@@ -116,7 +116,7 @@ __device__ inline void store_phasors8(char* dst, const R* re, const R* im) {
 __global__ __launch_bounds__(256) void coords_kernel(CoordsArgs a) {
     const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int t = blockIdx.y;
-    if (s >= a.kpad) return;
+    if (s >= eff_kpad(a.kpad, a.dyn_kpad)) return;
     const double* R = a.eq2tops + 9 * t;
     const double x = a.crd_eq[s], y = a.crd_eq[a.kpad + s], z = a.crd_eq[2 * a.kpad + s];
     double* o = a.crd_top + (size_t)t * 3 * a.kpad;
@@ -145,6 +145,9 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     using R = typename std::conditional<MODE == MODE_FP64, double, float>::type;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     BT* sb = reinterpret_cast<BT*>(smem);
+    const int64_t kact = eff_kpad(a.kpad, a.dyn_kpad);
+    const int64_t wg0 = (int64_t)blockIdx.x * PH_SRC_PER_WG;
+    if (wg0 >= kact) return;                       // uniform per workgroup
     const int ant = blockIdx.y;
     const int tb = blockIdx.z;
     const int P = a.beam_px;
@@ -157,8 +160,7 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     const double* cy = cx + a.kpad;
     const double* cz = cx + 2 * a.kpad;
     const double* isq = a.isqrt;
-    const int64_t wg0 = (int64_t)blockIdx.x * PH_SRC_PER_WG;
-    const int64_t wg1 = min(wg0 + (int64_t)PH_SRC_PER_WG, a.kpad);
+    const int64_t wg1 = min(wg0 + (int64_t)PH_SRC_PER_WG, kact);
     char* wrow = reinterpret_cast<char*>(a.W) + (size_t)tb * a.nfreq * a.unit_stride +
                  (size_t)ant * a.row_stride;
     const size_t elem = (MODE == MODE_FP64) ? 16 : 8;
@@ -306,8 +308,9 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
     const size_t off = (size_t)(row % T) * T + (col % T);
     const ACC* p = reinterpret_cast<const ACC*>(r.partials) + ((size_t)u * r.nsplit * r.ntiles + tid) * 2 * tsz + off;
     const size_t split_stride = (size_t)r.ntiles * 2 * tsz;
+    const int nsplit = (int)((eff_kpad((int64_t)r.nsplit * KSPLIT, r.dyn_kpad) + KSPLIT - 1) / KSPLIT);
     double re = 0.0, im = 0.0;
-    for (int s = 0; s < r.nsplit; ++s) {
+    for (int s = 0; s < nsplit; ++s) {
         re += (double)p[(size_t)s * split_stride];
         im += (double)p[(size_t)s * split_stride + tsz];
     }

@@ -148,6 +148,66 @@ int hv_simulate(hv_ctx* ctx, int mode, const hv_sky* sky, int64_t nsrc_used,
                 const double* freqs, int nfreq, const void* eq2tops, int ntimes,
                 void* vis_out, int reps, hv_stage_times* times);
 
+/* ---------------------------------------------------------------------------
+ * Serving (no reference counterpart: the reference has no request loop).
+ * A server admits queued jobs into slots as slots and device memory free up;
+ * admitted jobs run concurrently on per-slot streams, replaying a hipGraph
+ * captured once per slot and shape bucket.  Results are bitwise identical
+ * to hv_simulate on the same inputs, whatever else is running.
+ * ------------------------------------------------------------------------- */
+typedef struct hv_server hv_server;
+
+#define HV_JOB_FAIL_TEST 1   /* flags: fail this job at admission (host-side test hook) */
+
+typedef struct hv_job {
+    int mode;                  /* HV_MODE_*                                        */
+    int flags;                 /* 0 or HV_JOB_FAIL_TEST                            */
+    const hv_sky* sky;         /* resident catalog; the first nsrc sources are used */
+    int64_t nsrc;
+    const hv_beam* beam;       /* resident beam, nant x P x P                      */
+    const double* antpos;      /* host (nant,3), copied at submit                  */
+    int nant;
+    const double* freqs;       /* host (nfreq,) Hz, copied at submit               */
+    int nfreq;
+    const double* eq2tops;     /* host (ntimes,3,3), copied at submit              */
+    int ntimes;
+    void* vis_out;             /* host (F,T,N,N) complex64 (complex128 for FP64),
+                                  written before the job is reported finished      */
+} hv_job;
+
+typedef struct hv_job_info {
+    int64_t id;
+    int status;                /* HV_OK or the HV_ERR_* code of the failure        */
+    int slot;
+    int graph_hit;             /* 1 if a previously captured graph was replayed    */
+    int reserved;
+    double t_submit, t_admit, t_done;   /* seconds on the CLOCK_MONOTONIC clock   */
+    int64_t footprint;         /* device bytes of the slot the job needed          */
+    double flops;              /* useful Hermitian flops 4 N (N+1) K T F           */
+    char error[160];
+} hv_job_info;
+
+typedef struct hv_server_stats_t {
+    int64_t submitted, completed, failed;
+    int64_t graph_captures, graph_launches, direct_launches;
+    int64_t peak_running, pending, running;
+    size_t workspace_bytes, budget_bytes;
+} hv_server_stats_t;
+
+/* nslots in [1,16]; budget = device bytes for slot workspaces (0 = half of
+ * what the context has left); use_graphs = 0 launches kernels directly. */
+int hv_server_create(hv_ctx* ctx, int nslots, size_t budget, int use_graphs, hv_server** out);
+/* Queue a job (non-blocking).  Shape errors return HV_ERR_ARG; a job that can
+ * never fit the budget returns HV_ERR_NOMEM. */
+int hv_server_submit(hv_server* srv, const hv_job* job, int64_t* id);
+/* Collect up to max finished jobs (each is reported once). */
+int hv_server_poll(hv_server* srv, hv_job_info* infos, int max, int* n);
+/* Block until job id finishes (timeout_s < 0: no limit); HV_ERR_BUSY on timeout. */
+int hv_server_wait(hv_server* srv, int64_t id, double timeout_s, hv_job_info* info);
+int hv_server_stats(hv_server* srv, hv_server_stats_t* out);
+/* Fail queued jobs, wait for running ones, release everything. */
+int hv_server_destroy(hv_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

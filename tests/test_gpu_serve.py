@@ -1,0 +1,111 @@
+"""Serving loop on the GPU: continuous batching, graphs per shape bucket,
+footprint admission, failure isolation, and results identical alone and in a
+mix."""
+import numpy as np
+import pytest
+
+from hera_gpu_amd import configs, stages
+from hera_gpu_amd.serve import FAIL_TEST, Server
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def catalogs():
+    out = {}
+    for lay in ("hex37", "disk128"):
+        antpos, beam, crd, flux = configs.serving_catalog(lay, nsrc_max=40000)
+        out[lay] = (antpos, stages.Beam(beam), stages.Sky(crd, flux))
+    yield out
+    for _, b, s in out.values():
+        b.close()
+        s.close()
+
+
+def mixed_jobs(n, seed=0):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        lay = ("hex37", "disk128")[int(rng.integers(0, 2))]
+        nsrc = int(rng.integers(1, 40000))
+        t = int(rng.choice([1, 2, 3]))
+        f = int(rng.choice([1, 2, 5]))
+        gemm = ("split_bf16", "fp32", "split_fp16")[int(rng.integers(0, 3))]
+        jobs.append((i, lay, nsrc, t, f, gemm))
+    return jobs
+
+
+def job_inputs(cat, i, lay, t, f):
+    antpos, beam, sky = cat[lay]
+    eq = configs.eq2tops_lst(t, lst0_deg=13.0 * i).astype(np.float64)
+    freqs = np.linspace(100e6, 200e6, f) if f > 1 else np.array([150e6 + i])
+    return antpos, beam, sky, eq, freqs
+
+
+def test_mixed_results_equal_alone_bitwise(catalogs):
+    jobs = mixed_jobs(24)
+    with Server(nslots=4) as srv:
+        ids = {}
+        for (i, lay, nsrc, t, f, gemm) in jobs:
+            antpos, beam, sky, eq, freqs = job_inputs(catalogs, i, lay, t, f)
+            ids[srv.submit(sky, beam, antpos, freqs, eq, nsrc=nsrc, gemm=gemm)] = (i, lay, nsrc, t, f, gemm)
+        done = {}
+        for jid in ids:
+            d = srv.wait(jid, timeout_s=120)
+            assert d.status == 0, d.error
+            done[jid] = d
+        st = srv.stats()
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    assert st["peak_running"] >= 2, "jobs never overlapped on the device"
+    for jid, (i, lay, nsrc, t, f, gemm) in ids.items():
+        antpos, beam, sky, eq, freqs = job_inputs(catalogs, i, lay, t, f)
+        alone, _ = stages.simulate(sky, beam, antpos, freqs, eq, gemm=gemm, nsrc=nsrc)
+        np.testing.assert_array_equal(done[jid].vis, alone, err_msg=f"job {i} {lay} {nsrc} {gemm}")
+
+
+def test_graph_replay_same_bucket(catalogs):
+    antpos, beam, sky, eq, freqs = job_inputs(catalogs, 3, "disk128", 2, 2)
+    with Server(nslots=1) as srv:
+        a = srv.wait(srv.submit(sky, beam, antpos, freqs, eq, nsrc=20000), 120)
+        # same bucket (16384 < Kpad <= 32768), different source count and times
+        eq2 = configs.eq2tops_lst(2, lst0_deg=77.0).astype(np.float64)
+        b = srv.wait(srv.submit(sky, beam, antpos, freqs, eq2, nsrc=31000), 120)
+        st = srv.stats()
+    assert a.status == 0 and b.status == 0
+    assert not a.graph_hit and b.graph_hit
+    assert st["graph_captures"] == 1 and st["graph_launches"] == 2
+    alone, _ = stages.simulate(sky, beam, antpos, freqs, eq2, nsrc=31000, gemm="split_bf16")
+    np.testing.assert_array_equal(b.vis, alone)
+
+
+def test_admission_by_footprint(catalogs):
+    antpos, beam, sky, eq, freqs = job_inputs(catalogs, 5, "disk128", 3, 5)
+    probe = Server(nslots=4)
+    jid = probe.submit(sky, beam, antpos, freqs, eq, nsrc=40000)
+    need = probe.wait(jid, 120).footprint
+    probe.close()
+    # a budget that fits one such job but not two: four slots, one job at a time
+    with Server(nslots=4, budget_bytes=int(need * 1.5)) as srv:
+        ids = [srv.submit(sky, beam, antpos, freqs, eq, nsrc=40000 - k) for k in range(5)]
+        res = [srv.wait(i, 120) for i in ids]
+        st = srv.stats()
+        assert all(r.status == 0 for r in res)
+        assert st["peak_running"] == 1
+        # a job that can never fit is refused at submission
+        big_ap, big_beam, big_sky = catalogs["disk128"]
+        with pytest.raises(MemoryError):
+            srv.submit(big_sky, big_beam, big_ap, np.linspace(1e8, 2e8, 64), configs.eq2tops_lst(32), nsrc=40000)
+
+
+def test_failed_job_is_isolated(catalogs):
+    antpos, beam, sky, eq, freqs = job_inputs(catalogs, 7, "hex37", 1, 1)
+    with Server(nslots=2) as srv:
+        ok1 = srv.submit(sky, beam, antpos, freqs, eq, nsrc=5000)
+        bad = srv.submit(sky, beam, antpos, freqs, eq, nsrc=5000, flags=FAIL_TEST)
+        ok2 = srv.submit(sky, beam, antpos, freqs, eq, nsrc=6000)
+        r = {i: srv.wait(i, 60) for i in (ok1, bad, ok2)}
+    assert r[ok1].status == 0 and r[ok2].status == 0
+    assert r[bad].status != 0 and "injected" in r[bad].error
+    with Server(nslots=1) as srv:
+        with pytest.raises(AssertionError):
+            srv.submit(sky, beam, antpos, freqs, eq, nsrc=10 ** 7)
