@@ -85,6 +85,8 @@ SIGNATURES = {
     "hv_beam_free": (_i, [_vp]),
     "hv_simulate": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i,
                          C.POINTER(StageTimes)]),
+    "hv_bench_copy": (_i, [_vp, _sz, _i, C.POINTER(_d)]),
+    "hv_bench_mfma": (_i, [_vp, _i, _i, C.POINTER(_d)]),
     "hv_server_create": (_i, [_vp, _i, _sz, _i, C.POINTER(_vp)]),
     "hv_server_submit": (_i, [_vp, C.POINTER(Job), C.POINTER(_i64)]),
     "hv_server_poll": (_i, [_vp, C.POINTER(JobInfo), _i, C.POINTER(_i)]),

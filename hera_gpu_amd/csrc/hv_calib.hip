@@ -1,0 +1,147 @@
+// Calibration microbenchmarks (no reference counterpart): a plain device copy
+// of known size (HBM bandwidth, and the yardstick that scales rocprofv3
+// FETCH_SIZE / WRITE_SIZE counters to bytes), and register-resident MFMA
+// loops that measure the matrix-core peak of each arithmetic used by the
+// contraction.
+#include "hv_host.h"
+
+using namespace hv;
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// Each wave runs `iters` rounds of 8 independent MFMA chains and writes one
+// value per lane so the work is not removed.
+template <int MODE>
+__global__ __launch_bounds__(256) void mfma_kernel(float* out, int iters, float seed) {
+    const int lane = threadIdx.x & 63;
+    float r = 0.f;
+    if constexpr (MODE == MODE_FP32) {
+        f32x16 acc[4] = {};
+        float a = seed + lane, b = seed - lane;
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[k], 0, 0, 0);
+        for (int k = 0; k < 4; ++k) r += acc[k][0] + acc[k][15];
+    } else if constexpr (MODE == MODE_SPLIT_BF16) {
+        f32x16 acc[4] = {};
+        bf16x8 a, b;
+        for (int e = 0; e < 8; ++e) { a[e] = (__bf16)(seed + e); b[e] = (__bf16)(seed - lane); }
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[k], 0, 0, 0);
+        for (int k = 0; k < 4; ++k) r += acc[k][0] + acc[k][15];
+    } else if constexpr (MODE == MODE_SPLIT_FP16) {
+        f32x16 acc[4] = {};
+        f16x8 a, b;
+        for (int e = 0; e < 8; ++e) { a[e] = (_Float16)(seed + e); b[e] = (_Float16)(seed - lane); }
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[k], 0, 0, 0);
+        for (int k = 0; k < 4; ++k) r += acc[k][0] + acc[k][15];
+    } else {
+        f64x4 acc[8] = {};
+        double a = seed + lane, b = seed - lane;
+        for (int it = 0; it < iters; ++it)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[k], 0, 0, 0);
+        for (int k = 0; k < 8; ++k) r += (float)(acc[k][0] + acc[k][3]);
+    }
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = r;
+}
+
+// flops of one kernel-loop iteration per wave
+double mfma_flops_per_iter(int mode) {
+    switch (mode) {
+        case MODE_FP32: return 4.0 * 32 * 32 * 2 * 2;
+        case MODE_SPLIT_BF16:
+        case MODE_SPLIT_FP16: return 4.0 * 32 * 32 * 16 * 2;
+        default: return 8.0 * 16 * 16 * 4 * 2;
+    }
+}
+
+}  // namespace
+
+extern "C" int hv_bench_copy(hv_ctx* c, size_t bytes, int reps, double* ms_per_copy) {
+    if (!c || !ms_per_copy || reps < 1 || bytes < 16) return fail(HV_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_HIP(hipSetDevice(c->device));
+    bytes = bytes / 16 * 16;
+    void *a = nullptr, *b = nullptr;
+    HV_CHECK(dev_alloc(c, &a, bytes));
+    int rc = dev_alloc(c, &b, bytes);
+    if (rc != HV_OK) {
+        dev_free(c, a, bytes);
+        return rc;
+    }
+    const size_t n = bytes / 16;
+    const int grid = 256 * 32;
+    hipEvent_t e0, e1;
+    rc = HV_OK;
+    if (hipMemsetAsync(a, 1, bytes, c->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        rc = fail(HV_ERR_HIP, "setup failed");
+    if (rc == HV_OK) {
+        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, c->stream, (const uint4*)a, (uint4*)b, n);
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, c->stream, (const uint4*)a, (uint4*)b, n);
+        (void)hipEventRecord(e1, c->stream);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(HV_ERR_HIP, "copy timing failed");
+        *ms_per_copy = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    dev_free(c, a, bytes);
+    dev_free(c, b, bytes);
+    return rc;
+}
+
+extern "C" int hv_bench_mfma(hv_ctx* c, int mode, int iters, double* tflops) {
+    if (!c || !tflops || iters < 1 || mode < HV_MODE_FP32 || mode > HV_MODE_FP64)
+        return fail(HV_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_HIP(hipSetDevice(c->device));
+    int dev = 0, cus = 0;
+    HV_HIP(hipGetDevice(&dev));
+    HV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = cus * 8;   // 8 workgroups of 4 waves per CU
+    void* out = nullptr;
+    HV_CHECK(dev_alloc(c, &out, (size_t)grid * 256 * 4));
+    hipEvent_t e0, e1;
+    HV_HIP(hipEventCreate(&e0));
+    HV_HIP(hipEventCreate(&e1));
+    auto launch = [&](int it) {
+        switch (mode) {
+            case MODE_FP32: hipLaunchKernelGGL(mfma_kernel<MODE_FP32>, dim3(grid), dim3(256), 0, c->stream, (float*)out, it, 1.0f); break;
+            case MODE_SPLIT_BF16: hipLaunchKernelGGL(mfma_kernel<MODE_SPLIT_BF16>, dim3(grid), dim3(256), 0, c->stream, (float*)out, it, 1.0f); break;
+            case MODE_SPLIT_FP16: hipLaunchKernelGGL(mfma_kernel<MODE_SPLIT_FP16>, dim3(grid), dim3(256), 0, c->stream, (float*)out, it, 1.0f); break;
+            default: hipLaunchKernelGGL(mfma_kernel<MODE_FP64>, dim3(grid), dim3(256), 0, c->stream, (float*)out, it, 1.0f); break;
+        }
+    };
+    launch(16);
+    (void)hipEventRecord(e0, c->stream);
+    launch(iters);
+    (void)hipEventRecord(e1, c->stream);
+    float ms = 0;
+    int rc = HV_OK;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        rc = fail(HV_ERR_HIP, "mfma timing failed");
+    const double waves = (double)grid * 4;
+    *tflops = waves * iters * mfma_flops_per_iter(mode) / (ms * 1e-3) / 1e12;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(c, out, (size_t)grid * 256 * 4);
+    return rc;
+}

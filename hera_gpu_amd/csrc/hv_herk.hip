@@ -175,6 +175,13 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
                 tre[i][j][e] = 0; tim[i][j][e] = 0;
             }
 
+    // A wave whose 32x32 (complex) region lies strictly below the diagonal of
+    // a diagonal tile, or wholly in the antenna padding, skips its MFMAs and
+    // its partial write (the reduce stage reads the mirrored entry instead);
+    // it still helps stage operands.  Co-resident workgroups use the idle
+    // SIMD time.
+    const bool active = (bi * TILE + wr * WT < a.nant) && (bj * TILE + wc * WT < a.nant) && !(diag && wr > wc);
+
     // lane -> (row within block, k slot group)
     const int lrow = F64 ? (lane & 15) : (lane & 31);
     const int lh = F64 ? (lane >> 4) : (lane >> 5);
@@ -189,7 +196,9 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
         const char* As = lds[buf][0];
         const char* Bs = diag ? lds[buf][0] : lds[buf][1];
 
-        if constexpr (MODE == MODE_FP32) {
+        if (!active) {
+            // nothing to compute for this wave
+        } else if constexpr (MODE == MODE_FP32) {
             // lane half lh holds (x_s, y_s, x_s+1, y_s+1), s = 4q + 2 lh
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -305,6 +314,7 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
             else tim[i][j] += im[i][j];
         }
 
+    if (!active) return;
     // ---- write the (Re, Im) partial tile
     constexpr size_t TSZ = (size_t)TILE * TILE;
     AR* out = reinterpret_cast<AR*>(a.partials) +

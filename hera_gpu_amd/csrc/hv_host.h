@@ -56,7 +56,9 @@ struct hv_ctx {
     int device = 0;
     size_t budget = 0;
     size_t used = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;    // main stream (contraction, reduce, copies)
+    hipStream_t stream2 = nullptr;   // phasor stream of the two-stream pipeline
+    hipEvent_t ev[5] = {};           // pipeline events (no timing)
     std::mutex mu;       // guards `ws` and calls on `stream`
     std::mutex mem_mu;   // guards `used`
     hv::Workspace ws;
@@ -119,10 +121,9 @@ struct JobDev {
 
 struct StageTimer {
     bool on = false;
-    hipStream_t s = nullptr;
     std::vector<hipEvent_t> ev[4][2];
-    int begin(int stage);
-    int end(int stage);
+    int begin(int stage, hipStream_t s);
+    int end(int stage, hipStream_t s);
     int collect(hv_stage_times* t);
     ~StageTimer();
 };
@@ -132,5 +133,10 @@ int prepare_workspace(hv_ctx* c, Workspace& ws, const Plan& p, hipStream_t s, bo
 // Enqueue the kernels of one job on stream s (capturable: no allocation,
 // no memset, no synchronisation).
 int enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t s, StageTimer& tm);
+// Two-stream variant for direct calls: the phasor build of batch b+1 (on
+// ctx->stream2) overlaps the contraction and reduce of batch b (on
+// ctx->stream) through double-buffered operands and partials.  `p` must be
+// planned with half the operand budget; ws holds 2x its W and partials.
+int enqueue_job_pipelined(hv_ctx* c, const JobDev& j, const Plan& p, Workspace& ws, StageTimer& tm);
 
 }  // namespace hv

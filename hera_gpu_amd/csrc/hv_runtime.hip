@@ -114,10 +114,16 @@ extern "C" int hv_ctx_create(int device, size_t hbm_budget, hv_ctx** out) {
         return fail(HV_ERR_HIP, "hipMemGetInfo failed");
     }
     c->budget = hbm_budget ? std::min(hbm_budget, fr) : (size_t)(0.9 * (double)fr);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(HV_ERR_HIP, "hipStreamCreate failed");
     }
+    for (auto& e : c->ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            delete c;
+            return fail(HV_ERR_HIP, "hipEventCreate failed");
+        }
     *out = c;
     return HV_OK;
 }
@@ -126,7 +132,10 @@ extern "C" int hv_ctx_destroy(hv_ctx* c) {
     if (!c) return HV_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream2);
     release(c, c->ws);
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HV_OK;
@@ -160,7 +169,7 @@ float hv::fp16_scale(double max_amp) {
 }
 
 // -------------------------------------------------------------- the job
-int StageTimer::begin(int stage) {
+int StageTimer::begin(int stage, hipStream_t s) {
     if (!on) return HV_OK;
     hipEvent_t e;
     HV_HIP(hipEventCreate(&e));
@@ -168,7 +177,7 @@ int StageTimer::begin(int stage) {
     ev[stage][0].push_back(e);
     return HV_OK;
 }
-int StageTimer::end(int stage) {
+int StageTimer::end(int stage, hipStream_t s) {
     if (!on) return HV_OK;
     hipEvent_t e;
     HV_HIP(hipEventCreate(&e));
@@ -240,75 +249,112 @@ int hv::prepare_workspace(hv_ctx* c, Workspace& ws, const Plan& p, hipStream_t s
     return HV_OK;
 }
 
-int hv::enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t stream, StageTimer& tm) {
-    const int T = j.ntimes, F = j.nfreq, N = j.nant;
-    const int64_t kpad = p.kpad;
-    double* crd_top = static_cast<double*>(ws.buf[B_CRD_TOP].p);
-
-    CoordsArgs ca{j.eq2tops, j.crd_eq, crd_top, kpad, T, j.dyn_kpad};
-    HV_CHECK(tm.begin(0));
+static int enqueue_coords(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t stream, StageTimer& tm) {
+    CoordsArgs ca{j.eq2tops, j.crd_eq, static_cast<double*>(ws.buf[B_CRD_TOP].p), p.kpad, j.ntimes, j.dyn_kpad};
+    HV_CHECK(tm.begin(0, stream));
     HV_HIP(launch_coords(ca, stream));
-    HV_CHECK(tm.end(0));
+    HV_CHECK(tm.end(0, stream));
+    return HV_OK;
+}
 
-    for (int t0 = 0; t0 < T; t0 += p.tb) {
-        const int nt = std::min(p.tb, T - t0);
-        for (int f0 = 0; f0 < F; f0 += p.fb) {
-            const int nf = std::min(p.fb, F - f0);
-            PhasorArgs pa{};
-            pa.crd_top = crd_top + (size_t)t0 * 3 * kpad;
-            pa.isqrt = j.isqrt;
-            pa.beam = j.beam;
-            pa.antpos = j.antpos;
-            pa.inv_lambda = j.inv_lambda + f0;
-            pa.W = ws.buf[B_W].p;
-            pa.unit_stride = p.unit_bytes;
-            pa.row_stride = p.row_bytes;
-            pa.kpad = kpad;
-            pa.nant = N;
-            pa.beam_px = j.beam_px;
-            pa.nfreq = nf;
-            pa.ntime = nt;
-            pa.amp_scale = j.amp_scale;
-            pa.in_f64 = j.beam_f64;
-            pa.dyn_kpad = j.dyn_kpad;
-            HV_CHECK(tm.begin(1));
-            HV_HIP(launch_phasor(j.mode, pa, stream));
-            HV_CHECK(tm.end(1));
+static int enqueue_phasor(const JobDev& j, const Plan& p, Workspace& ws, void* W, int t0, int nt, int f0, int nf,
+                          hipStream_t stream, StageTimer& tm) {
+    PhasorArgs pa{};
+    pa.crd_top = static_cast<double*>(ws.buf[B_CRD_TOP].p) + (size_t)t0 * 3 * p.kpad;
+    pa.isqrt = j.isqrt;
+    pa.beam = j.beam;
+    pa.antpos = j.antpos;
+    pa.inv_lambda = j.inv_lambda + f0;
+    pa.W = W;
+    pa.unit_stride = p.unit_bytes;
+    pa.row_stride = p.row_bytes;
+    pa.kpad = p.kpad;
+    pa.nant = j.nant;
+    pa.beam_px = j.beam_px;
+    pa.nfreq = nf;
+    pa.ntime = nt;
+    pa.amp_scale = j.amp_scale;
+    pa.in_f64 = j.beam_f64;
+    pa.dyn_kpad = j.dyn_kpad;
+    HV_CHECK(tm.begin(1, stream));
+    HV_HIP(launch_phasor(j.mode, pa, stream));
+    HV_CHECK(tm.end(1, stream));
+    return HV_OK;
+}
 
-            HerkArgs ha{};
-            ha.W = ws.buf[B_W].p;
-            ha.unit_stride = p.unit_bytes;
-            ha.row_stride = p.row_bytes;
-            ha.partials = ws.buf[B_PART].p;
-            ha.nant = N;
-            ha.nb = p.nb;
-            ha.ntiles = p.ntiles;
-            ha.kpad = kpad;
-            ha.nsplit = p.nsplit;
-            ha.nunits = nt * nf;
-            ha.dyn_kpad = j.dyn_kpad;
-            HV_CHECK(tm.begin(2));
-            HV_HIP(launch_herk(j.mode, ha, stream));
-            HV_CHECK(tm.end(2));
+static int enqueue_contract(const JobDev& j, const Plan& p, const void* W, void* part, int t0, int nt, int f0,
+                            int nf, hipStream_t stream, StageTimer& tm) {
+    HerkArgs ha{};
+    ha.W = W;
+    ha.unit_stride = p.unit_bytes;
+    ha.row_stride = p.row_bytes;
+    ha.partials = part;
+    ha.nant = j.nant;
+    ha.nb = p.nb;
+    ha.ntiles = p.ntiles;
+    ha.kpad = p.kpad;
+    ha.nsplit = p.nsplit;
+    ha.nunits = nt * nf;
+    ha.dyn_kpad = j.dyn_kpad;
+    HV_CHECK(tm.begin(2, stream));
+    HV_HIP(launch_herk(j.mode, ha, stream));
+    HV_CHECK(tm.end(2, stream));
 
-            ReduceArgs ra{};
-            ra.partials = ws.buf[B_PART].p;
-            ra.vis = j.vis;
-            ra.t0 = t0;
-            ra.f0 = f0;
-            ra.nfreq_launch = nf;
-            ra.ntimes_total = T;
-            ra.nant = N;
-            ra.nb = p.nb;
-            ra.ntiles = p.ntiles;
-            ra.nsplit = p.nsplit;
-            ra.nunits = nt * nf;
-            ra.tile = p.tile;
-            ra.scale = 1.0 / ((double)j.amp_scale * (double)j.amp_scale);
-            ra.dyn_kpad = j.dyn_kpad;
-            HV_CHECK(tm.begin(3));
-            HV_HIP(launch_reduce(j.mode, ra, stream));
-            HV_CHECK(tm.end(3));
+    ReduceArgs ra{};
+    ra.partials = part;
+    ra.vis = j.vis;
+    ra.t0 = t0;
+    ra.f0 = f0;
+    ra.nfreq_launch = nf;
+    ra.ntimes_total = j.ntimes;
+    ra.nant = j.nant;
+    ra.nb = p.nb;
+    ra.ntiles = p.ntiles;
+    ra.nsplit = p.nsplit;
+    ra.nunits = nt * nf;
+    ra.tile = p.tile;
+    ra.scale = 1.0 / ((double)j.amp_scale * (double)j.amp_scale);
+    ra.dyn_kpad = j.dyn_kpad;
+    HV_CHECK(tm.begin(3, stream));
+    HV_HIP(launch_reduce(j.mode, ra, stream));
+    HV_CHECK(tm.end(3, stream));
+    return HV_OK;
+}
+
+int hv::enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t stream, StageTimer& tm) {
+    HV_CHECK(enqueue_coords(j, p, ws, stream, tm));
+    for (int t0 = 0; t0 < j.ntimes; t0 += p.tb) {
+        const int nt = std::min(p.tb, j.ntimes - t0);
+        for (int f0 = 0; f0 < j.nfreq; f0 += p.fb) {
+            const int nf = std::min(p.fb, j.nfreq - f0);
+            HV_CHECK(enqueue_phasor(j, p, ws, ws.buf[B_W].p, t0, nt, f0, nf, stream, tm));
+            HV_CHECK(enqueue_contract(j, p, ws.buf[B_W].p, ws.buf[B_PART].p, t0, nt, f0, nf, stream, tm));
+        }
+    }
+    return HV_OK;
+}
+
+int hv::enqueue_job_pipelined(hv_ctx* c, const JobDev& j, const Plan& p, Workspace& ws, StageTimer& tm) {
+    hipStream_t sm = c->stream, sp = c->stream2;
+    hipEvent_t start = c->ev[0], ph_done[2] = {c->ev[1], c->ev[2]}, hk_done[2] = {c->ev[3], c->ev[4]};
+    char* W[2] = {static_cast<char*>(ws.buf[B_W].p), static_cast<char*>(ws.buf[B_W].p) + p.w_bytes};
+    char* P[2] = {static_cast<char*>(ws.buf[B_PART].p), static_cast<char*>(ws.buf[B_PART].p) + p.part_bytes};
+    // the phasor stream starts after everything already queued on the main stream
+    HV_HIP(hipEventRecord(start, sm));
+    HV_HIP(hipStreamWaitEvent(sp, start, 0));
+    HV_CHECK(enqueue_coords(j, p, ws, sp, tm));
+    int b = 0;
+    for (int t0 = 0; t0 < j.ntimes; t0 += p.tb) {
+        const int nt = std::min(p.tb, j.ntimes - t0);
+        for (int f0 = 0; f0 < j.nfreq; f0 += p.fb, ++b) {
+            const int nf = std::min(p.fb, j.nfreq - f0);
+            const int k = b & 1;
+            if (b >= 2) HV_HIP(hipStreamWaitEvent(sp, hk_done[k], 0));   // contraction b-2 done with W[k]
+            HV_CHECK(enqueue_phasor(j, p, ws, W[k], t0, nt, f0, nf, sp, tm));
+            HV_HIP(hipEventRecord(ph_done[k], sp));
+            HV_HIP(hipStreamWaitEvent(sm, ph_done[k], 0));
+            HV_CHECK(enqueue_contract(j, p, W[k], P[k], t0, nt, f0, nf, sm, tm));
+            HV_HIP(hipEventRecord(hk_done[k], sm));
         }
     }
     return HV_OK;
@@ -316,8 +362,17 @@ int hv::enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t s
 
 static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
     Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c));
-    HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
-    return enqueue_job(j, p, c->ws, c->stream, tm);
+    const int64_t batches = (int64_t)((j.ntimes + p.tb - 1) / p.tb) * ((j.nfreq + p.fb - 1) / p.fb);
+    if (batches < 2) {
+        HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
+        return enqueue_job(j, p, c->ws, c->stream, tm);
+    }
+    // several batches: double-buffer operands and partials at half the budget each
+    p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c) / 2);
+    HV_CHECK(ensure(c, c->ws, B_CRD_TOP, p.crd_top_bytes, c->stream));
+    HV_CHECK(ensure(c, c->ws, B_W, 2 * p.w_bytes, c->stream));
+    HV_CHECK(ensure(c, c->ws, B_PART, 2 * p.part_bytes, c->stream));
+    return enqueue_job_pipelined(c, j, p, c->ws, tm);
 }
 
 // ---------------------------------------------------------- argument checks
@@ -559,7 +614,6 @@ extern "C" int hv_simulate(hv_ctx* c, int mode, const hv_sky* sky, int64_t nsrc_
     HV_HIP(hipEventCreate(&e1));
     StageTimer tm;
     tm.on = times != nullptr;
-    tm.s = c->stream;
     HV_HIP(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; ++r) HV_CHECK(run_job(c, j, tm));
     HV_HIP(hipEventRecord(e1, c->stream));

@@ -167,3 +167,18 @@ def simulate(sky: Sky, beam: Beam, antpos, freqs, eq2tops, gemm="fp32", nsrc=Non
                                  C.byref(st) if timing else None)
     _lib.check(rc, "hv_simulate")
     return out, (st.as_dict() if timing else None)
+
+
+def bench_copy(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> dict:
+    """Plain device copy of nbytes: ms per copy and GB/s (read + write)."""
+    ms = C.c_double()
+    _lib.check(_lib.load().hv_bench_copy(_lib.context(device).handle, nbytes, reps, C.byref(ms)), "hv_bench_copy")
+    return {"bytes": nbytes, "ms": ms.value, "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+
+
+def bench_mfma(gemm: str, iters: int = 20000, device: int = 0) -> float:
+    """Measured matrix-core TFLOP/s of the MFMA instruction used by `gemm`."""
+    t = C.c_double()
+    _lib.check(_lib.load().hv_bench_mfma(_lib.context(device).handle, _mode(gemm), iters, C.byref(t)),
+               "hv_bench_mfma")
+    return t.value

@@ -149,6 +149,15 @@ int hv_simulate(hv_ctx* ctx, int mode, const hv_sky* sky, int64_t nsrc_used,
                 void* vis_out, int reps, hv_stage_times* times);
 
 /* ---------------------------------------------------------------------------
+ * Calibration (no reference counterpart): a plain device copy of `bytes`
+ * (ms per copy; HBM bandwidth = 2*bytes/time, and the yardstick for
+ * rocprofv3 byte counters) and register-resident MFMA loops of each
+ * arithmetic (measured matrix-core TFLOP/s).
+ * ------------------------------------------------------------------------- */
+int hv_bench_copy(hv_ctx* ctx, size_t bytes, int reps, double* ms_per_copy);
+int hv_bench_mfma(hv_ctx* ctx, int mode, int iters, double* tflops);
+
+/* ---------------------------------------------------------------------------
  * Serving (no reference counterpart: the reference has no request loop).
  * A server admits queued jobs into slots as slots and device memory free up;
  * admitted jobs run concurrently on per-slot streams, replaying a hipGraph
