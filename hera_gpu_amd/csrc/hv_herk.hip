@@ -43,7 +43,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int STAGE_BYTES = 128;
 constexpr int LDS_STRIDE = 144;
 
 int herk_tile(int /*mode*/) { return 64; }
@@ -99,7 +98,7 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
     constexpr int WT = TILE / 2;                  // wave tile edge (complex)
     constexpr int NS = WT / BLK;                  // subtiles per wave per dim
     constexpr int NCH = TILE * (STAGE_BYTES / 16) / 256;   // 16-B chunks / thread / operand
-    constexpr int SRC_STAGE = F64 ? 8 : 16;
+    constexpr int SRC_STAGE = stage_src(F64);
     constexpr int STAGES_SPLIT = KSPLIT / SRC_STAGE;
     constexpr int FOLD = FOLD_SRC / SRC_STAGE;
     using ACC = typename std::conditional<F64, f64x4, f32x16>::type;
@@ -125,30 +124,32 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
 
     const char* Wu = reinterpret_cast<const char*>(a.W) + (size_t)unit * a.unit_stride;
 
-    // ---- loader addressing
-    const char* gA[NCH];
-    const char* gB[NCH];
+    // ---- loader addressing: a stage of TILE antennas is one contiguous
+    // TILE*128-byte block (stage-major operands); thread chunk c covers bytes
+    // [16 idx, 16 idx + 16) of it, i.e. row idx/8, 16-byte column idx%8
+    int roff[NCH];
     int loff[NCH];
     bool vA[NCH], vB[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const int idx = tid + 256 * c;
         const int r = idx >> 3, c16 = idx & 7;
-        const int ra = bi * TILE + r, rb = bj * TILE + r;
-        vA[c] = ra < a.nant;
-        vB[c] = rb < a.nant;
-        gA[c] = Wu + (size_t)(vA[c] ? ra : 0) * a.row_stride + c16 * 16;
-        gB[c] = Wu + (size_t)(vB[c] ? rb : 0) * a.row_stride + c16 * 16;
+        vA[c] = bi * TILE + r < a.nant;
+        vB[c] = bj * TILE + r < a.nant;
+        roff[c] = idx * 16;
         loff[c] = r * LDS_STRIDE + c16 * 16;
     }
+    const size_t stage_bytes = (size_t)a.nant * STAGE_BYTES;
+    const char* gA0 = Wu + (size_t)bi * TILE * STAGE_BYTES;
+    const char* gB0 = Wu + (size_t)bj * TILE * STAGE_BYTES;
     uint4 rA[NCH], rB[NCH];
     const uint4 zero4 = make_uint4(0, 0, 0, 0);
     auto gload = [&](int st) {
-        const size_t o = (size_t)st * STAGE_BYTES;
+        const size_t o = (size_t)st * stage_bytes;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-            rA[c] = vA[c] ? *reinterpret_cast<const uint4*>(gA[c] + o) : zero4;
-            if (!diag) rB[c] = vB[c] ? *reinterpret_cast<const uint4*>(gB[c] + o) : zero4;
+            rA[c] = vA[c] ? *reinterpret_cast<const uint4*>(gA0 + o + roff[c]) : zero4;
+            if (!diag) rB[c] = vB[c] ? *reinterpret_cast<const uint4*>(gB0 + o + roff[c]) : zero4;
         }
     };
     auto sstore = [&](int buf) {
@@ -326,8 +327,8 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
 #pragma unroll
             for (int e = 0; e < NACC; ++e) {
                 int r, c;
-                if constexpr (F64) {           // 16x16 f64 block: col = lane%16, row = 4*(lane/16)+e
-                    r = 4 * (lane >> 4) + e;
+                if constexpr (F64) {           // 16x16 f64 block: col = lane%16, row = lane/16 + 4e
+                    r = (lane >> 4) + 4 * e;
                     c = lane & 15;
                 } else {                       // 32x32 f32 block
                     r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);

@@ -8,16 +8,19 @@
 //   isqrt      double[Kpad]            sqrt(I_sky), zero beyond K
 //   beam       real[N][P][P]           [ant][y][x]
 //   crd_top    double[T][3][Kpad]      rotated source directions
-//   W          operand[units][N][row]  phasor rows, one per antenna (below)
+//   W          operand[units][stage][N][128 B]  phasors, stage-major (below)
 //   partials   acc[units][nsplit][ntiles][2][TILE][TILE]   (Re, Im) per split
 //   vis        complex[F][T][N][N]
 //
-// Phasor operand rows interleave (re, im) per source so a row is the
-// complex phasor v[a][0..Kpad):
-//   FP32 / FP64 : complex64 / complex128 per source;
-//   split modes : 32-byte groups of 4 sources, [8 hi halves | 8 lo halves],
-//                 each half ordered (x0 y0 x1 y1 x2 y2 x3 y3).
-// Row bytes: 8*Kpad (FP32 and split), 16*Kpad (FP64).
+// Phasor operands are stage-major: the sources are cut into stages of
+// STAGE_SRC(mode) sources (128 bytes per antenna), and stage st of antenna a
+// sits at byte ((st * N + a) * 128) of its unit, so one contraction stage of
+// 64 antennas is one contiguous 8 KB block.  Within a stage, (re, im)
+// interleave per source:
+//   FP32 / FP64 : complex64 (16 sources) / complex128 (8 sources);
+//   split modes : 4 groups of 4 sources, each 32 bytes [8 hi | 8 lo] halves
+//                 ordered (x0 y0 x1 y1 x2 y2 x3 y3).
+// Bytes per unit: N * Kpad * 8 (FP32 and split), N * Kpad * 16 (FP64).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,9 +35,16 @@ constexpr double C_LIGHT = 299792458.0;
 
 enum Mode { MODE_FP32 = 0, MODE_SPLIT_BF16 = 1, MODE_SPLIT_FP16 = 2, MODE_FP64 = 3 };
 
+constexpr int STAGE_BYTES = 128;
 inline bool mode_is_f64(int mode) { return mode == MODE_FP64; }
+__host__ __device__ constexpr int stage_src(bool f64) { return f64 ? 8 : 16; }
+// bytes of one antenna over all sources (the unit holds N of these)
 inline size_t operand_row_bytes(int mode, int64_t kpad) {
     return (size_t)kpad * (mode_is_f64(mode) ? 16 : 8);
+}
+// byte offset of (stage, antenna) inside a unit
+__host__ __device__ inline size_t operand_offset(int64_t stage, int nant, int ant) {
+    return ((size_t)stage * nant + ant) * STAGE_BYTES;
 }
 
 // Every stage takes the padded source count `kpad` that sizes its buffers
@@ -70,7 +80,6 @@ struct PhasorArgs {
     const double* inv_lambda;// [nfreq] nu/c of the frequencies of this launch
     void* W;                 // operand base of unit 0
     size_t unit_stride;      // bytes between units
-    size_t row_stride;       // bytes between antenna rows
     int64_t kpad;
     int nant;
     int beam_px;
@@ -84,7 +93,6 @@ struct PhasorArgs {
 struct HerkArgs {
     const void* W;           // operand base of unit 0
     size_t unit_stride;
-    size_t row_stride;
     void* partials;          // [units][nsplit][ntiles][2][TILE][TILE]
     int nant;                // valid rows
     int nb;                  // tile blocks per dim
@@ -122,10 +130,9 @@ struct BeamArgs {
     int f64;
 };
 
-struct SplitArgs {           // complex phasors -> operand rows (hv_k_herk)
+struct SplitArgs {           // complex phasors -> operands (hv_k_herk)
     const void* v;           // [N][K] complex64 or complex128
     void* W;
-    size_t row_stride;
     int64_t nsrc;
     int64_t kpad;
     int nant;

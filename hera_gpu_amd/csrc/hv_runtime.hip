@@ -267,7 +267,6 @@ static int enqueue_phasor(const JobDev& j, const Plan& p, Workspace& ws, void* W
     pa.inv_lambda = j.inv_lambda + f0;
     pa.W = W;
     pa.unit_stride = p.unit_bytes;
-    pa.row_stride = p.row_bytes;
     pa.kpad = p.kpad;
     pa.nant = j.nant;
     pa.beam_px = j.beam_px;
@@ -287,7 +286,6 @@ static int enqueue_contract(const JobDev& j, const Plan& p, const void* W, void*
     HerkArgs ha{};
     ha.W = W;
     ha.unit_stride = p.unit_bytes;
-    ha.row_stride = p.row_bytes;
     ha.partials = part;
     ha.nant = j.nant;
     ha.nb = p.nb;
@@ -679,25 +677,27 @@ extern "C" int hv_k_beam(hv_ctx* c, int dtype, const void* bm_cube, int nant, in
     return HV_OK;
 }
 
-// Decode one operand row of `mode` into complex values (float or double).
+// Decode stage-major operands of `mode` into complex values (float or double).
 template <typename R>
-static void decode_rows(int mode, const std::vector<uint8_t>& W, size_t row_bytes, int nant, int64_t nsrc,
-                        double inv_scale, R* out) {
+static void decode_operands(int mode, const std::vector<uint8_t>& W, int nant, int64_t nsrc, double inv_scale,
+                            R* out) {
+    const int sst = stage_src(mode == HV_MODE_FP64);
     for (int a = 0; a < nant; ++a) {
-        const uint8_t* row = W.data() + (size_t)a * row_bytes;
         for (int64_t s = 0; s < nsrc; ++s) {
+            const uint8_t* st = W.data() + operand_offset(s / sst, nant, a);
+            const int i = (int)(s % sst);
             double re, im;
             if (mode == HV_MODE_FP64) {
-                const double* d = reinterpret_cast<const double*>(row) + 2 * s;
+                const double* d = reinterpret_cast<const double*>(st) + 2 * i;
                 re = d[0];
                 im = d[1];
             } else if (mode == HV_MODE_FP32) {
-                const float* f = reinterpret_cast<const float*>(row) + 2 * s;
+                const float* f = reinterpret_cast<const float*>(st) + 2 * i;
                 re = f[0];
                 im = f[1];
             } else {
-                const uint16_t* g = reinterpret_cast<const uint16_t*>(row + (s / 4) * 32);
-                const int e = (int)(s % 4);
+                const uint16_t* g = reinterpret_cast<const uint16_t*>(st + (i / 4) * 32);
+                const int e = i % 4;
                 auto dec = [&](uint16_t h) -> double {
                     if (mode == HV_MODE_SPLIT_BF16) {
                         uint32_t u = (uint32_t)h << 16;
@@ -750,7 +750,6 @@ extern "C" int hv_k_phasor(hv_ctx* c, int mode, int dtype, const void* bm_cube, 
     pa.inv_lambda = (const double*)c->ws.buf[B_INVLAM].p;
     pa.W = c->ws.buf[B_W].p;
     pa.unit_stride = row_bytes * nant;
-    pa.row_stride = row_bytes;
     pa.kpad = kpad;
     pa.nant = nant;
     pa.beam_px = beam_px;
@@ -762,8 +761,8 @@ extern "C" int hv_k_phasor(hv_ctx* c, int mode, int dtype, const void* bm_cube, 
     std::vector<uint8_t> W(row_bytes * nant);
     HV_HIP(hipMemcpyAsync(W.data(), c->ws.buf[B_W].p, W.size(), hipMemcpyDeviceToHost, c->stream));
     HV_HIP(hipStreamSynchronize(c->stream));
-    if (mode == HV_MODE_FP64) decode_rows<double>(mode, W, row_bytes, nant, nsrc, 1.0 / scale, (double*)v_out);
-    else decode_rows<float>(mode, W, row_bytes, nant, nsrc, 1.0 / scale, (float*)v_out);
+    if (mode == HV_MODE_FP64) decode_operands<double>(mode, W, nant, nsrc, 1.0 / scale, (double*)v_out);
+    else decode_operands<float>(mode, W, nant, nsrc, 1.0 / scale, (float*)v_out);
     return HV_OK;
 }
 
@@ -786,7 +785,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
         for (size_t i = 0; i < (size_t)nant * nsrc * 2; ++i) m = std::max(m, (double)std::fabs(f[i]));
         scale = fp16_scale(m);
     }
-    SplitArgs sa{c->ws.buf[B_AUX].p, c->ws.buf[B_W].p, row_bytes, nsrc, kpad, nant, scale};
+    SplitArgs sa{c->ws.buf[B_AUX].p, c->ws.buf[B_W].p, nsrc, kpad, nant, scale};
     HV_HIP(launch_split(mode, sa, c->stream));
     const int tile = herk_tile(mode);
     const int nb = (nant + tile - 1) / tile;
@@ -797,7 +796,6 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     HerkArgs ha{};
     ha.W = c->ws.buf[B_W].p;
     ha.unit_stride = row_bytes * nant;
-    ha.row_stride = row_bytes;
     ha.partials = c->ws.buf[B_PART].p;
     ha.nant = nant;
     ha.nb = nb;

@@ -21,17 +21,19 @@ template <typename R>
 __device__ inline R clampr(R v, R lo, R hi) { return fmin(fmax(v, lo), hi); }
 
 // Clamp-to-edge bilinear interpolation of beam[y][x] (P x P) at direction
-// cosines (x, y).  The texel coordinate is formed in FP64 and rounded to R
-// (the reference forms it in double and stores it in DTYPE, vis.py:63,69).
+// cosines (x, y).  The texel coordinate and its integer part are formed in
+// FP64 and only the fraction is rounded to R (the reference forms the
+// coordinate in double and stores it in DTYPE, vis.py:63,69; keeping it in
+// FP64 removes the (P-1) 2^-24 quantisation of the interpolation weight).
 template <typename R>
 __device__ inline R beam_bilinear(const R* __restrict__ b, int P, double x, double y) {
-    const R bx = (R)((double)(P - 1) * (0.5 * x + 0.5));
-    const R by = (R)((double)(P - 1) * (0.5 * y + 0.5));
-    const R px = floor(bx), py = floor(by);
-    const R fx = bx - px, fy = by - py;
-    const R pm = (R)(P - 1);
-    const int ix0 = (int)clampr(px, (R)0, pm), ix1 = (int)clampr(px + (R)1, (R)0, pm);
-    const int iy0 = (int)clampr(py, (R)0, pm), iy1 = (int)clampr(py + (R)1, (R)0, pm);
+    const double bx = (double)(P - 1) * (0.5 * x + 0.5);
+    const double by = (double)(P - 1) * (0.5 * y + 0.5);
+    const double px = floor(bx), py = floor(by);
+    const R fx = (R)(bx - px), fy = (R)(by - py);
+    const double pm = (double)(P - 1);
+    const int ix0 = (int)clampr(px, 0.0, pm), ix1 = (int)clampr(px + 1.0, 0.0, pm);
+    const int iy0 = (int)clampr(py, 0.0, pm), iy1 = (int)clampr(py + 1.0, 0.0, pm);
     const R v00 = b[iy0 * P + ix0], v01 = b[iy0 * P + ix1];
     const R v10 = b[iy1 * P + ix0], v11 = b[iy1 * P + ix1];
     const R top = fma(fx, v01 - v00, v00);
@@ -182,9 +184,9 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     const double* cz = cx + 2 * a.kpad;
     const double* isq = a.isqrt;
     const int64_t wg1 = min(wg0 + (int64_t)PH_SRC_PER_WG, kact);
-    char* wrow = reinterpret_cast<char*>(a.W) + (size_t)tb * a.nfreq * a.unit_stride +
-                 (size_t)ant * a.row_stride;
-    const size_t elem = (MODE == MODE_FP64) ? 16 : 8;
+    char* wunit = reinterpret_cast<char*>(a.W) + (size_t)tb * a.nfreq * a.unit_stride;
+    constexpr int SST = stage_src(MODE == MODE_FP64);
+    constexpr size_t elem = (MODE == MODE_FP64) ? 16 : 8;
 
     for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
          s0 += PH_THREADS * PH_SRC_PER_THREAD) {
@@ -209,7 +211,8 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
                 re[j] = amp[j] * cs;
                 im[j] = amp[j] * sn;
             }
-            store_phasors8<MODE, R>(wrow + (size_t)u * a.unit_stride + (size_t)s0 * elem, re, im);
+            store_phasors8<MODE, R>(wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
+                                        (size_t)(s0 % SST) * elem, re, im);
         }
     }
 }
@@ -291,8 +294,10 @@ __global__ __launch_bounds__(256) void split_kernel(SplitArgs a) {
         re[j] = s < a.nsrc ? v[2 * s] * (R)a.scale : (R)0;
         im[j] = s < a.nsrc ? v[2 * s + 1] * (R)a.scale : (R)0;
     }
-    const size_t elem = (MODE == MODE_FP64) ? 16 : 8;
-    store_phasors8<MODE, R>(reinterpret_cast<char*>(a.W) + (size_t)ant * a.row_stride + s0 * elem, re, im);
+    constexpr int SST = stage_src(MODE == MODE_FP64);
+    constexpr size_t elem = (MODE == MODE_FP64) ? 16 : 8;
+    store_phasors8<MODE, R>(reinterpret_cast<char*>(a.W) + operand_offset(s0 / SST, a.nant, ant) +
+                                (size_t)(s0 % SST) * elem, re, im);
 }
 
 hipError_t launch_split(int mode, const SplitArgs& a, hipStream_t s) {

@@ -132,10 +132,14 @@ def test_fixture_split_modes_close_to_fp32():
 def test_common_shift_invariance_km_phases(gemm):
     w = configs.make_disk128(nsrc=20000, ntimes=2)
     freqs = np.array([100e6, 200e6])
-    shift = np.array([1000.0, -700.0, 0.0], dtype=np.float32)
+    # positions on a 1/64 m grid so that both they and the shifted copies are
+    # exact in float32 (the precision=1 cast must not change the baselines)
+    antpos = (np.round(w.antpos.astype(np.float64) * 64) / 64).astype(np.float32)
+    shift = np.array([1024.0, -704.0, 0.0], dtype=np.float32)
+    assert np.array_equal((antpos + shift).astype(np.float64) - shift, antpos.astype(np.float64))
     p = 2 if gemm == "fp64" else 1
-    a = vis_gpu(w.antpos, freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, precision=p, gemm=gemm)
-    b = vis_gpu(w.antpos + shift, freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, precision=p, gemm=gemm)
+    a = vis_gpu(antpos, freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, precision=p, gemm=gemm)
+    b = vis_gpu(antpos + shift, freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, precision=p, gemm=gemm)
     assert rel_err(b, a) < (1e-12 if gemm == "fp64" else 2 * VIS_TOL[gemm])
 
 
