@@ -145,3 +145,60 @@ extern "C" int hv_bench_mfma(hv_ctx* c, int mode, int iters, double* tflops) {
     dev_free(c, out, (size_t)grid * 256 * 4);
     return rc;
 }
+
+// The Hermitian contraction alone on synthetic operands (a constant byte
+// pattern: finite, nonzero values in every format), for tile and prefetch
+// variants; tile = pf = 0 selects the production choice.
+extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int units, int tile, int pf, int reps,
+                             double* ms_per_launch) {
+    if (!c || !ms_per_launch || nant < 1 || nsrc < 1 || units < 1 || reps < 1 || mode < HV_MODE_FP32 ||
+        mode > HV_MODE_FP64)
+        return fail(HV_ERR_ARG, "bad argument");
+    if (tile == 0) tile = herk_tile(mode, nant);
+    if (pf == 0) pf = herk_prefetch(mode, tile);
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_HIP(hipSetDevice(c->device));
+    const int64_t kpad = round_up(nsrc, KALIGN);
+    const size_t unit_bytes = (size_t)nant * operand_row_bytes(mode, kpad);
+    const int nb = (nant + tile - 1) / tile;
+    const int ntiles = nb * (nb + 1) / 2;
+    const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
+    const size_t part = (size_t)units * nsplit * ntiles * 2 * tile * tile * (mode == HV_MODE_FP64 ? 8 : 4);
+    void *W = nullptr, *P = nullptr;
+    HV_CHECK(dev_alloc(c, &W, unit_bytes * units));
+    int rc = dev_alloc(c, &P, part);
+    if (rc != HV_OK) {
+        dev_free(c, W, unit_bytes * units);
+        return rc;
+    }
+    HerkArgs ha{};
+    ha.W = W;
+    ha.unit_stride = unit_bytes;
+    ha.partials = P;
+    ha.nant = nant;
+    ha.nb = nb;
+    ha.ntiles = ntiles;
+    ha.kpad = kpad;
+    ha.nsplit = nsplit;
+    ha.nunits = units;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMemsetAsync(W, 0x3c, unit_bytes * units, c->stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(HV_ERR_HIP, "setup failed");
+    if (rc == HV_OK && launch_herk_variant(mode, tile, pf, ha, c->stream) != hipSuccess)
+        rc = fail(HV_ERR_ARG, "no such contraction variant (mode, tile, prefetch)");
+    if (rc == HV_OK) {
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r) (void)launch_herk_variant(mode, tile, pf, ha, c->stream);
+        (void)hipEventRecord(e1, c->stream);
+        float ms = 0;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(HV_ERR_HIP, "contraction timing failed");
+        *ms_per_launch = ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(c, W, unit_bytes * units);
+    dev_free(c, P, part);
+    return rc;
+}

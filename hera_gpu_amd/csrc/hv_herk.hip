@@ -45,7 +45,10 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDS_STRIDE = 144;
 
-int herk_tile(int /*mode*/) { return 64; }
+// 128x128 tiles halve the operand bytes per MFMA flop, which the split
+// modes need to stay MFMA-bound; FP32 (16x slower MFMA) and FP64 are far
+// from their bandwidth limit at 64x64, and small arrays waste less there.
+int herk_tile(int mode, int nant) { return (nant > 96 && (mode == MODE_SPLIT_BF16 || mode == MODE_SPLIT_FP16)) ? 128 : 64; }
 
 // (x, y) pairs of 16-bit values -> (y, -x): rotate each 32-bit word by 16 and
 // flip the sign bit of the upper half.
@@ -91,19 +94,249 @@ __device__ inline void decode_work(const HerkArgs& a, int64_t L, int& unit, int&
     }
 }
 
+// Compile-time shape of one kernel variant.
 template <int MODE, int TILE>
-__global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
-    constexpr bool F64 = (MODE == MODE_FP64);
-    constexpr int BLK = F64 ? 16 : 32;            // MFMA block edge
-    constexpr int WT = TILE / 2;                  // wave tile edge (complex)
-    constexpr int NS = WT / BLK;                  // subtiles per wave per dim
-    constexpr int NCH = TILE * (STAGE_BYTES / 16) / 256;   // 16-B chunks / thread / operand
-    constexpr int SRC_STAGE = stage_src(F64);
-    constexpr int STAGES_SPLIT = KSPLIT / SRC_STAGE;
-    constexpr int FOLD = FOLD_SRC / SRC_STAGE;
+struct Cfg {
+    static constexpr bool F64 = (MODE == MODE_FP64);
+    static constexpr int BLK = F64 ? 16 : 32;             // MFMA block edge
+    static constexpr int WT = TILE / 2;                   // wave tile edge (complex)
+    static constexpr int NS = WT / BLK;                   // subtiles per wave per dim
+    static constexpr int NCH = TILE * (STAGE_BYTES / 16) / 256;   // 16-B chunks / thread / operand
+    static constexpr int SRC_STAGE = stage_src(F64);
+    static constexpr int STAGES_SPLIT = KSPLIT / SRC_STAGE;
+    // 64-tiles fold the MFMA accumulators into FP32 register totals every
+    // FOLD_SRC sources; 128-tiles (split modes only) have no registers to
+    // spare for totals and accumulate each 8192-source split in one chain.
+    static constexpr bool FOLD_ON = !F64 && TILE == 64;
+    static constexpr int FOLD = FOLD_ON ? FOLD_SRC / SRC_STAGE : STAGES_SPLIT;
+    // Im as P1 - P2 (FP32, FP64) or as one chain of w'.w (split modes)
+    static constexpr bool PAIR_IM = (MODE == MODE_FP32 || MODE == MODE_FP64);
     using ACC = typename std::conditional<F64, f64x4, f32x16>::type;
     using AR = typename std::conditional<F64, double, float>::type;
-    constexpr int NACC = F64 ? 4 : 16;
+    static constexpr int NACC = F64 ? 4 : 16;
+};
+
+// MFMAs of one LDS stage for one wave: Re += w_a . w_b and the Im chain(s)
+// for every active subtile (bit i*NS+j of smask).
+template <int MODE, int TILE, bool ALLON>
+__device__ __forceinline__ void stage_mma(const char* __restrict__ As, const char* __restrict__ Bs, int wr, int wc,
+                                          int lrow, int lh, unsigned smask,
+                                          typename Cfg<MODE, TILE>::ACC (&re)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
+                                          typename Cfg<MODE, TILE>::ACC (&im)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
+                                          typename Cfg<MODE, TILE>::ACC (&p2)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS]) {
+    using C = Cfg<MODE, TILE>;
+    constexpr int NS = C::NS, BLK = C::BLK, WT = C::WT;
+#define ON(i, j) (ALLON || (smask & (1u << ((i) * NS + (j)))))
+    if constexpr (MODE == MODE_FP32) {
+        // lane half lh holds (x_s, y_s, x_s+1, y_s+1), s = 4q + 2 lh
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f32x4 av[NS], bv[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                av[i] = *reinterpret_cast<const f32x4*>(As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + q * 32 + lh * 16);
+                bv[i] = *reinterpret_cast<const f32x4*>(Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + q * 32 + lh * 16);
+            }
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j)
+                    if (ON(i, j)) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            re[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][e], bv[j][e], re[i][j], 0, 0, 0);
+#pragma unroll
+                        for (int e = 0; e < 4; e += 2) {
+                            im[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][e + 1], bv[j][e], im[i][j], 0, 0, 0);
+                            p2[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][e], bv[j][e + 1], p2[i][j], 0, 0, 0);
+                        }
+                    }
+        }
+    } else if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            const int g = 2 * jj + lh;
+            u32x4 ahi[NS], alo[NS], phi[NS], plo[NS], bhi[NS], blo[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                const char* pa = As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + g * 32;
+                ahi[i] = *reinterpret_cast<const u32x4*>(pa);
+                alo[i] = *reinterpret_cast<const u32x4*>(pa + 16);
+                phi[i] = rot_neg16(ahi[i]);
+                plo[i] = rot_neg16(alo[i]);
+                const char* pb = Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + g * 32;
+                bhi[i] = *reinterpret_cast<const u32x4*>(pb);
+                blo[i] = *reinterpret_cast<const u32x4*>(pb + 16);
+            }
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j)
+                    if (ON(i, j)) {
+                        re[i][j] = mfma16<MODE>(ahi[i], bhi[j], re[i][j]);
+                        im[i][j] = mfma16<MODE>(phi[i], bhi[j], im[i][j]);
+                        re[i][j] = mfma16<MODE>(ahi[i], blo[j], re[i][j]);
+                        im[i][j] = mfma16<MODE>(phi[i], blo[j], im[i][j]);
+                        re[i][j] = mfma16<MODE>(alo[i], bhi[j], re[i][j]);
+                        im[i][j] = mfma16<MODE>(plo[i], bhi[j], im[i][j]);
+                    }
+        }
+    } else {  // FP64: lane quarter lh holds (x_s, y_s, x_s+1, y_s+1), s = 2 lh
+        f64x4 av[NS], bv[NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            av[i] = *reinterpret_cast<const f64x4*>(As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + lh * 32);
+            bv[i] = *reinterpret_cast<const f64x4*>(Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + lh * 32);
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+                if (ON(i, j)) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][e], bv[j][e], re[i][j], 0, 0, 0);
+#pragma unroll
+                    for (int e = 0; e < 4; e += 2) {
+                        im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][e + 1], bv[j][e], im[i][j], 0, 0, 0);
+                        p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][e], bv[j][e + 1], p2[i][j], 0, 0, 0);
+                    }
+                }
+    }
+#undef ON
+}
+
+// Loader state of one thread (see herk_kernel): two register sets so that
+// the loads of stage st+2 are in flight while stage st is computed and stage
+// st+1 is written to LDS.
+template <int NCH>
+struct LoopState {
+    const char* gA;
+    const char* gB;
+    size_t stage_bytes;
+    u32x4 rA[2][NCH], rB[2][NCH];
+    uint32_t mA[NCH], mB[NCH];
+    int roffA[NCH], roffB[NCH], loff[NCH];
+};
+
+template <int NCH, int SET>
+__device__ __forceinline__ void load_stage(LoopState<NCH>& ls, int st) {
+    const size_t o = (size_t)st * ls.stage_bytes;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        ls.rA[SET][c] = *reinterpret_cast<const u32x4*>(ls.gA + o + ls.roffA[c]);
+        ls.rB[SET][c] = *reinterpret_cast<const u32x4*>(ls.gB + o + ls.roffB[c]);
+    }
+}
+
+template <int NCH, int SET>
+__device__ __forceinline__ void store_stage(LoopState<NCH>& ls, char* dst, int op_bytes) {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        *reinterpret_cast<u32x4*>(dst + ls.loff[c]) = ls.rA[SET][c] & ls.mA[c];
+        *reinterpret_cast<u32x4*>(dst + op_bytes + ls.loff[c]) = ls.rB[SET][c] & ls.mB[c];
+    }
+}
+
+// Two stages (st in LDS buffer 0, st+1 in buffer 1) of one accumulation
+// group.  PF = 2: stage st's operands are in LDS and stage st+1's in register
+// set 1 on entry, and each half issues the loads two stages ahead.  PF = 1:
+// only stage st is staged on entry and each half loads one stage ahead.
+template <int MODE, int TILE, int KIND, int PF>
+__device__ __forceinline__ void two_stages(LoopState<Cfg<MODE, TILE>::NCH>& ls, char* lds, int st, int s_end, int wr,
+                                           int wc, int lrow, int lh, unsigned smask,
+                                           typename Cfg<MODE, TILE>::ACC (&re)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
+                                           typename Cfg<MODE, TILE>::ACC (&im)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
+                                           typename Cfg<MODE, TILE>::ACC (&p2)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS]) {
+    constexpr int NCH = Cfg<MODE, TILE>::NCH;
+    constexpr int OP = TILE * LDS_STRIDE;        // bytes of one operand buffer
+    char* b0 = lds;
+    char* b1 = lds + 2 * OP;
+    __syncthreads();
+    if constexpr (PF == 2) {
+        if (st + 2 < s_end) load_stage<NCH, 0>(ls, st + 2);
+    } else {
+        load_stage<NCH, 1>(ls, st + 1);                // s_end - st is even: st + 1 < s_end
+    }
+    if constexpr (KIND != 0) stage_mma<MODE, TILE, KIND == 1>(b0, b0 + OP, wr, wc, lrow, lh, smask, re, im, p2);
+    store_stage<NCH, 1>(ls, b1, OP);                   // stage st+1
+    __syncthreads();
+    if constexpr (PF == 2) {
+        if (st + 3 < s_end) load_stage<NCH, 1>(ls, st + 3);
+    } else {
+        if (st + 2 < s_end) load_stage<NCH, 0>(ls, st + 2);
+    }
+    if constexpr (KIND != 0) stage_mma<MODE, TILE, KIND == 1>(b1, b1 + OP, wr, wc, lrow, lh, smask, re, im, p2);
+    if (st + 2 < s_end) store_stage<NCH, 0>(ls, b0, OP);
+}
+
+// Stages [s_begin, s_end) of one split (an even count).  KIND 0: the wave
+// only stages operands; 1: every subtile of the wave is computed; 2: the
+// subtiles of smask.  Keeping the MFMAs free of branches in the common case
+// (KIND 1) lets the accumulators live in AGPRs across stages.  With the fold
+// on, each FOLD-stage group accumulates from zero and is added to the FP32
+// totals (tre, tim); without it the whole split is one group written to
+// (tre, tim) at the end.
+template <int MODE, int TILE, int KIND, int PF>
+__device__ __forceinline__ void split_loop(LoopState<Cfg<MODE, TILE>::NCH>& ls, char* lds, int s_begin, int s_end,
+                                           int wr, int wc, int lrow, int lh, unsigned smask,
+                                           typename Cfg<MODE, TILE>::ACC (&tre)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
+                                           typename Cfg<MODE, TILE>::ACC (&tim)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS]) {
+    using C = Cfg<MODE, TILE>;
+    constexpr int NS = C::NS, NACC = C::NACC;
+    using ACC = typename C::ACC;
+    auto zero = [](ACC (&x)[NS][NS]) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+#pragma unroll
+                for (int e = 0; e < NACC; ++e) x[i][j][e] = 0;
+    };
+    if constexpr (C::FOLD_ON) {
+        zero(tre);
+        zero(tim);
+        for (int st = s_begin; st < s_end;) {
+            const int g_end = min(st + C::FOLD, s_end);
+            ACC re[NS][NS], im[NS][NS], p2[NS][NS];
+            zero(re);
+            zero(im);
+            zero(p2);
+            for (; st < g_end; st += 2)
+                two_stages<MODE, TILE, KIND, PF>(ls, lds, st, s_end, wr, wc, lrow, lh, smask, re, im, p2);
+            if constexpr (KIND != 0) {
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) {
+                        tre[i][j] += re[i][j];
+                        if constexpr (C::PAIR_IM) tim[i][j] += im[i][j] - p2[i][j];   // P1 - P2
+                        else tim[i][j] += im[i][j];
+                    }
+            }
+        }
+    } else {
+        ACC p2[NS][NS];
+        zero(tre);
+        zero(tim);
+        zero(p2);
+        for (int st = s_begin; st < s_end; st += 2)
+            two_stages<MODE, TILE, KIND, PF>(ls, lds, st, s_end, wr, wc, lrow, lh, smask, tre, tim, p2);
+        if constexpr (C::PAIR_IM && KIND != 0) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) tim[i][j] -= p2[i][j];                 // P1 - P2
+        }
+    }
+}
+
+template <int MODE, int TILE, int PF>
+__global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
+    using C = Cfg<MODE, TILE>;
+    constexpr int NS = C::NS, BLK = C::BLK, WT = C::WT, NCH = C::NCH, NACC = C::NACC;
+    using ACC = typename C::ACC;
+    using AR = typename C::AR;
 
     __shared__ __attribute__((aligned(16))) char lds[2][2][TILE * LDS_STRIDE];
 
@@ -117,205 +350,82 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
     decode_tile(tile, a.nb, bi, bj);
     const bool diag = (bi == bj);
 
-    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / SRC_STAGE);
-    const int s_begin = split * STAGES_SPLIT;
-    const int s_end = min(s_begin + STAGES_SPLIT, kst);
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / C::SRC_STAGE);
+    const int s_begin = split * C::STAGES_SPLIT;
+    const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
     if (s_begin >= s_end) return;   // split beyond the job's sources (uniform per workgroup)
 
-    const char* Wu = reinterpret_cast<const char*>(a.W) + (size_t)unit * a.unit_stride;
-
-    // ---- loader addressing: a stage of TILE antennas is one contiguous
-    // TILE*128-byte block (stage-major operands); thread chunk c covers bytes
-    // [16 idx, 16 idx + 16) of it, i.e. row idx/8, 16-byte column idx%8
-    int roff[NCH];
-    int loff[NCH];
-    bool vA[NCH], vB[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        const int idx = tid + 256 * c;
-        const int r = idx >> 3, c16 = idx & 7;
-        vA[c] = bi * TILE + r < a.nant;
-        vB[c] = bj * TILE + r < a.nant;
-        roff[c] = idx * 16;
-        loff[c] = r * LDS_STRIDE + c16 * 16;
-    }
-    const size_t stage_bytes = (size_t)a.nant * STAGE_BYTES;
-    const char* gA0 = Wu + (size_t)bi * TILE * STAGE_BYTES;
-    const char* gB0 = Wu + (size_t)bj * TILE * STAGE_BYTES;
-    uint4 rA[NCH], rB[NCH];
-    const uint4 zero4 = make_uint4(0, 0, 0, 0);
-    auto gload = [&](int st) {
-        const size_t o = (size_t)st * stage_bytes;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            rA[c] = vA[c] ? *reinterpret_cast<const uint4*>(gA0 + o + roff[c]) : zero4;
-            if (!diag) rB[c] = vB[c] ? *reinterpret_cast<const uint4*>(gB0 + o + roff[c]) : zero4;
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            *reinterpret_cast<uint4*>(&lds[buf][0][loff[c]]) = rA[c];
-            if (!diag) *reinterpret_cast<uint4*>(&lds[buf][1][loff[c]]) = rB[c];
-        }
-    };
-
-    // Re chain; Im either as one chain of w'.w (split modes) or as P1 - P2
-    // with P1 = sum y_a x_b and P2 = sum x_a y_b (FP32, FP64), which cancel
-    // exactly when rows a and b are equal.
-    constexpr bool PAIR_IM = (MODE == MODE_FP32 || MODE == MODE_FP64);
-    ACC re[NS][NS], im[NS][NS], p1[NS][NS], p2[NS][NS];
-    ACC tre[NS][NS], tim[NS][NS];
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
-#pragma unroll
-        for (int j = 0; j < NS; ++j)
-#pragma unroll
-            for (int e = 0; e < NACC; ++e) {
-                re[i][j][e] = 0; im[i][j][e] = 0; p1[i][j][e] = 0; p2[i][j][e] = 0;
-                tre[i][j][e] = 0; tim[i][j][e] = 0;
-            }
-
-    // A wave whose 32x32 (complex) region lies strictly below the diagonal of
-    // a diagonal tile, or wholly in the antenna padding, skips its MFMAs and
-    // its partial write (the reduce stage reads the mirrored entry instead);
-    // it still helps stage operands.  Co-resident workgroups use the idle
-    // SIMD time.
-    const bool active = (bi * TILE + wr * WT < a.nant) && (bj * TILE + wc * WT < a.nant) && !(diag && wr > wc);
-
-    // lane -> (row within block, k slot group)
-    const int lrow = F64 ? (lane & 15) : (lane & 31);
-    const int lh = F64 ? (lane >> 4) : (lane >> 5);
-
-    gload(s_begin);
-    sstore(0);
-    int since_fold = 0;
-    for (int st = s_begin; st < s_end; ++st) {
-        const int buf = (st - s_begin) & 1;
-        __syncthreads();
-        if (st + 1 < s_end) gload(st + 1);
-        const char* As = lds[buf][0];
-        const char* Bs = diag ? lds[buf][0] : lds[buf][1];
-
-        if (!active) {
-            // nothing to compute for this wave
-        } else if constexpr (MODE == MODE_FP32) {
-            // lane half lh holds (x_s, y_s, x_s+1, y_s+1), s = 4q + 2 lh
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                f32x4 av[NS], bv[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                    av[i] = *reinterpret_cast<const f32x4*>(As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + q * 32 + lh * 16);
-                    bv[i] = *reinterpret_cast<const f32x4*>(Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + q * 32 + lh * 16);
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-#pragma unroll
-                    for (int i = 0; i < NS; ++i)
-#pragma unroll
-                        for (int j = 0; j < NS; ++j)
-                            re[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][e], bv[j][e], re[i][j], 0, 0, 0);
-#pragma unroll
-                for (int e = 0; e < 4; e += 2)
-#pragma unroll
-                    for (int i = 0; i < NS; ++i)
-#pragma unroll
-                        for (int j = 0; j < NS; ++j) {
-                            p1[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][e + 1], bv[j][e], p1[i][j], 0, 0, 0);
-                            p2[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[i][e], bv[j][e + 1], p2[i][j], 0, 0, 0);
-                        }
-            }
-        } else if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
-#pragma unroll
-            for (int jj = 0; jj < 2; ++jj) {
-                const int g = 2 * jj + lh;
-                u32x4 ahi[NS], alo[NS], phi[NS], plo[NS], bhi[NS], blo[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                    const char* pa = As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + g * 32;
-                    ahi[i] = *reinterpret_cast<const u32x4*>(pa);
-                    alo[i] = *reinterpret_cast<const u32x4*>(pa + 16);
-                    phi[i] = rot_neg16(ahi[i]);
-                    plo[i] = rot_neg16(alo[i]);
-                    const char* pb = Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + g * 32;
-                    bhi[i] = *reinterpret_cast<const u32x4*>(pb);
-                    blo[i] = *reinterpret_cast<const u32x4*>(pb + 16);
-                }
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS; ++j) {
-                        re[i][j] = mfma16<MODE>(ahi[i], bhi[j], re[i][j]);
-                        im[i][j] = mfma16<MODE>(phi[i], bhi[j], im[i][j]);
-                    }
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS; ++j) {
-                        re[i][j] = mfma16<MODE>(ahi[i], blo[j], re[i][j]);
-                        im[i][j] = mfma16<MODE>(phi[i], blo[j], im[i][j]);
-                    }
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS; ++j) {
-                        re[i][j] = mfma16<MODE>(alo[i], bhi[j], re[i][j]);
-                        im[i][j] = mfma16<MODE>(plo[i], bhi[j], im[i][j]);
-                    }
-            }
-        } else {  // FP64: lane quarter lh holds (x_s, y_s, x_s+1, y_s+1), s = 2 lh
-            f64x4 av[NS], bv[NS];
-#pragma unroll
-            for (int i = 0; i < NS; ++i) {
-                av[i] = *reinterpret_cast<const f64x4*>(As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + lh * 32);
-                bv[i] = *reinterpret_cast<const f64x4*>(Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + lh * 32);
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS; ++j)
-                        re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][e], bv[j][e], re[i][j], 0, 0, 0);
-#pragma unroll
-            for (int e = 0; e < 4; e += 2)
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS; ++j) {
-                        p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][e + 1], bv[j][e], p1[i][j], 0, 0, 0);
-                        p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][e], bv[j][e + 1], p2[i][j], 0, 0, 0);
-                    }
-        }
-
-        if (st + 1 < s_end) sstore(buf ^ 1);
-        if constexpr (!F64) {
-            if (++since_fold == FOLD) {
-                since_fold = 0;
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS; ++j) {
-                        tre[i][j] += re[i][j];
-                        if constexpr (PAIR_IM) tim[i][j] += p1[i][j] - p2[i][j];
-                        else tim[i][j] += im[i][j];
-#pragma unroll
-                        for (int e = 0; e < NACC; ++e) { re[i][j][e] = 0; im[i][j][e] = 0; p1[i][j][e] = 0; p2[i][j][e] = 0; }
-                    }
-            }
-        }
-    }
+    // A subtile that lies strictly below the diagonal of a diagonal tile, or
+    // wholly in the antenna padding, is skipped (no MFMAs, no partial write;
+    // the reduce stage reads the mirrored entry instead).  A wave with no
+    // subtile left still helps stage operands.
+    unsigned smask = 0;
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            tre[i][j] += re[i][j];
-            if constexpr (PAIR_IM) tim[i][j] += p1[i][j] - p2[i][j];
-            else tim[i][j] += im[i][j];
+            const int r0 = wr * WT + i * BLK, c0 = wc * WT + j * BLK;   // tile coordinates
+            const bool on = (bi * TILE + r0 < a.nant) && (bj * TILE + c0 < a.nant) && !(diag && r0 > c0);
+            smask |= (unsigned)on << (i * NS + j);
         }
+    smask = __builtin_amdgcn_readfirstlane(smask);
 
-    if (!active) return;
+    // ---- loader: a stage of TILE antennas is one contiguous TILE*128-byte
+    // block (stage-major operands); thread chunk c covers bytes
+    // [16 idx, 16 idx + 16) of it, i.e. row idx/8, 16-byte column idx%8.
+    // Loads are unconditional (rows past the last antenna re-read the last
+    // antenna's row) and padding rows are zeroed when the registers go to
+    // LDS, after the stage's MFMAs, so the loads stay in flight meanwhile.
+    const char* Wu = reinterpret_cast<const char*>(a.W) + (size_t)unit * a.unit_stride;
+    const size_t stage_bytes = (size_t)a.nant * STAGE_BYTES;
+    const char* gA = Wu + (size_t)bi * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
+    const char* gB = Wu + (size_t)bj * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
+    // (staging uses a native vector type and a bitwise mask: a ternary on a
+    // struct type would be lowered through private memory)
+    int roffA[NCH], roffB[NCH], loff[NCH];
+    uint32_t mA[NCH], mB[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int idx = tid + 256 * c;
+        const int r = idx >> 3, c16 = idx & 7;
+        const int ra = bi * TILE + r, rb = bj * TILE + r;
+        mA[c] = ra < a.nant ? 0xffffffffu : 0u;
+        mB[c] = rb < a.nant ? 0xffffffffu : 0u;
+        roffA[c] = (min(ra, a.nant - 1) - bi * TILE) * STAGE_BYTES + c16 * 16;
+        roffB[c] = (min(rb, a.nant - 1) - bj * TILE) * STAGE_BYTES + c16 * 16;
+        loff[c] = r * LDS_STRIDE + c16 * 16;
+    }
+    LoopState<NCH> ls;
+    ls.gA = gA;
+    ls.gB = gB;
+    ls.stage_bytes = stage_bytes;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        ls.mA[c] = mA[c]; ls.mB[c] = mB[c];
+        ls.roffA[c] = roffA[c]; ls.roffB[c] = roffB[c]; ls.loff[c] = loff[c];
+    }
+    // prologue: stage s_begin to LDS buffer 0, stage s_begin+1 in flight
+    char* lbase = &lds[0][0][0];
+    load_stage<NCH, 0>(ls, 0);
+    store_stage<NCH, 0>(ls, lbase, TILE * LDS_STRIDE);
+    if constexpr (PF == 2) load_stage<NCH, 1>(ls, 1);   // splits hold a multiple of 4 stages
+
+    // lane -> (row within block, k slot group)
+    const int lrow = C::F64 ? (lane & 15) : (lane & 31);
+    const int lh = C::F64 ? (lane >> 4) : (lane >> 5);
+
+    ACC tre[NS][NS], tim[NS][NS];
+    const unsigned full = (1u << (NS * NS)) - 1;
+    // stage indices below are relative to s_begin (ls.gA/gB already point there)
+    const int n_st = s_end - s_begin;
+    if (smask == full)
+        split_loop<MODE, TILE, 1, PF>(ls, lbase, 0, n_st, wr, wc, lrow, lh, smask, tre, tim);
+    else if (smask)
+        split_loop<MODE, TILE, 2, PF>(ls, lbase, 0, n_st, wr, wc, lrow, lh, smask, tre, tim);
+    else
+        split_loop<MODE, TILE, 0, PF>(ls, lbase, 0, n_st, wr, wc, lrow, lh, smask, tre, tim);
+    if (!smask) return;
+
     // ---- write the (Re, Im) partial tile
     constexpr size_t TSZ = (size_t)TILE * TILE;
     AR* out = reinterpret_cast<AR*>(a.partials) +
@@ -323,11 +433,12 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
 #pragma unroll
     for (int i = 0; i < NS; ++i)
 #pragma unroll
-        for (int j = 0; j < NS; ++j)
+        for (int j = 0; j < NS; ++j) {
+            if (!(smask & (1u << (i * NS + j)))) continue;
 #pragma unroll
             for (int e = 0; e < NACC; ++e) {
                 int r, c;
-                if constexpr (F64) {           // 16x16 f64 block: col = lane%16, row = lane/16 + 4e
+                if constexpr (C::F64) {        // 16x16 f64 block: col = lane%16, row = lane/16 + 4e
                     r = (lane >> 4) + 4 * e;
                     c = lane & 15;
                 } else {                       // 32x32 f32 block
@@ -339,24 +450,35 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
                 out[(size_t)row * TILE + col] = tre[i][j][e];
                 out[TSZ + (size_t)row * TILE + col] = tim[i][j][e];
             }
+        }
 }
 
-template <int MODE, int TILE>
+template <int MODE, int TILE, int PF>
 static hipError_t launch_herk_t(const HerkArgs& a, hipStream_t s) {
     const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
     if (nblk <= 0) return hipSuccess;
-    hipLaunchKernelGGL((herk_kernel<MODE, TILE>), dim3((unsigned)nblk), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((herk_kernel<MODE, TILE, PF>), dim3((unsigned)nblk), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_herk(int mode, const HerkArgs& a, hipStream_t s) {
-    switch (mode) {
-        case MODE_FP32: return launch_herk_t<MODE_FP32, 64>(a, s);
-        case MODE_SPLIT_BF16: return launch_herk_t<MODE_SPLIT_BF16, 64>(a, s);
-        case MODE_SPLIT_FP16: return launch_herk_t<MODE_SPLIT_FP16, 64>(a, s);
-        case MODE_FP64: return launch_herk_t<MODE_FP64, 64>(a, s);
-    }
+// Kernel variants: tile edge (64 / 128) x prefetch depth (1 / 2 stages).
+// variant 0 selects the default for the mode and antenna count.
+hipError_t launch_herk_variant(int mode, int tile, int pf, const HerkArgs& a, hipStream_t s) {
+#define HV_V(M, T, P) if (mode == M && tile == T && pf == P) return launch_herk_t<M, T, P>(a, s)
+    HV_V(MODE_FP32, 64, 1); HV_V(MODE_FP32, 64, 2);
+    HV_V(MODE_SPLIT_BF16, 64, 1); HV_V(MODE_SPLIT_BF16, 64, 2);
+    HV_V(MODE_SPLIT_BF16, 128, 1); HV_V(MODE_SPLIT_BF16, 128, 2);
+    HV_V(MODE_SPLIT_FP16, 64, 1); HV_V(MODE_SPLIT_FP16, 64, 2);
+    HV_V(MODE_SPLIT_FP16, 128, 1); HV_V(MODE_SPLIT_FP16, 128, 2);
+    HV_V(MODE_FP64, 64, 1);
+#undef HV_V
     return hipErrorInvalidValue;
+}
+
+int herk_prefetch(int mode, int tile) { return (mode == MODE_FP64 || tile == 128) ? 1 : 2; }
+
+hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {
+    return launch_herk_variant(mode, tile, herk_prefetch(mode, tile), a, s);
 }
 
 }  // namespace hv

@@ -139,11 +139,13 @@ struct SplitArgs {           // complex phasors -> operands (hv_k_herk)
     float scale;
 };
 
-int herk_tile(int mode);
+int herk_tile(int mode, int nant);
 
 hipError_t launch_coords(const CoordsArgs& a, hipStream_t s);
 hipError_t launch_phasor(int mode, const PhasorArgs& a, hipStream_t s);
-hipError_t launch_herk(int mode, const HerkArgs& a, hipStream_t s);
+hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s);
+hipError_t launch_herk_variant(int mode, int tile, int pf, const HerkArgs& a, hipStream_t s);
+int herk_prefetch(int mode, int tile);
 hipError_t launch_reduce(int mode, const ReduceArgs& a, hipStream_t s);
 hipError_t launch_beam(const BeamArgs& a, hipStream_t s);
 hipError_t launch_split(int mode, const SplitArgs& a, hipStream_t s);

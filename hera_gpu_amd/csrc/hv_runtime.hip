@@ -219,7 +219,7 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.ntimes = ntimes;
     p.nfreq = nfreq;
     p.kpad = kpad;
-    p.tile = herk_tile(mode);
+    p.tile = herk_tile(mode, nant);
     p.nb = (nant + p.tile - 1) / p.tile;
     p.ntiles = p.nb * (p.nb + 1) / 2;
     p.nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
@@ -295,7 +295,7 @@ static int enqueue_contract(const JobDev& j, const Plan& p, const void* W, void*
     ha.nunits = nt * nf;
     ha.dyn_kpad = j.dyn_kpad;
     HV_CHECK(tm.begin(2, stream));
-    HV_HIP(launch_herk(j.mode, ha, stream));
+    HV_HIP(launch_herk(j.mode, p.tile, ha, stream));
     HV_CHECK(tm.end(2, stream));
 
     ReduceArgs ra{};
@@ -787,7 +787,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     }
     SplitArgs sa{c->ws.buf[B_AUX].p, c->ws.buf[B_W].p, nsrc, kpad, nant, scale};
     HV_HIP(launch_split(mode, sa, c->stream));
-    const int tile = herk_tile(mode);
+    const int tile = herk_tile(mode, nant);
     const int nb = (nant + tile - 1) / tile;
     const int ntiles = nb * (nb + 1) / 2;
     const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
@@ -803,7 +803,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     ha.kpad = kpad;
     ha.nsplit = nsplit;
     ha.nunits = 1;
-    HV_HIP(launch_herk(mode, ha, c->stream));
+    HV_HIP(launch_herk(mode, tile, ha, c->stream));
     ReduceArgs ra{};
     ra.partials = c->ws.buf[B_PART].p;
     ra.vis = c->ws.buf[B_OUT].p;

@@ -182,3 +182,13 @@ def bench_mfma(gemm: str, iters: int = 20000, device: int = 0) -> float:
     _lib.check(_lib.load().hv_bench_mfma(_lib.context(device).handle, _mode(gemm), iters, C.byref(t)),
                "hv_bench_mfma")
     return t.value
+
+
+def bench_herk(gemm: str, nant: int, nsrc: int, units: int = 1, tile: int = 0, pf: int = 0, reps: int = 5,
+               device: int = 0) -> dict:
+    """Contraction alone on synthetic operands: ms per launch and useful TFLOP/s."""
+    ms = C.c_double()
+    _lib.check(_lib.load().hv_bench_herk(_lib.context(device).handle, _mode(gemm), nant, nsrc, units, tile, pf,
+                                         reps, C.byref(ms)), "hv_bench_herk")
+    flops = 4.0 * nant * (nant + 1) * nsrc * units
+    return {"ms": ms.value, "tflops": flops / (ms.value * 1e-3) / 1e12}

@@ -156,6 +156,11 @@ int hv_simulate(hv_ctx* ctx, int mode, const hv_sky* sky, int64_t nsrc_used,
  * ------------------------------------------------------------------------- */
 int hv_bench_copy(hv_ctx* ctx, size_t bytes, int reps, double* ms_per_copy);
 int hv_bench_mfma(hv_ctx* ctx, int mode, int iters, double* tflops);
+/* The Hermitian contraction alone on synthetic operands of `units` units of
+ * nant x nsrc; tile (64/128) and prefetch depth (1/2) select a kernel variant,
+ * 0 = the production choice.  Returns ms per launch. */
+int hv_bench_herk(hv_ctx* ctx, int mode, int nant, int64_t nsrc, int units, int tile, int pf, int reps,
+                  double* ms_per_launch);
 
 /* ---------------------------------------------------------------------------
  * Serving (no reference counterpart: the reference has no request loop).
