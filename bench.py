@@ -78,6 +78,43 @@ def cpu_baseline(w, nsrc, gpu_unit):
     return res
 
 
+def serving_leg(njobs, tflops_hint, gemms, nslots, rate=None):
+    """The mixed-job serving stream: Poisson arrivals of jobs with N in {37,128,350},
+    Nsrc log-uniform in [1e3, 2^20], ntimes in {1,8,32}, Nfreq in {1,16,64}."""
+    from hera_gpu_amd import configs, serve, stages
+
+    layouts = {}
+    for lay in ("hex37", "disk128", "hera350"):
+        antpos, beam, crd, flux = configs.serving_catalog(lay)
+        layouts[lay] = (antpos, stages.Beam(beam), stages.Sky(crd, flux))
+    probe = configs.make_serving_stream(njobs, rate_hz=1.0)
+    mean_flops = float(np.mean([herk_flops(len(layouts[j.layout][0]), j.nsrc, j.ntimes * j.nfreq) for j in probe]))
+    if rate is None:
+        # 70% of the capacity implied by the throughput measured on the main workload
+        rate = 0.7 * tflops_hint * 1e12 / mean_flops
+    jobs = configs.make_serving_stream(njobs, rate_hz=rate)
+
+    def make_args(j):
+        antpos, beam, sky = layouts[j.layout]
+        eq, freqs = configs.job_times_freqs(j)
+        return sky, beam, antpos, freqs, eq, j.nsrc
+
+    out = {"jobs": njobs, "rate_hz": rate, "mean_job_flops": mean_flops, "slots": nslots,
+           "latency": "scheduled arrival to result in host memory"}
+    for gemm in gemms:
+        log(f"[bench] serving {njobs} jobs at {rate:.2f}/s with {gemm}")
+        with serve.Server(nslots=nslots) as srv:
+            summ, _ = serve.run_load(srv, jobs, make_args, gemm=gemm, progress=log)
+            summ["stats"] = srv.stats()
+        out[gemm] = summ
+        log(f"[bench] serving {gemm}: p50 {summ['p50_ms']:.0f} ms p99 {summ['p99_ms']:.0f} ms "
+            f"{summ['tflops']:.1f} TFLOP/s")
+    for _, b, s_ in layouts.values():
+        b.close()
+        s_.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +126,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 sample comparison")
     ap.add_argument("--cpu-nsrc", type=int, default=2 ** 16)
+    ap.add_argument("--serve-jobs", type=int, default=120, help="serving stream length (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default: 70%% of capacity)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -164,6 +204,11 @@ def main():
         cpu = cpu_baseline(w, nsrc, gpu_unit)
         log(f"[bench] cpu: {cpu['seconds']:.1f} s, {cpu['value']:.4f} TFLOP/s")
 
+    serving = None
+    if args.serve_jobs > 0:
+        gemms = [args.gemm] + (["fp32"] if args.gemm != "fp32" and not args.no_fp32 else [])
+        serving = serving_leg(args.serve_jobs, value, gemms, args.serve_slots, args.serve_rate)
+
     peak = PEAK_TFLOPS[args.gemm]
     out = {
         "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
@@ -186,7 +231,9 @@ def main():
         "speedup_vs_fp32": (value / fp32["value"]) if fp32 else None,
         "accuracy": accuracy,
         "cpu_baseline": cpu,
-        "serving": None,
+        "serving": serving,
+        "p99_latency_ms": serving[args.gemm]["p99_ms"] if serving else None,
+        "p50_latency_ms": serving[args.gemm]["p50_ms"] if serving else None,
         "wall_s_timed": wall,
     }
     print(json.dumps(out), flush=True)

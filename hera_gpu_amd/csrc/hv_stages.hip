@@ -8,9 +8,9 @@
 //   delay      tau = antpos . crd_top / c                      vis.py:233-236
 //   phasor     v = A sqrt(I) exp(+i 2 pi nu tau)               vis.py:92-110,159
 // Here the delay is never materialised: each phasor thread forms the path
-// difference d = antpos . crd_top in FP64, converts it to turns with
-// nu / c, keeps only the fractional turn (exact in FP64 for the |turns| < 2^52
-// that occur) and evaluates sin/cos of that small angle.
+// difference d = antpos . crd_top in FP64, converts it to quarter turns with
+// 4 nu / c, keeps only the fractional part (exact in FP64 for |turns| < 2^49)
+// and the quadrant, and evaluates sin/cos of that small angle.
 #include "hv_internal.h"
 
 namespace hv {
@@ -41,34 +41,37 @@ __device__ inline R beam_bilinear(const R* __restrict__ b, int P, double x, doub
     return fma(fy, bot - top, top);
 }
 
-// sin and cos of 2 pi t for a phase t in turns (FP32 result, FP64 reduction).
-// t is reduced to quarter turns: 4t = q + u with q = rint(4t) and
-// |u| <= 1/2 exact in FP64; then sin/cos(pi/2 u) come from odd/even
-// polynomials in u fitted here by iteratively reweighted least squares
-// (Lawson's minimax iteration) on [-1/2, 1/2] (max error 8.9e-8 / 7.4e-8 when
-// evaluated in FP32), and the quadrant q mod 4 rotates the pair.
-__device__ inline void sincos_turns(double turns, float* s, float* c) {
-    const double t4 = 4.0 * turns;
+// sin and cos of a phase given in quarter turns t4 (FP32 result, FP64
+// reduction): t4 = q + u with q = rint(t4) and |u| <= 1/2 exact in FP64;
+// sin/cos(pi/2 u) come from odd/even polynomials in u fitted here by
+// iteratively reweighted least squares (Lawson's minimax iteration) on
+// [-1/2, 1/2] (max error 8.9e-8 / 7.4e-8 when evaluated in FP32), and the
+// quadrant q mod 4 rotates the pair.  q mod 4 is read from the low mantissa
+// bits of q + 1.5 * 2^52, exact for |q| < 2^51.
+__device__ inline void sincos_quarter_turns(double t4, float* s, float* c) {
     const double q = rint(t4);
     const float u = (float)(t4 - q);
-    const double qf = q - 4.0 * floor(0.25 * q);     // q mod 4, exact
-    const int k = (int)qf;
+    const int k = (int)(__double2loint(q + 6755399441055744.0) & 3);
     const float u2 = u * u;
     const float ps = u * fmaf(u2, fmaf(u2, fmaf(u2, -0.0046021766029298306f, 0.07968022674322128f),
                                       -0.6459634900093079f), 1.5707963705062866f);
     const float pc = fmaf(u2, fmaf(u2, fmaf(u2, fmaf(u2, 0.0008997500990517437f, -0.020859001204371452f),
                                             0.2536691725254059f), -1.2337005138397217f), 1.0f);
     const bool swap = k & 1;
-    float sv = swap ? pc : ps;
-    float cv = swap ? ps : pc;
-    if (k == 1 || k == 2) cv = -cv;   // quadrants 1, 2: cos < 0 side
-    if (k >= 2) sv = -sv;             // quadrants 2, 3: sin < 0 side
-    *s = sv;
-    *c = cv;
+    const float sv = swap ? pc : ps;
+    const float cv = swap ? ps : pc;
+    // quadrants 2, 3 negate sin; quadrants 1, 2 negate cos
+    *s = __int_as_float(__float_as_int(sv) ^ ((k & 2) << 30));
+    *c = __int_as_float(__float_as_int(cv) ^ (((k + 1) & 2) << 30));
 }
-__device__ inline void sincos_turns(double turns, double* s, double* c) {
-    const double r = turns - rint(turns);
-    sincospi(2.0 * r, s, c);
+__device__ inline void sincos_quarter_turns(double t4, double* s, double* c) {
+    const double r = t4 - rint(t4);            // [-1/2, 1/2] quarter turns
+    sincospi(0.5 * r, s, c);                   // angle pi/2 r, then rotate by the quadrant
+    const int k = (int)(__double2loint(rint(t4) + 6755399441055744.0) & 3);
+    const double sv = (k & 1) ? *c : *s;
+    const double cv = (k & 1) ? *s : *c;
+    *s = (k & 2) ? -sv : sv;
+    *c = ((k + 1) & 2) ? -cv : cv;
 }
 
 // Round-to-nearest-even float -> bf16 / fp16 bit patterns and back.
@@ -202,12 +205,12 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
             d[j] = fma(ax, x, fma(ay, y, az * z));
         }
         for (int u = 0; u < a.nfreq; ++u) {
-            const double il = a.inv_lambda[u];
+            const double il4 = 4.0 * a.inv_lambda[u];   // quarter turns per metre (exact scaling)
             R re[8], im[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 R sn, cs;
-                sincos_turns(d[j] * il, &sn, &cs);
+                sincos_quarter_turns(d[j] * il4, &sn, &cs);
                 re[j] = amp[j] * cs;
                 im[j] = amp[j] * sn;
             }

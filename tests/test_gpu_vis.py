@@ -31,8 +31,17 @@ def load(name):
 
 def run(d, gemm):
     precision = 2 if gemm == "fp64" else 1
-    return vis_gpu(d["antpos"], d["freqs"], d["eq2tops"], d["crd_eq"], d["I_sky"], d["bm_cube"],
+    # vis_gpu takes sqrt(I_sky) in the dtype it is given, as vis.py:174 does;
+    # for the FP64 path pass float64 intensities so it sees the oracle's values
+    I_sky = d["I_sky"].astype(np.float64) if precision == 2 else d["I_sky"]
+    return vis_gpu(d["antpos"], d["freqs"], d["eq2tops"], d["crd_eq"], I_sky, d["bm_cube"],
                    precision=precision, gemm=gemm)
+
+
+def effective_intensity(I_sky, precision):
+    """The intensity vis_gpu computes with: (sqrt(I) in I's dtype, cast like vis.py:174)^2."""
+    rd = np.float32 if precision == 1 else np.float64
+    return np.sqrt(I_sky).astype(rd).astype(np.float64) ** 2
 
 
 # ------------------------------------------------ reference API tests
@@ -104,7 +113,7 @@ def test_compare_cpu_and_phase(precision):
         assert np.median(np.abs(v - v_cpu)) < lim
         assert np.median(np.abs(np.angle(v) - np.angle(v_cpu))) < lim
         # the reference's median bounds are loose; the oracle gives the real bar
-        ref = vis_oracle(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube)[0]
+        ref = vis_oracle(antpos, freq, eq2tops, crd_eq, effective_intensity(I_sky, precision), bm_cube)[0]
         assert rel_err(v, ref) < (1e-12 if precision == 2 else 2e-6)
 
 
