@@ -69,6 +69,32 @@ double mfma_flops_per_iter(int mode) {
     }
 }
 
+// Random finite operands in the storage format of `mode` (values in [-1, 1]).
+template <int MODE>
+__global__ __launch_bounds__(256) void fill_kernel(uint32_t* w, size_t nwords, uint32_t seed) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nwords; i += stride) {
+        // FP64 operands hash per 8-byte element, the others per 4-byte word
+        const uint32_t e = (uint32_t)(MODE == MODE_FP64 ? i >> 1 : i);
+        uint32_t h = e * 2654435761u ^ seed;
+        h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+        const float x = (float)(int)(h & 0xffff) * (1.0f / 32768.0f) - 1.0f;
+        const float y = (float)(int)(h >> 16) * (1.0f / 32768.0f) - 1.0f;
+        uint32_t v;
+        if constexpr (MODE == MODE_SPLIT_BF16) {
+            v = (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)x) | ((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)y) << 16);
+        } else if constexpr (MODE == MODE_SPLIT_FP16) {
+            v = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x) | ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)y) << 16);
+        } else if constexpr (MODE == MODE_FP32) {
+            v = __float_as_uint(x);
+        } else {
+            const uint64_t b = __builtin_bit_cast(uint64_t, (double)x);
+            v = (i & 1) ? (uint32_t)(b >> 32) : (uint32_t)b;
+        }
+        w[i] = v;
+    }
+}
+
 }  // namespace
 
 extern "C" int hv_bench_copy(hv_ctx* c, size_t bytes, int reps, double* ms_per_copy) {
@@ -146,9 +172,9 @@ extern "C" int hv_bench_mfma(hv_ctx* c, int mode, int iters, double* tflops) {
     return rc;
 }
 
-// The Hermitian contraction alone on synthetic operands (a constant byte
-// pattern: finite, nonzero values in every format), for tile and prefetch
-// variants; tile = pf = 0 selects the production choice.
+// The Hermitian contraction alone on synthetic operands (random finite values
+// in [-1, 1] in the mode's storage format), for tile and prefetch variants;
+// tile = pf = 0 selects the production choice.
 extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int units, int tile, int pf, int reps,
                              double* ms_per_launch) {
     if (!c || !ms_per_launch || nant < 1 || nsrc < 1 || units < 1 || reps < 1 || mode < HV_MODE_FP32 ||
@@ -182,8 +208,14 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
     ha.nsplit = nsplit;
     ha.nunits = units;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipMemsetAsync(W, 0x3c, unit_bytes * units, c->stream) != hipSuccess ||
-        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    const size_t nwords = unit_bytes * units / 4;
+    switch (mode) {
+        case HV_MODE_FP32: hipLaunchKernelGGL(fill_kernel<MODE_FP32>, dim3(4096), dim3(256), 0, c->stream, (uint32_t*)W, nwords, 12345u); break;
+        case HV_MODE_SPLIT_BF16: hipLaunchKernelGGL(fill_kernel<MODE_SPLIT_BF16>, dim3(4096), dim3(256), 0, c->stream, (uint32_t*)W, nwords, 12345u); break;
+        case HV_MODE_SPLIT_FP16: hipLaunchKernelGGL(fill_kernel<MODE_SPLIT_FP16>, dim3(4096), dim3(256), 0, c->stream, (uint32_t*)W, nwords, 12345u); break;
+        default: hipLaunchKernelGGL(fill_kernel<MODE_FP64>, dim3(4096), dim3(256), 0, c->stream, (uint32_t*)W, nwords, 12345u); break;
+    }
+    if (hipGetLastError() != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail(HV_ERR_HIP, "setup failed");
     if (rc == HV_OK && launch_herk_variant(mode, tile, pf, ha, c->stream) != hipSuccess)
         rc = fail(HV_ERR_ARG, "no such contraction variant (mode, tile, prefetch)");
@@ -201,4 +233,11 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
     dev_free(c, W, unit_bytes * units);
     dev_free(c, P, part);
     return rc;
+}
+
+extern "C" int hv_set_herk_variant(int tile, int pf) {
+    if (!(tile == 0 || tile == 64 || tile == 128) || pf < 0 || pf > 3)
+        return fail(HV_ERR_ARG, "tile must be 0, 64 or 128 and pf 0..3");
+    set_herk_variant(tile, pf);
+    return HV_OK;
 }

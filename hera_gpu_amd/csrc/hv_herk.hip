@@ -39,16 +39,27 @@ namespace hv {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDS_STRIDE = 144;
 
-// 128x128 tiles halve the operand bytes per MFMA flop, which the split
-// modes need to stay MFMA-bound; FP32 (16x slower MFMA) and FP64 are far
-// from their bandwidth limit at 64x64, and small arrays waste less there.
-int herk_tile(int mode, int nant) { return (nant > 96 && (mode == MODE_SPLIT_BF16 || mode == MODE_SPLIT_FP16)) ? 128 : 64; }
+// 64x64 tiles: measured faster than 128x128 for every mode on MI355X (the
+// 128 variant needs one wave per SIMD), and they waste less on small arrays.
+static int g_tile_override = 0, g_pf_override = 0;   // 0 = built-in choice
+
+void set_herk_variant(int tile, int pf) {
+    g_tile_override = tile;
+    g_pf_override = pf;
+}
+
+int herk_tile(int mode, int nant) {
+    (void)mode;
+    (void)nant;
+    return g_tile_override ? g_tile_override : 64;
+}
 
 // (x, y) pairs of 16-bit values -> (y, -x): rotate each 32-bit word by 16 and
 // flip the sign bit of the upper half.
@@ -118,7 +129,16 @@ struct Cfg {
 
 // MFMAs of one LDS stage for one wave: Re += w_a . w_b and the Im chain(s)
 // for every active subtile (bit i*NS+j of smask).
-template <int MODE, int TILE, bool ALLON>
+// Byte offset of 16-byte chunk q of tile row r in an LDS operand buffer:
+// padded rows (register-staged kernel) or 128-byte rows with the chunk index
+// XOR-swizzled by the row (ring kernel; conflict-free for 8 consecutive rows).
+template <bool SWZ>
+__device__ __forceinline__ int lds_off(int r, int q) {
+    if constexpr (SWZ) return r * STAGE_BYTES + ((q ^ (r & 7)) << 4);
+    else return r * LDS_STRIDE + (q << 4);
+}
+
+template <int MODE, int TILE, bool ALLON, bool SWZ = false>
 __device__ __forceinline__ void stage_mma(const char* __restrict__ As, const char* __restrict__ Bs, int wr, int wc,
                                           int lrow, int lh, unsigned smask,
                                           typename Cfg<MODE, TILE>::ACC (&re)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
@@ -134,8 +154,8 @@ __device__ __forceinline__ void stage_mma(const char* __restrict__ As, const cha
             f32x4 av[NS], bv[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
-                av[i] = *reinterpret_cast<const f32x4*>(As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + q * 32 + lh * 16);
-                bv[i] = *reinterpret_cast<const f32x4*>(Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + q * 32 + lh * 16);
+                av[i] = *reinterpret_cast<const f32x4*>(As + lds_off<SWZ>(wr * WT + i * BLK + lrow, 2 * q + lh));
+                bv[i] = *reinterpret_cast<const f32x4*>(Bs + lds_off<SWZ>(wc * WT + i * BLK + lrow, 2 * q + lh));
             }
 #pragma unroll
             for (int i = 0; i < NS; ++i)
@@ -159,14 +179,13 @@ __device__ __forceinline__ void stage_mma(const char* __restrict__ As, const cha
             u32x4 ahi[NS], alo[NS], phi[NS], plo[NS], bhi[NS], blo[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
-                const char* pa = As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + g * 32;
-                ahi[i] = *reinterpret_cast<const u32x4*>(pa);
-                alo[i] = *reinterpret_cast<const u32x4*>(pa + 16);
+                const int ra = wr * WT + i * BLK + lrow, rb = wc * WT + i * BLK + lrow;
+                ahi[i] = *reinterpret_cast<const u32x4*>(As + lds_off<SWZ>(ra, 2 * g));
+                alo[i] = *reinterpret_cast<const u32x4*>(As + lds_off<SWZ>(ra, 2 * g + 1));
                 phi[i] = rot_neg16(ahi[i]);
                 plo[i] = rot_neg16(alo[i]);
-                const char* pb = Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + g * 32;
-                bhi[i] = *reinterpret_cast<const u32x4*>(pb);
-                blo[i] = *reinterpret_cast<const u32x4*>(pb + 16);
+                bhi[i] = *reinterpret_cast<const u32x4*>(Bs + lds_off<SWZ>(rb, 2 * g));
+                blo[i] = *reinterpret_cast<const u32x4*>(Bs + lds_off<SWZ>(rb, 2 * g + 1));
             }
 #pragma unroll
             for (int i = 0; i < NS; ++i)
@@ -185,8 +204,13 @@ __device__ __forceinline__ void stage_mma(const char* __restrict__ As, const cha
         f64x4 av[NS], bv[NS];
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-            av[i] = *reinterpret_cast<const f64x4*>(As + (wr * WT + i * BLK + lrow) * LDS_STRIDE + lh * 32);
-            bv[i] = *reinterpret_cast<const f64x4*>(Bs + (wc * WT + i * BLK + lrow) * LDS_STRIDE + lh * 32);
+            const int ra = wr * WT + i * BLK + lrow, rb = wc * WT + i * BLK + lrow;
+            const f64x2 a0 = *reinterpret_cast<const f64x2*>(As + lds_off<SWZ>(ra, 2 * lh));
+            const f64x2 a1 = *reinterpret_cast<const f64x2*>(As + lds_off<SWZ>(ra, 2 * lh + 1));
+            const f64x2 b0 = *reinterpret_cast<const f64x2*>(Bs + lds_off<SWZ>(rb, 2 * lh));
+            const f64x2 b1 = *reinterpret_cast<const f64x2*>(Bs + lds_off<SWZ>(rb, 2 * lh + 1));
+            av[i] = f64x4{a0[0], a0[1], a1[0], a1[1]};
+            bv[i] = f64x4{b0[0], b0[1], b1[0], b1[1]};
         }
 #pragma unroll
         for (int i = 0; i < NS; ++i)
@@ -453,6 +477,194 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
         }
 }
 
+// ---------------------------------------------------------------------------
+// Ring variant: operands go from global memory straight into LDS
+// (global_load_lds_dwordx4, no staging registers) through a ring of RING
+// slots, issued RING-1 stages ahead.  Every stage issues the same number of
+// loads (past the last stage the last one is re-read into a slot nobody reads
+// again), so one explicit `s_waitcnt vmcnt` count retires exactly the stage
+// about to be used; a raw s_barrier (no fence, so the in-flight loads stay in
+// flight) then publishes it to the other waves.  Each slot is its own LDS
+// array so the compiler can tell the slots apart and adds no waits of its
+// own.  Rows are 128 bytes with XOR-swizzled 16-byte chunks.  Rows past the
+// last antenna load the last antenna's row: their MFMA outputs land only in
+// rows/columns that are never read.
+constexpr int RING = 4;
+
+template <int TILE>
+struct RingLds {
+    static constexpr int OP = TILE * STAGE_BYTES;      // bytes of one operand tile
+};
+
+// vmcnt(n) with no wait on the other counters (gfx9 encoding: vmcnt[3:0] in
+// bits 3:0, vmcnt[5:4] in bits 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt out of range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+#define HV_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+
+// Issue the direct-to-LDS loads of one stage (A and B tiles) into `slot`.
+template <int TILE>
+__device__ __forceinline__ void ring_issue(char* slot, const char* gA, const char* gB, const int* offA, const int* offB,
+                                           int wave) {
+    constexpr int NCH = TILE * (STAGE_BYTES / 16) / 256;
+    constexpr int OP = RingLds<TILE>::OP;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        char* dst = slot + (c * 4 + wave) * 1024;     // 64 lanes x 16 B, lane-contiguous
+        __builtin_amdgcn_global_load_lds(gA + offA[c], HV_LDS(dst), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(gB + offB[c], HV_LDS(dst + OP), 16, 0, 0);
+    }
+}
+
+template <int MODE, int TILE>
+__global__ __launch_bounds__(256) void herk_ring_kernel(HerkArgs a) {
+    using C = Cfg<MODE, TILE>;
+    constexpr int NS = C::NS, BLK = C::BLK, WT = C::WT, NCH = C::NCH, NACC = C::NACC;
+    constexpr int OP = RingLds<TILE>::OP;
+    constexpr int PER_STAGE = 2 * NCH;                 // loads per thread per stage
+    using ACC = typename C::ACC;
+    using AR = typename C::AR;
+
+    __shared__ __attribute__((aligned(1024))) char slot0[2 * OP];
+    __shared__ __attribute__((aligned(1024))) char slot1[2 * OP];
+    __shared__ __attribute__((aligned(1024))) char slot2[2 * OP];
+    __shared__ __attribute__((aligned(1024))) char slot3[2 * OP];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+
+    int unit, split, tile, bi, bj;
+    decode_work(a, blockIdx.x, unit, split, tile);
+    decode_tile(tile, a.nb, bi, bj);
+    const bool diag = (bi == bj);
+
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / C::SRC_STAGE);
+    const int s_begin = split * C::STAGES_SPLIT;
+    const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
+    if (s_begin >= s_end) return;   // split beyond the job's sources (uniform per workgroup)
+    const int n_st = s_end - s_begin;  // a multiple of 4
+
+    unsigned smask = 0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int r0 = wr * WT + i * BLK, c0 = wc * WT + j * BLK;
+            const bool on = (bi * TILE + r0 < a.nant) && (bj * TILE + c0 < a.nant) && !(diag && r0 > c0);
+            smask |= (unsigned)on << (i * NS + j);
+        }
+    smask = __builtin_amdgcn_readfirstlane(smask);
+
+    // global offsets of this lane's chunks: chunk id (c*4 + wave)*64 + lane
+    // lands at LDS byte 16*id = row*128 + 16*(lane & 7), so it must hold the
+    // row's logical chunk (lane & 7) ^ (row & 7)
+    const char* Wu = reinterpret_cast<const char*>(a.W) + (size_t)unit * a.unit_stride;
+    const size_t stage_bytes = (size_t)a.nant * STAGE_BYTES;
+    const char* gA = Wu + (size_t)bi * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
+    const char* gB = Wu + (size_t)bj * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
+    int offA[NCH], offB[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int r = (c * 4 + wave) * 8 + (lane >> 3);
+        const int q = (lane & 7) ^ (r & 7);
+        offA[c] = (min(bi * TILE + r, a.nant - 1) - bi * TILE) * STAGE_BYTES + q * 16;
+        offB[c] = (min(bj * TILE + r, a.nant - 1) - bj * TILE) * STAGE_BYTES + q * 16;
+    }
+    char* slots[RING] = {slot0, slot1, slot2, slot3};
+    auto stage_ptrA = [&](int st) { return gA + (size_t)min(st, n_st - 1) * stage_bytes; };
+    auto stage_ptrB = [&](int st) { return gB + (size_t)min(st, n_st - 1) * stage_bytes; };
+    ring_issue<TILE>(slot0, stage_ptrA(0), stage_ptrB(0), offA, offB, wave);
+    ring_issue<TILE>(slot1, stage_ptrA(1), stage_ptrB(1), offA, offB, wave);
+    ring_issue<TILE>(slot2, stage_ptrA(2), stage_ptrB(2), offA, offB, wave);
+
+    const int lrow = C::F64 ? (lane & 15) : (lane & 31);
+    const int lh = C::F64 ? (lane >> 4) : (lane >> 5);
+
+    ACC tre[NS][NS], tim[NS][NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+#pragma unroll
+            for (int e = 0; e < NACC; ++e) { tre[i][j][e] = 0; tim[i][j][e] = 0; }
+
+    const unsigned full = (1u << (NS * NS)) - 1;
+    for (int st = 0; st < n_st;) {
+        const int g_end = min(st + C::FOLD, n_st);
+        ACC re[NS][NS], im[NS][NS], p2[NS][NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+#pragma unroll
+                for (int e = 0; e < NACC; ++e) { re[i][j][e] = 0; im[i][j][e] = 0; p2[i][j][e] = 0; }
+        for (; st < g_end; st += RING) {
+#pragma unroll
+            for (int k = 0; k < RING; ++k) {
+                wait_vmcnt<(RING - 2) * PER_STAGE>();      // stage st+k has landed (this wave)
+                __builtin_amdgcn_s_barrier();              // ... for every wave; slot k-1 is free
+                ring_issue<TILE>(slots[(k + RING - 1) % RING], stage_ptrA(st + k + RING - 1),
+                                 stage_ptrB(st + k + RING - 1), offA, offB, wave);
+                const char* As = slots[k];
+                if (smask == full)
+                    stage_mma<MODE, TILE, true, true>(As, As + OP, wr, wc, lrow, lh, smask, re, im, p2);
+                else if (smask)
+                    stage_mma<MODE, TILE, false, true>(As, As + OP, wr, wc, lrow, lh, smask, re, im, p2);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                tre[i][j] += re[i][j];
+                if constexpr (C::PAIR_IM) tim[i][j] += im[i][j] - p2[i][j];   // P1 - P2
+                else tim[i][j] += im[i][j];
+            }
+    }
+    wait_vmcnt<0>();                                   // no load may outlive the workgroup
+    if (!smask) return;
+
+    constexpr size_t TSZ = (size_t)TILE * TILE;
+    AR* out = reinterpret_cast<AR*>(a.partials) +
+              (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ;
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (!(smask & (1u << (i * NS + j)))) continue;
+#pragma unroll
+            for (int e = 0; e < NACC; ++e) {
+                int r, c;
+                if constexpr (C::F64) {
+                    r = (lane >> 4) + 4 * e;
+                    c = lane & 15;
+                } else {
+                    r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                    c = lane & 31;
+                }
+                const int row = wr * WT + i * BLK + r;
+                const int col = wc * WT + j * BLK + c;
+                out[(size_t)row * TILE + col] = tre[i][j][e];
+                out[TSZ + (size_t)row * TILE + col] = tim[i][j][e];
+            }
+        }
+}
+#undef HV_LDS
+
+template <int MODE, int TILE>
+static hipError_t launch_herk_ring_t(const HerkArgs& a, hipStream_t s) {
+    const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL((herk_ring_kernel<MODE, TILE>), dim3((unsigned)nblk), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <int MODE, int TILE, int PF>
 static hipError_t launch_herk_t(const HerkArgs& a, hipStream_t s) {
     const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
@@ -472,10 +684,23 @@ hipError_t launch_herk_variant(int mode, int tile, int pf, const HerkArgs& a, hi
     HV_V(MODE_SPLIT_FP16, 128, 1); HV_V(MODE_SPLIT_FP16, 128, 2);
     HV_V(MODE_FP64, 64, 1);
 #undef HV_V
+    // pf = 3: ring variant (direct-to-LDS loads, 3 stages ahead)
+    if (pf == 3) {
+        if (mode == MODE_FP32 && tile == 64) return launch_herk_ring_t<MODE_FP32, 64>(a, s);
+        if (mode == MODE_SPLIT_BF16 && tile == 64) return launch_herk_ring_t<MODE_SPLIT_BF16, 64>(a, s);
+        if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_herk_ring_t<MODE_SPLIT_BF16, 128>(a, s);
+        if (mode == MODE_SPLIT_FP16 && tile == 64) return launch_herk_ring_t<MODE_SPLIT_FP16, 64>(a, s);
+        if (mode == MODE_SPLIT_FP16 && tile == 128) return launch_herk_ring_t<MODE_SPLIT_FP16, 128>(a, s);
+        if (mode == MODE_FP64 && tile == 64) return launch_herk_ring_t<MODE_FP64, 64>(a, s);
+    }
     return hipErrorInvalidValue;
 }
 
-int herk_prefetch(int mode, int tile) { return (mode == MODE_FP64 || tile == 128) ? 1 : 2; }
+int herk_prefetch(int mode, int tile) {
+    (void)mode;
+    (void)tile;
+    return g_pf_override ? g_pf_override : 1;
+}
 
 hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {
     return launch_herk_variant(mode, tile, herk_prefetch(mode, tile), a, s);

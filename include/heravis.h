@@ -161,6 +161,10 @@ int hv_bench_mfma(hv_ctx* ctx, int mode, int iters, double* tflops);
  * 0 = the production choice.  Returns ms per launch. */
 int hv_bench_herk(hv_ctx* ctx, int mode, int nant, int64_t nsrc, int units, int tile, int pf, int reps,
                   double* ms_per_launch);
+/* Process-wide choice of contraction variant for later calls (tuning and
+ * tests): tile 64/128, pf 1/2 (register-staged prefetch depth) or 3 (LDS
+ * ring); 0 restores the built-in choice.  Not safe while other calls run. */
+int hv_set_herk_variant(int tile, int pf);
 
 /* ---------------------------------------------------------------------------
  * Serving (no reference counterpart: the reference has no request loop).

@@ -152,3 +152,30 @@ def test_herk_colocated_is_exact(gemm):
     V = stages.herk(v, gemm=gemm)
     assert np.all(V.imag == 0)
     assert np.all(V.real == K)
+
+
+# every contraction variant (tile x prefetch scheme) against FP64 NumPy
+VARIANTS = [(64, 1), (64, 2), (64, 3), (128, 1), (128, 2)]
+
+
+@pytest.mark.parametrize("tile,pf", VARIANTS)
+@pytest.mark.parametrize("gemm", MODES)
+def test_herk_variants(gemm, tile, pf):
+    from hera_gpu_amd import _lib
+    if tile == 128 and gemm in ("fp32", "fp64"):
+        pytest.skip("128-tiles exist for the split modes only")
+    if gemm == "fp64" and pf == 2:
+        pytest.skip("FP64 has prefetch depths 1 and 3")
+    lib = _lib.load()
+    _lib.check(lib.hv_set_herk_variant(tile, pf))
+    try:
+        for N, K in ((37, 1000), (130, 20000), (350, 8192 * 2 + 64)):
+            rng = np.random.default_rng(N + K + tile + pf)
+            cd = np.complex128 if gemm == "fp64" else np.complex64
+            v = ((rng.standard_normal((N, K)) + 1j * rng.standard_normal((N, K))) * rng.uniform(0.01, 3, K)).astype(cd)
+            V = stages.herk(v, gemm=gemm)
+            ref = v.astype(np.complex128) @ v.astype(np.complex128).conj().T
+            assert vo.rel_err(V, ref) < HERK_TOL[gemm], (N, K)
+            np.testing.assert_array_equal(V, V.conj().T)
+    finally:
+        _lib.check(lib.hv_set_herk_variant(0, 0))
