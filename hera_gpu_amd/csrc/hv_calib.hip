@@ -236,8 +236,8 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
 }
 
 extern "C" int hv_set_herk_variant(int tile, int pf) {
-    if (!(tile == 0 || tile == 64 || tile == 128) || pf < 0 || pf > 3)
-        return fail(HV_ERR_ARG, "tile must be 0, 64 or 128 and pf 0..3");
+    if (!(tile == 0 || tile == 64 || tile == 128) || pf < 0 || pf > 4)
+        return fail(HV_ERR_ARG, "tile must be 0, 64 or 128 and pf 0..4");
     set_herk_variant(tile, pf);
     return HV_OK;
 }

@@ -46,8 +46,12 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDS_STRIDE = 144;
 
-// 64x64 tiles: measured faster than 128x128 for every mode on MI355X (the
-// 128 variant needs one wave per SIMD), and they waste less on small arrays.
+// Built-in variants (measured on MI355X, 8 units x 2^18 sources): split
+// modes at 256+ antennas take 128-tiles on the persistent LDS-ring grid (the
+// tiles of a split share their operand stages in L2: HBM reads ~1.1x the
+// operand bytes against ~3.5x for 64-tiles); smaller arrays take 64-tiles on
+// the LDS ring, which waste less on the triangle's edge.  FP32 and FP64 keep
+// the register-staged 64-tile kernel, their fastest.
 static int g_tile_override = 0, g_pf_override = 0;   // 0 = built-in choice
 
 void set_herk_variant(int tile, int pf) {
@@ -55,10 +59,11 @@ void set_herk_variant(int tile, int pf) {
     g_pf_override = pf;
 }
 
+static bool split_mode(int mode) { return mode == MODE_SPLIT_BF16 || mode == MODE_SPLIT_FP16; }
+
 int herk_tile(int mode, int nant) {
-    (void)mode;
-    (void)nant;
-    return g_tile_override ? g_tile_override : 64;
+    if (g_tile_override) return g_tile_override;
+    return (split_mode(mode) && nant >= 256) ? 128 : 64;
 }
 
 // (x, y) pairs of 16-bit values -> (y, -x): rotate each 32-bit word by 16 and
@@ -520,12 +525,262 @@ __device__ __forceinline__ void ring_issue(char* slot, const char* gA, const cha
     }
 }
 
+// s_waitcnt with vmcnt(VM) and lgkmcnt(LGKM), expcnt left alone.
+template <int VM, int LGKM>
+__device__ __forceinline__ void wait_vm_lgkm() {
+    static_assert(VM >= 0 && VM < 64 && LGKM >= 0 && LGKM < 16, "count out of range");
+    __builtin_amdgcn_s_waitcnt((VM & 0xF) | ((VM >> 4) << 14) | (0x7 << 4) | (LGKM << 8));
+}
+
+// Split-mode operand fragments of one 8-source MFMA step for one wave: hi
+// and lo words of the wave's A rows and B rows (swizzled ring layout).
+template <int NS>
+struct SplitFrag {
+    u32x4 ahi[NS], alo[NS], bhi[NS], blo[NS];
+};
+
+// Read the fragments of source group g (chunks 2g = hi, 2g+1 = lo).
+template <int NS>
+__device__ __forceinline__ void read_frag(SplitFrag<NS>& f, const char* As, const char* Bs, int ra0, int rb0, int g) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        f.ahi[i] = *reinterpret_cast<const u32x4*>(As + lds_off<true>(ra0 + 32 * i, 2 * g));
+        f.alo[i] = *reinterpret_cast<const u32x4*>(As + lds_off<true>(ra0 + 32 * i, 2 * g + 1));
+        f.bhi[i] = *reinterpret_cast<const u32x4*>(Bs + lds_off<true>(rb0 + 32 * i, 2 * g));
+        f.blo[i] = *reinterpret_cast<const u32x4*>(Bs + lds_off<true>(rb0 + 32 * i, 2 * g + 1));
+    }
+}
+
+// Re += a.b and Im += a'.b with a' = (y, -x), as hi.hi + hi.lo + lo.hi.
+template <int MODE, int NS, bool ALLON>
+__device__ __forceinline__ void frag_mma(const SplitFrag<NS>& f, unsigned smask, f32x16 (&re)[NS][NS],
+                                         f32x16 (&im)[NS][NS]) {
+    u32x4 phi[NS], plo[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        phi[i] = rot_neg16(f.ahi[i]);
+        plo[i] = rot_neg16(f.alo[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+            if (ALLON || (smask & (1u << (i * NS + j)))) {
+                re[i][j] = mfma16<MODE>(f.ahi[i], f.bhi[j], re[i][j]);
+                im[i][j] = mfma16<MODE>(phi[i], f.bhi[j], im[i][j]);
+                re[i][j] = mfma16<MODE>(f.ahi[i], f.blo[j], re[i][j]);
+                im[i][j] = mfma16<MODE>(phi[i], f.blo[j], im[i][j]);
+                re[i][j] = mfma16<MODE>(f.alo[i], f.bhi[j], re[i][j]);
+                im[i][j] = mfma16<MODE>(plo[i], f.bhi[j], im[i][j]);
+            }
+}
+
+// Scheduling pattern for one MFMA group of NMF MFMAs issued together with
+// NV vector-memory loads and ND LDS reads: one MFMA, then one load and one
+// read, and so on, so the wave never leaves the matrix pipe idle while it
+// issues memory instructions (sched_group_barrier masks: 0x008 MFMA, 0x020
+// VMEM read, 0x100 DS read).
+template <int NMF, int NV, int ND>
+__device__ __forceinline__ void interleave_mfma() {
+#pragma unroll
+    for (int i = 0; i < NMF; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < NV) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        if (i < ND) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+}
+
+// One split of the ring loop (stages [0, n_st) of the split at gA/gB).
+// KIND 0: the wave only issues its share of the loads; 1: it computes every
+// block of its wave tile (blocks outside smask compute values nobody
+// writes); 2: only the blocks of smask.  The kind is fixed per wave before
+// the loop, so the MFMA chains have no branches and the accumulators stay in
+// AGPRs.  On exit every load of the split has landed and every wave has
+// passed its last LDS read, so the slots can take the next split.
+template <int MODE, int TILE, int KIND>
+__device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s3, const char* gA, const char* gB,
+                                           size_t stage_bytes, int n_st, const int* offA, const int* offB, int wave,
+                                           int wr, int wc, int lrow, int lh, unsigned smask,
+                                           typename Cfg<MODE, TILE>::ACC (&tre)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
+                                           typename Cfg<MODE, TILE>::ACC (&tim)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS]) {
+    using C = Cfg<MODE, TILE>;
+    constexpr int NS = C::NS, NACC = C::NACC, NCH = C::NCH;
+    constexpr int OP = RingLds<TILE>::OP;
+    constexpr int PER_STAGE = 2 * NCH;                 // loads per thread per stage
+    using ACC = typename C::ACC;
+    char* slots[RING] = {s0, s1, s2, s3};
+    // stages past the last one re-read the last stage (keeps the count per
+    // stage constant for the vmcnt arithmetic)
+#define HV_ISSUE(slot, st)                                                                    \
+    ring_issue<TILE>(slot, gA + (size_t)min(st, n_st - 1) * stage_bytes,                      \
+                     gB + (size_t)min(st, n_st - 1) * stage_bytes, offA, offB, wave)
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+#pragma unroll
+            for (int e = 0; e < NACC; ++e) { tre[i][j][e] = 0; tim[i][j][e] = 0; }
+
+    if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
+        // Software-pipelined split modes: a stage is two 8-source steps; the
+        // LDS reads of the next step are issued before the MFMAs of the
+        // current one.  The stage barrier sits between the two steps: by then
+        // every wave has its reads of stage k back (lgkmcnt 0), so slot k
+        // takes stage k+RING, and stage k+1 has landed for every wave.  The
+        // ring starts RING stages deep; in steady state stages k+2..k+RING
+        // are in flight at the wait.
+        const int ra0 = wr * C::WT + lrow, rb0 = wc * C::WT + lrow;
+        HV_ISSUE(s0, 0);
+        HV_ISSUE(s1, 1);
+        HV_ISSUE(s2, 2);
+        HV_ISSUE(s3, 3);
+        wait_vmcnt<(RING - 1) * PER_STAGE>();
+        __builtin_amdgcn_s_barrier();
+        // 64-tiles: groups of FOLD_SRC sources accumulate from zero and are
+        // added to the FP32 totals; 128-tiles (one wave per SIMD, no
+        // registers for a second set of accumulators) run one chain per split
+        // straight into the totals.
+        constexpr int RFOLD = (TILE == 64) ? FOLD_SRC / C::SRC_STAGE : C::STAGES_SPLIT;
+        constexpr int NMF = 6 * NS * NS;               // MFMAs per 8-source step
+        constexpr int NRD = 4 * NS;                    // LDS reads per 8-source step
+        SplitFrag<NS> fa, fb;
+        read_frag<NS>(fa, s0, s0 + OP, ra0, rb0, lh);
+        for (int st = 0; st < n_st;) {
+            const int g_end = min(st + RFOLD, n_st);
+            ACC gre[NS][NS], gim[NS][NS];
+            if constexpr (TILE == 64) {
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int j = 0; j < NS; ++j)
+#pragma unroll
+                        for (int e = 0; e < NACC; ++e) { gre[i][j][e] = 0; gim[i][j][e] = 0; }
+            }
+            ACC (&re)[NS][NS] = (TILE == 64) ? gre : tre;
+            ACC (&im)[NS][NS] = (TILE == 64) ? gim : tim;
+            for (; st < g_end; st += RING) {
+#pragma unroll
+                for (int k = 0; k < RING; ++k) {
+                    char* cur = slots[k];
+                    char* nxt = slots[(k + 1) % RING];
+                    // fa's reads went out a whole MFMA group ago: retire them
+                    // before the next reads, so no wait below has to let
+                    // younger reads stay in flight
+                    wait_vm_lgkm<63, 0>();
+                    __builtin_amdgcn_sched_barrier(0);
+                    read_frag<NS>(fb, cur, cur + OP, ra0, rb0, 2 + lh);
+                    if constexpr (KIND != 0) {
+                        frag_mma<MODE, NS, KIND == 1>(fa, smask, re, im);
+                        interleave_mfma<NMF, 0, NRD>();
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                    wait_vm_lgkm<(RING - 2) * PER_STAGE, 0>();
+                    __builtin_amdgcn_s_barrier();
+                    __builtin_amdgcn_sched_barrier(0);
+                    HV_ISSUE(cur, st + k + RING);
+                    read_frag<NS>(fa, nxt, nxt + OP, ra0, rb0, lh);   // past the end: a re-read, unused
+                    if constexpr (KIND != 0) {
+                        frag_mma<MODE, NS, KIND == 1>(fb, smask, re, im);
+                        interleave_mfma<NMF, PER_STAGE, NRD>();
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            if constexpr (KIND != 0 && TILE == 64) {
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) {
+                        tre[i][j] += gre[i][j];
+                        tim[i][j] += gim[i][j];
+                    }
+            }
+        }
+        wait_vm_lgkm<0, 0>();
+        __builtin_amdgcn_s_barrier();
+        return;
+    }
+
+    HV_ISSUE(s0, 0);
+    HV_ISSUE(s1, 1);
+    HV_ISSUE(s2, 2);
+
+    if constexpr (C::FOLD_ON) {
+        // groups of FOLD stages accumulate from zero and are added to the
+        // FP32 totals
+        for (int st = 0; st < n_st;) {
+            const int g_end = min(st + C::FOLD, n_st);
+            ACC re[NS][NS], im[NS][NS], p2[NS][NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j)
+#pragma unroll
+                    for (int e = 0; e < NACC; ++e) { re[i][j][e] = 0; im[i][j][e] = 0; p2[i][j][e] = 0; }
+            for (; st < g_end; st += RING) {
+#pragma unroll
+                for (int k = 0; k < RING; ++k) {
+                    wait_vmcnt<(RING - 2) * PER_STAGE>();  // stage st+k has landed (this wave)
+                    __builtin_amdgcn_s_barrier();          // ... for every wave; slot k-1 is free
+                    HV_ISSUE(slots[(k + RING - 1) % RING], st + k + RING - 1);
+                    if constexpr (KIND != 0)
+                        stage_mma<MODE, TILE, KIND == 1, true>(slots[k], slots[k] + OP, wr, wc, lrow, lh, smask, re,
+                                                               im, p2);
+                }
+            }
+            if constexpr (KIND != 0) {
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) {
+                        tre[i][j] += re[i][j];
+                        if constexpr (C::PAIR_IM) tim[i][j] += im[i][j] - p2[i][j];   // P1 - P2
+                        else tim[i][j] += im[i][j];
+                    }
+            }
+        }
+    } else {
+        // one MFMA chain per split, straight into the totals
+        ACC p2[NS][NS];
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+#pragma unroll
+                for (int e = 0; e < NACC; ++e) p2[i][j][e] = 0;
+        for (int st = 0; st < n_st; st += RING) {
+#pragma unroll
+            for (int k = 0; k < RING; ++k) {
+                wait_vmcnt<(RING - 2) * PER_STAGE>();
+                __builtin_amdgcn_s_barrier();
+                HV_ISSUE(slots[(k + RING - 1) % RING], st + k + RING - 1);
+                if constexpr (KIND != 0)
+                    stage_mma<MODE, TILE, KIND == 1, true>(slots[k], slots[k] + OP, wr, wc, lrow, lh, smask, tre, tim,
+                                                           p2);
+            }
+        }
+        if constexpr (C::PAIR_IM && KIND != 0) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) tim[i][j] -= p2[i][j];
+        }
+    }
+#undef HV_ISSUE
+    wait_vmcnt<0>();                 // the re-reads past the end have landed too
+    __builtin_amdgcn_s_barrier();    // every wave is done with the slots
+}
+
+// Work item L (a (unit, split, tile) triple, numbered as decode_work reads
+// it) per loop trip; the grid is either one workgroup per item or a
+// persistent grid of a multiple of 8 workgroups, which keeps every
+// workgroup's items on its own XCD and the tiles of one split running
+// side by side (their shared operand stages hit in L2).
 template <int MODE, int TILE>
 __global__ __launch_bounds__(256) void herk_ring_kernel(HerkArgs a) {
     using C = Cfg<MODE, TILE>;
     constexpr int NS = C::NS, BLK = C::BLK, WT = C::WT, NCH = C::NCH, NACC = C::NACC;
     constexpr int OP = RingLds<TILE>::OP;
-    constexpr int PER_STAGE = 2 * NCH;                 // loads per thread per stage
     using ACC = typename C::ACC;
     using AR = typename C::AR;
 
@@ -538,130 +793,120 @@ __global__ __launch_bounds__(256) void herk_ring_kernel(HerkArgs a) {
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-
-    int unit, split, tile, bi, bj;
-    decode_work(a, blockIdx.x, unit, split, tile);
-    decode_tile(tile, a.nb, bi, bj);
-    const bool diag = (bi == bj);
-
-    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / C::SRC_STAGE);
-    const int s_begin = split * C::STAGES_SPLIT;
-    const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
-    if (s_begin >= s_end) return;   // split beyond the job's sources (uniform per workgroup)
-    const int n_st = s_end - s_begin;  // a multiple of 4
-
-    unsigned smask = 0;
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int r0 = wr * WT + i * BLK, c0 = wc * WT + j * BLK;
-            const bool on = (bi * TILE + r0 < a.nant) && (bj * TILE + c0 < a.nant) && !(diag && r0 > c0);
-            smask |= (unsigned)on << (i * NS + j);
-        }
-    smask = __builtin_amdgcn_readfirstlane(smask);
-
-    // global offsets of this lane's chunks: chunk id (c*4 + wave)*64 + lane
-    // lands at LDS byte 16*id = row*128 + 16*(lane & 7), so it must hold the
-    // row's logical chunk (lane & 7) ^ (row & 7)
-    const char* Wu = reinterpret_cast<const char*>(a.W) + (size_t)unit * a.unit_stride;
-    const size_t stage_bytes = (size_t)a.nant * STAGE_BYTES;
-    const char* gA = Wu + (size_t)bi * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
-    const char* gB = Wu + (size_t)bj * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
-    int offA[NCH], offB[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        const int r = (c * 4 + wave) * 8 + (lane >> 3);
-        const int q = (lane & 7) ^ (r & 7);
-        offA[c] = (min(bi * TILE + r, a.nant - 1) - bi * TILE) * STAGE_BYTES + q * 16;
-        offB[c] = (min(bj * TILE + r, a.nant - 1) - bj * TILE) * STAGE_BYTES + q * 16;
-    }
-    char* slots[RING] = {slot0, slot1, slot2, slot3};
-    auto stage_ptrA = [&](int st) { return gA + (size_t)min(st, n_st - 1) * stage_bytes; };
-    auto stage_ptrB = [&](int st) { return gB + (size_t)min(st, n_st - 1) * stage_bytes; };
-    ring_issue<TILE>(slot0, stage_ptrA(0), stage_ptrB(0), offA, offB, wave);
-    ring_issue<TILE>(slot1, stage_ptrA(1), stage_ptrB(1), offA, offB, wave);
-    ring_issue<TILE>(slot2, stage_ptrA(2), stage_ptrB(2), offA, offB, wave);
-
     const int lrow = C::F64 ? (lane & 15) : (lane & 31);
     const int lh = C::F64 ? (lane >> 4) : (lane >> 5);
-
-    ACC tre[NS][NS], tim[NS][NS];
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
-#pragma unroll
-        for (int j = 0; j < NS; ++j)
-#pragma unroll
-            for (int e = 0; e < NACC; ++e) { tre[i][j][e] = 0; tim[i][j][e] = 0; }
-
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / C::SRC_STAGE);
+    const int64_t nitems = (int64_t)a.nunits * a.nsplit * a.ntiles;
+    const size_t stage_bytes = (size_t)a.nant * STAGE_BYTES;
     const unsigned full = (1u << (NS * NS)) - 1;
-    for (int st = 0; st < n_st;) {
-        const int g_end = min(st + C::FOLD, n_st);
-        ACC re[NS][NS], im[NS][NS], p2[NS][NS];
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-#pragma unroll
-            for (int j = 0; j < NS; ++j)
-#pragma unroll
-                for (int e = 0; e < NACC; ++e) { re[i][j][e] = 0; im[i][j][e] = 0; p2[i][j][e] = 0; }
-        for (; st < g_end; st += RING) {
-#pragma unroll
-            for (int k = 0; k < RING; ++k) {
-                wait_vmcnt<(RING - 2) * PER_STAGE>();      // stage st+k has landed (this wave)
-                __builtin_amdgcn_s_barrier();              // ... for every wave; slot k-1 is free
-                ring_issue<TILE>(slots[(k + RING - 1) % RING], stage_ptrA(st + k + RING - 1),
-                                 stage_ptrB(st + k + RING - 1), offA, offB, wave);
-                const char* As = slots[k];
-                if (smask == full)
-                    stage_mma<MODE, TILE, true, true>(As, As + OP, wr, wc, lrow, lh, smask, re, im, p2);
-                else if (smask)
-                    stage_mma<MODE, TILE, false, true>(As, As + OP, wr, wc, lrow, lh, smask, re, im, p2);
-            }
-        }
+
+    for (int64_t L = blockIdx.x; L < nitems; L += gridDim.x) {
+        int unit, split, tile, bi, bj;
+        decode_work(a, L, unit, split, tile);
+        decode_tile(tile, a.nb, bi, bj);
+        const bool diag = (bi == bj);
+        const int s_begin = split * C::STAGES_SPLIT;
+        const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
+        if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
+        const int n_st = s_end - s_begin; // a multiple of RING
+
+        unsigned smask = 0;
 #pragma unroll
         for (int i = 0; i < NS; ++i)
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
-                tre[i][j] += re[i][j];
-                if constexpr (C::PAIR_IM) tim[i][j] += im[i][j] - p2[i][j];   // P1 - P2
-                else tim[i][j] += im[i][j];
+                const int r0 = wr * WT + i * BLK, c0 = wc * WT + j * BLK;
+                const bool on = (bi * TILE + r0 < a.nant) && (bj * TILE + c0 < a.nant) && !(diag && r0 > c0);
+                smask |= (unsigned)on << (i * NS + j);
+            }
+        smask = __builtin_amdgcn_readfirstlane(smask);
+
+        // global offsets of this lane's chunks: chunk id (c*4 + wave)*64 + lane
+        // lands at LDS byte 16*id = row*128 + 16*(lane & 7), so it must hold
+        // the row's logical chunk (lane & 7) ^ (row & 7)
+        const char* Wu = reinterpret_cast<const char*>(a.W) + (size_t)unit * a.unit_stride;
+        const char* gA = Wu + (size_t)bi * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
+        const char* gB = Wu + (size_t)bj * TILE * STAGE_BYTES + (size_t)s_begin * stage_bytes;
+        int offA[NCH], offB[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const int r = (c * 4 + wave) * 8 + (lane >> 3);
+            const int q = (lane & 7) ^ (r & 7);
+            offA[c] = (min(bi * TILE + r, a.nant - 1) - bi * TILE) * STAGE_BYTES + q * 16;
+            offB[c] = (min(bj * TILE + r, a.nant - 1) - bj * TILE) * STAGE_BYTES + q * 16;
+        }
+
+        ACC tre[NS][NS], tim[NS][NS];
+        // 128-tiles run one wave per SIMD: a wave with a partial mask
+        // computes its whole wave tile, which costs the workgroup no time
+        // (another wave of the tile has a full mask or as much work)
+        if (smask == full || (TILE == 128 && smask))
+            ring_split<MODE, TILE, 1>(slot0, slot1, slot2, slot3, gA, gB, stage_bytes, n_st, offA, offB, wave, wr, wc,
+                                      lrow, lh, smask, tre, tim);
+        else if (smask)
+            ring_split<MODE, TILE, 2>(slot0, slot1, slot2, slot3, gA, gB, stage_bytes, n_st, offA, offB, wave, wr, wc,
+                                      lrow, lh, smask, tre, tim);
+        else
+            ring_split<MODE, TILE, 0>(slot0, slot1, slot2, slot3, gA, gB, stage_bytes, n_st, offA, offB, wave, wr, wc,
+                                      lrow, lh, smask, tre, tim);
+        if (!smask) continue;
+
+        constexpr size_t TSZ = (size_t)TILE * TILE;
+        AR* out = reinterpret_cast<AR*>(a.partials) +
+                  (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ;
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                if (!(smask & (1u << (i * NS + j)))) continue;
+#pragma unroll
+                for (int e = 0; e < NACC; ++e) {
+                    int r, c;
+                    if constexpr (C::F64) {
+                        r = (lane >> 4) + 4 * e;
+                        c = lane & 15;
+                    } else {
+                        r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                        c = lane & 31;
+                    }
+                    const int row = wr * WT + i * BLK + r;
+                    const int col = wc * WT + j * BLK + c;
+                    out[(size_t)row * TILE + col] = tre[i][j][e];
+                    out[TSZ + (size_t)row * TILE + col] = tim[i][j][e];
+                }
             }
     }
-    wait_vmcnt<0>();                                   // no load may outlive the workgroup
-    if (!smask) return;
-
-    constexpr size_t TSZ = (size_t)TILE * TILE;
-    AR* out = reinterpret_cast<AR*>(a.partials) +
-              (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ;
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            if (!(smask & (1u << (i * NS + j)))) continue;
-#pragma unroll
-            for (int e = 0; e < NACC; ++e) {
-                int r, c;
-                if constexpr (C::F64) {
-                    r = (lane >> 4) + 4 * e;
-                    c = lane & 15;
-                } else {
-                    r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-                    c = lane & 31;
-                }
-                const int row = wr * WT + i * BLK + r;
-                const int col = wc * WT + j * BLK + c;
-                out[(size_t)row * TILE + col] = tre[i][j][e];
-                out[TSZ + (size_t)row * TILE + col] = tim[i][j][e];
-            }
-        }
 }
 #undef HV_LDS
 
+// Resident workgroups of a ring kernel on the whole device (a multiple of 8),
+// or 0 if the runtime cannot say.
+// Computed once per process (thread-safe static initialisation), for the
+// device current at the first call; the library drives one device type.
 template <int MODE, int TILE>
-static hipError_t launch_herk_ring_t(const HerkArgs& a, hipStream_t s) {
+static int64_t ring_resident() {
+    static const int64_t cached = [] {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_ring_kernel<MODE, TILE>, 256, 0) != hipSuccess)
+            return (int64_t)0;
+        return (int64_t)ncu * per_cu / 8 * 8;
+    }();
+    return cached;
+}
+
+// pf 3: one workgroup per work item; pf 4: persistent grid.
+template <int MODE, int TILE>
+static hipError_t launch_herk_ring_t(const HerkArgs& a, bool persistent, hipStream_t s) {
     const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
     if (nblk <= 0) return hipSuccess;
-    hipLaunchKernelGGL((herk_ring_kernel<MODE, TILE>), dim3((unsigned)nblk), dim3(256), 0, s, a);
+    int64_t grid = nblk;
+    if (persistent) {
+        const int64_t res = ring_resident<MODE, TILE>();
+        if (res > 0 && res < nblk) grid = res;
+    }
+    hipLaunchKernelGGL((herk_ring_kernel<MODE, TILE>), dim3((unsigned)grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -684,22 +929,33 @@ hipError_t launch_herk_variant(int mode, int tile, int pf, const HerkArgs& a, hi
     HV_V(MODE_SPLIT_FP16, 128, 1); HV_V(MODE_SPLIT_FP16, 128, 2);
     HV_V(MODE_FP64, 64, 1);
 #undef HV_V
-    // pf = 3: ring variant (direct-to-LDS loads, 3 stages ahead)
-    if (pf == 3) {
-        if (mode == MODE_FP32 && tile == 64) return launch_herk_ring_t<MODE_FP32, 64>(a, s);
-        if (mode == MODE_SPLIT_BF16 && tile == 64) return launch_herk_ring_t<MODE_SPLIT_BF16, 64>(a, s);
-        if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_herk_ring_t<MODE_SPLIT_BF16, 128>(a, s);
-        if (mode == MODE_SPLIT_FP16 && tile == 64) return launch_herk_ring_t<MODE_SPLIT_FP16, 64>(a, s);
-        if (mode == MODE_SPLIT_FP16 && tile == 128) return launch_herk_ring_t<MODE_SPLIT_FP16, 128>(a, s);
-        if (mode == MODE_FP64 && tile == 64) return launch_herk_ring_t<MODE_FP64, 64>(a, s);
+    // pf = 3: ring variant (direct-to-LDS loads, 3 stages ahead), one
+    // workgroup per work item; pf = 4: the same on a persistent grid
+    if (pf == 3 || pf == 4) {
+        const bool p = (pf == 4);
+        if (mode == MODE_FP32 && tile == 64) return launch_herk_ring_t<MODE_FP32, 64>(a, p, s);
+        if (mode == MODE_SPLIT_BF16 && tile == 64) return launch_herk_ring_t<MODE_SPLIT_BF16, 64>(a, p, s);
+        if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_herk_ring_t<MODE_SPLIT_BF16, 128>(a, p, s);
+        if (mode == MODE_SPLIT_FP16 && tile == 64) return launch_herk_ring_t<MODE_SPLIT_FP16, 64>(a, p, s);
+        if (mode == MODE_SPLIT_FP16 && tile == 128) return launch_herk_ring_t<MODE_SPLIT_FP16, 128>(a, p, s);
+        if (mode == MODE_FP64 && tile == 64) return launch_herk_ring_t<MODE_FP64, 64>(a, p, s);
     }
     return hipErrorInvalidValue;
 }
 
+void herk_init() {
+    (void)ring_resident<MODE_FP32, 64>();
+    (void)ring_resident<MODE_SPLIT_BF16, 64>();
+    (void)ring_resident<MODE_SPLIT_BF16, 128>();
+    (void)ring_resident<MODE_SPLIT_FP16, 64>();
+    (void)ring_resident<MODE_SPLIT_FP16, 128>();
+    (void)ring_resident<MODE_FP64, 64>();
+}
+
 int herk_prefetch(int mode, int tile) {
-    (void)mode;
-    (void)tile;
-    return g_pf_override ? g_pf_override : 1;
+    if (g_pf_override) return g_pf_override;
+    if (!split_mode(mode)) return 1;
+    return tile == 128 ? 4 : 3;
 }
 
 hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {

@@ -147,6 +147,7 @@ hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s);
 hipError_t launch_herk_variant(int mode, int tile, int pf, const HerkArgs& a, hipStream_t s);
 int herk_prefetch(int mode, int tile);
 void set_herk_variant(int tile, int pf);
+void herk_init();   // caches per-kernel launch facts (resident workgroups) for the current device
 hipError_t launch_reduce(int mode, const ReduceArgs& a, hipStream_t s);
 hipError_t launch_beam(const BeamArgs& a, hipStream_t s);
 hipError_t launch_split(int mode, const SplitArgs& a, hipStream_t s);

@@ -124,6 +124,7 @@ extern "C" int hv_ctx_create(int device, size_t hbm_budget, hv_ctx** out) {
             delete c;
             return fail(HV_ERR_HIP, "hipEventCreate failed");
         }
+    herk_init();   // occupancy queries now, never inside a stream capture
     *out = c;
     return HV_OK;
 }

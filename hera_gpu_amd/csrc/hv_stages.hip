@@ -172,9 +172,11 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     BT* sb = reinterpret_cast<BT*>(smem);
     const int64_t kact = eff_kpad(a.kpad, a.dyn_kpad);
-    const int64_t wg0 = (int64_t)blockIdx.x * PH_SRC_PER_WG;
+    // antenna is the fastest grid index: the workgroups of one source chunk
+    // run side by side and share its coordinates in L2
+    const int64_t wg0 = (int64_t)blockIdx.y * PH_SRC_PER_WG;
     if (wg0 >= kact) return;                       // uniform per workgroup
-    const int ant = blockIdx.y;
+    const int ant = blockIdx.x;
     const int tb = blockIdx.z;
     const int P = a.beam_px;
     const BT* gb = reinterpret_cast<const BT*>(a.beam) + (size_t)ant * P * P;
@@ -223,7 +225,9 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
 template <int MODE, typename BT>
 static hipError_t launch_phasor_t(const PhasorArgs& a, hipStream_t s) {
     const size_t lds = (size_t)a.beam_px * a.beam_px * sizeof(BT);
-    dim3 grid((unsigned)((a.kpad + PH_SRC_PER_WG - 1) / PH_SRC_PER_WG), (unsigned)a.nant, (unsigned)a.ntime);
+    const int64_t nchunk = (a.kpad + PH_SRC_PER_WG - 1) / PH_SRC_PER_WG;
+    if (nchunk > 65535 || a.ntime > 65535) return hipErrorInvalidConfiguration;
+    dim3 grid((unsigned)a.nant, (unsigned)nchunk, (unsigned)a.ntime);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)phasor_kernel<MODE, BT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -157,13 +157,14 @@ int hv_simulate(hv_ctx* ctx, int mode, const hv_sky* sky, int64_t nsrc_used,
 int hv_bench_copy(hv_ctx* ctx, size_t bytes, int reps, double* ms_per_copy);
 int hv_bench_mfma(hv_ctx* ctx, int mode, int iters, double* tflops);
 /* The Hermitian contraction alone on synthetic operands of `units` units of
- * nant x nsrc; tile (64/128) and prefetch depth (1/2) select a kernel variant,
- * 0 = the production choice.  Returns ms per launch. */
+ * nant x nsrc; tile (64/128) and pf select a kernel variant (as for
+ * hv_set_herk_variant), 0 = the production choice.  Returns ms per launch. */
 int hv_bench_herk(hv_ctx* ctx, int mode, int nant, int64_t nsrc, int units, int tile, int pf, int reps,
                   double* ms_per_launch);
 /* Process-wide choice of contraction variant for later calls (tuning and
- * tests): tile 64/128, pf 1/2 (register-staged prefetch depth) or 3 (LDS
- * ring); 0 restores the built-in choice.  Not safe while other calls run. */
+ * tests): tile 64/128, pf 1/2 (register-staged prefetch depth), 3 (LDS ring,
+ * one workgroup per work item) or 4 (LDS ring on a persistent grid); 0
+ * restores the built-in choice.  Not safe while other calls run. */
 int hv_set_herk_variant(int tile, int pf);
 
 /* ---------------------------------------------------------------------------

@@ -155,7 +155,7 @@ def test_herk_colocated_is_exact(gemm):
 
 
 # every contraction variant (tile x prefetch scheme) against FP64 NumPy
-VARIANTS = [(64, 1), (64, 2), (64, 3), (128, 1), (128, 2)]
+VARIANTS = [(64, 1), (64, 2), (64, 3), (64, 4), (128, 1), (128, 2), (128, 3), (128, 4)]
 
 
 @pytest.mark.parametrize("tile,pf", VARIANTS)
@@ -165,7 +165,7 @@ def test_herk_variants(gemm, tile, pf):
     if tile == 128 and gemm in ("fp32", "fp64"):
         pytest.skip("128-tiles exist for the split modes only")
     if gemm == "fp64" and pf == 2:
-        pytest.skip("FP64 has prefetch depths 1 and 3")
+        pytest.skip("FP64 has prefetch depths 1, 3 and 4")
     lib = _lib.load()
     _lib.check(lib.hv_set_herk_variant(tile, pf))
     try:
@@ -179,3 +179,27 @@ def test_herk_variants(gemm, tile, pf):
             np.testing.assert_array_equal(V, V.conj().T)
     finally:
         _lib.check(lib.hv_set_herk_variant(0, 0))
+
+
+@pytest.mark.parametrize("gemm,tile", [("split_bf16", 64), ("split_bf16", 128), ("fp32", 64), ("fp64", 64)])
+def test_herk_persistent_grid_equals_one_workgroup_per_item(gemm, tile):
+    # more work items than resident workgroups, so each persistent workgroup
+    # runs several (unit, split, tile) items; same arithmetic per item
+    from hera_gpu_amd import _lib
+    lib = _lib.load()
+    N, K = 350, 8192 * 48 + 64
+    rng = np.random.default_rng(7)
+    cd = np.complex128 if gemm == "fp64" else np.complex64
+    v = (rng.standard_normal((N, K), dtype=np.float32) + 1j * rng.standard_normal((N, K), dtype=np.float32)).astype(cd)
+    out = []
+    try:
+        for pf in (3, 4):
+            _lib.check(lib.hv_set_herk_variant(tile, pf))
+            out.append(stages.herk(v, gemm=gemm))
+    finally:
+        _lib.check(lib.hv_set_herk_variant(0, 0))
+    np.testing.assert_array_equal(out[0], out[1])
+    # spot check one row against FP64 products
+    ref = v[5].astype(np.complex128) @ v.astype(np.complex128).conj().T
+    assert vo.rel_err(out[1][5], ref) < 3 * HERK_TOL[gemm]
+
