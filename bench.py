@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=120, help="serving stream length (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default: 70%% of capacity)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling target: copy calibration, then the workload's first time (all "
+                         "frequencies) through the production kernels; prints per-kernel figures")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -148,6 +151,22 @@ def main():
     sky = stages.Sky(w.crd_eq, w.I_sky)
     beam = stages.Beam(w.bm_cube)
     flops_job = herk_flops(N, K, units)
+
+    if args.kernels_only:
+        # a short, fixed sequence of dispatches for rocprofv3 counter runs:
+        # 3 copies of 1 GiB (the yardstick for FETCH_SIZE / WRITE_SIZE), then
+        # one time of the workload (F units) through the production kernels
+        cp = stages.bench_copy(1 << 30, 3)
+        _, st = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[:1], gemm=args.gemm, timing=True,
+                                want_output=False, dtype=np.float32)
+        u_launch = F / st["herk_n"]
+        print(json.dumps({"copy": cp, "gemm": args.gemm, "units": F, "herk_launches": st["herk_n"],
+                          "units_per_launch": u_launch, "herk_avg_ms": st["herk_ms"] / st["herk_n"],
+                          "phasor_avg_ms": st["phasor_ms"] / st["phasor_n"],
+                          "reduce_avg_ms": st["reduce_ms"] / st["reduce_n"],
+                          "operand_bytes_per_launch": (16 if args.gemm == "fp64" else 8) * N * K * u_launch}),
+              flush=True)
+        return
 
     log(f"[bench] {args.gemm}: warmup {args.warmup}, timed {args.steps}")
     st, wall = run_timed(stages, sky, beam, w, args.gemm, args.steps, args.warmup)

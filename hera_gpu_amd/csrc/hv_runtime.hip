@@ -6,6 +6,7 @@
 // contraction and the K-split reduction, all on the context stream.  The
 // plan replaces the reference's per-call JIT, texture upload, per-chunk
 // streams and host-side chunk sum (hera_gpu/vis.py:176-258).
+#include <cstdlib>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -359,10 +360,20 @@ int hv::enqueue_job_pipelined(hv_ctx* c, const JobDev& j, const Plan& p, Workspa
     return HV_OK;
 }
 
+// Batches of a direct call run on one stream (phasor build, then
+// contraction) unless HERAVIS_OVERLAP=1 asks for the two-stream schedule.
+// Measured on MI355X (HERA-350, 2^20 sources): the phasor build writes its
+// operands at close to the HBM write ceiling (~3.6 TB/s), so running it
+// beside the contraction slows both, and one stream is ~6% faster.
+static bool overlap_batches() {
+    const char* e = std::getenv("HERAVIS_OVERLAP");
+    return e && e[0] == '1';
+}
+
 static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
     Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c));
     const int64_t batches = (int64_t)((j.ntimes + p.tb - 1) / p.tb) * ((j.nfreq + p.fb - 1) / p.fb);
-    if (batches < 2) {
+    if (batches < 2 || !overlap_batches()) {
         HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
         return enqueue_job(j, p, c->ws, c->stream, tm);
     }
