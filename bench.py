@@ -24,6 +24,16 @@ import numpy as np
 METRIC = "vis GEMM TFLOP/s & p99 job latency, split-BF16 vs FP32 at ≤1e-5 rel err"
 PEAK_TFLOPS = {"split_bf16": 2500.0, "split_fp16": 2500.0, "fp32": 157.3, "fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
+# HBM traffic of one contraction launch at the bench's launch shape (workload,
+# arithmetic, units per launch), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# passes over `bench.py --kernels-only` (KiB per dispatch), scaled by what the
+# same counters read for a 1 GiB device copy: FETCH_SIZE x2.0, WRITE_SIZE x1.0.
+MEASURED_TRAFFIC = {
+    ("hera350", "split_bf16", 8): {"fetch_kib": 12448316, "write_kib": 540672, "fetch_scale": 2.0,
+                                   "write_scale": 1.0},
+}
+PF_NAMES = {1: "register-staged, 1 stage ahead", 2: "register-staged, 2 stages ahead",
+            3: "LDS ring, one workgroup per work item", 4: "LDS ring, persistent grid"}
 
 
 def log(*a):
@@ -126,6 +136,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 sample comparison")
     ap.add_argument("--cpu-nsrc", type=int, default=2 ** 16)
+    ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer vis_gpu timing")
     ap.add_argument("--serve-jobs", type=int, default=120, help="serving stream length (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=None, help="arrivals per second (default: 70%% of capacity)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -192,6 +203,20 @@ def main():
                 "herk_tflops": flops_job / st32["herk_n"] / (st32["herk_ms"] / st32["herk_n"] / 1e3) / 1e12}
         log(f"[bench] fp32: {st32['total_ms']:.1f} ms/step, {v32:.1f} TFLOP/s")
 
+    dropin = None
+    if not args.no_dropin:
+        # the drop-in call with host buffers: uploads, the same pipeline and
+        # the (F, T, N, N) result back over PCIe (never the headline value)
+        from hera_gpu_amd import vis_gpu
+        vis_gpu(w.antpos, w.freqs[:1], w.eq2tops[:1], w.crd_eq, w.I_sky, w.bm_cube, gemm=args.gemm)
+        t0 = time.perf_counter()
+        vis_gpu(w.antpos, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, gemm=args.gemm)
+        dt = time.perf_counter() - t0
+        dropin = {"seconds": dt, "TFLOP/s": flops_job / dt / 1e12,
+                  "output_bytes": units * N * N * 8,
+                  "note": "vis_gpu with host arrays: inputs uploaded and the result copied back inside the timing"}
+        log(f"[bench] drop-in vis_gpu (host buffers): {dt:.2f} s, {dropin['TFLOP/s']:.1f} TFLOP/s")
+
     accuracy = None
     if not args.no_accuracy:
         # FP64 path (pinned to the oracle by the tests) on a sample of units
@@ -229,6 +254,11 @@ def main():
         serving = serving_leg(args.serve_jobs, value, gemms, args.serve_slots, args.serve_rate)
 
     peak = PEAK_TFLOPS[args.gemm]
+    upl = units * args.steps / st["herk_n"]
+    tr = MEASURED_TRAFFIC.get((args.workload, args.gemm, int(round(upl))))
+    traffic = ((tr["fetch_kib"] * tr["fetch_scale"] + tr["write_kib"] * tr["write_scale"]) * 1024
+               if tr and abs(upl - round(upl)) < 1e-9 else None)
+    tile, pf = stages.herk_choice(args.gemm, N)
     out = {
         "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -239,9 +269,13 @@ def main():
                    "gemm": args.gemm, "flops_per_step": flops_job,
                    "flop_count": "useful Hermitian 4 N (N+1) K per unit"},
         "roofline": {"bound": "mfma", "achieved": herk_tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": herk_tflops / peak, "traffic": None, "kernel": f"herk_kernel<{args.gemm}>",
-                     "avg_launch_ms": herk_avg_ms, "launches": st["herk_n"],
-                     "flops_per_launch": flops_per_launch},
+                     "frac": herk_tflops / peak, "traffic": traffic,
+                     "kernel": f"Hermitian contraction, {args.gemm}, {tile}x{tile} tiles, {PF_NAMES[pf]}",
+                     "avg_launch_ms": herk_avg_ms, "launches": st["herk_n"], "units_per_launch": upl,
+                     "flops_per_launch": flops_per_launch,
+                     "operand_bytes_per_launch": elem * N * K * upl,
+                     "flop_count": "useful 4 N (N+1) K per unit; peak = published dense MFMA rate of the "
+                                   "arithmetic (split-BF16 spends 3 BF16 products per useful product)"},
         "stages_ms_per_step": stage_ms,
         "phasor": {"avg_launch_ms": phasor_avg_ms, "bytes_per_launch": phasor_bytes,
                    "GBps": phasor_bytes / (phasor_avg_ms / 1e3) / 1e9, "hbm_frac": phasor_bytes /
@@ -249,6 +283,7 @@ def main():
         "fp32": fp32,
         "speedup_vs_fp32": (value / fp32["value"]) if fp32 else None,
         "accuracy": accuracy,
+        "dropin_pcie_inclusive": dropin,
         "cpu_baseline": cpu,
         "serving": serving,
         "p99_latency_ms": serving[args.gemm]["p99_ms"] if serving else None,

@@ -241,3 +241,12 @@ extern "C" int hv_set_herk_variant(int tile, int pf) {
     set_herk_variant(tile, pf);
     return HV_OK;
 }
+
+extern "C" int hv_herk_choice(int mode, int nant, int* tile, int* pf) {
+    if (!tile || !pf || nant < 1) return fail(HV_ERR_ARG, "bad argument");
+    if (mode < MODE_FP32 || mode > MODE_FP64) return fail(HV_ERR_ARG, "unknown contraction mode");
+    *tile = herk_tile(mode, nant);
+    *pf = herk_prefetch(mode, *tile);
+    return HV_OK;
+}
+

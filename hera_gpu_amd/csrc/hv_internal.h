@@ -15,7 +15,8 @@
 // Phasor operands are stage-major: the sources are cut into stages of
 // STAGE_SRC(mode) sources (128 bytes per antenna), and stage st of antenna a
 // sits at byte ((st * N + a) * 128) of its unit, so one contraction stage of
-// 64 antennas is one contiguous 8 KB block.  Within a stage, (re, im)
+// a 64- or 128-antenna tile is one contiguous 8 or 16 KB block.  Within a
+// stage, (re, im)
 // interleave per source:
 //   FP32 / FP64 : complex64 (16 sources) / complex128 (8 sources);
 //   split modes : 4 groups of 4 sources, each 32 bytes [8 hi | 8 lo] halves

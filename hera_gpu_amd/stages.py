@@ -169,6 +169,13 @@ def simulate(sky: Sky, beam: Beam, antpos, freqs, eq2tops, gemm="fp32", nsrc=Non
     return out, (st.as_dict() if timing else None)
 
 
+def herk_choice(gemm: str, nant: int) -> tuple:
+    """(tile edge, pf) of the contraction variant calls of this arithmetic and size use."""
+    t, p = C.c_int(), C.c_int()
+    _lib.check(_lib.load().hv_herk_choice(_mode(gemm), nant, C.byref(t), C.byref(p)), "hv_herk_choice")
+    return t.value, p.value
+
+
 def bench_copy(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> dict:
     """Plain device copy of nbytes: ms per copy and GB/s (read + write)."""
     ms = C.c_double()

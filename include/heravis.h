@@ -166,6 +166,9 @@ int hv_bench_herk(hv_ctx* ctx, int mode, int nant, int64_t nsrc, int units, int 
  * one workgroup per work item) or 4 (LDS ring on a persistent grid); 0
  * restores the built-in choice.  Not safe while other calls run. */
 int hv_set_herk_variant(int tile, int pf);
+/* The contraction variant (tile edge, pf as above) that calls of this mode and
+ * antenna count use now. */
+int hv_herk_choice(int mode, int nant, int* tile, int* pf);
 
 /* ---------------------------------------------------------------------------
  * Serving (no reference counterpart: the reference has no request loop).
