@@ -47,11 +47,11 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int LDS_STRIDE = 144;
 
 // Built-in variants (measured on MI355X, 8 units x 2^18 sources): split
-// modes at 256+ antennas take 128-tiles on the persistent LDS-ring grid (the
-// tiles of a split share their operand stages in L2: HBM reads ~1.1x the
-// operand bytes against ~3.5x for 64-tiles); smaller arrays take 64-tiles on
-// the LDS ring, which waste less on the triangle's edge.  FP32 and FP64 keep
-// the register-staged 64-tile kernel, their fastest.
+// modes take the LDS-ring kernel on a persistent grid, with 128-tiles at 256+
+// antennas (the tiles of a split share their operand stages in L2: HBM reads
+// ~1.1x the operand bytes against ~3.5x for 64-tiles) and 64-tiles below,
+// which waste less on the triangle's edge.  FP32 and FP64 keep the
+// register-staged 64-tile kernel, their fastest.
 static int g_tile_override = 0, g_pf_override = 0;   // 0 = built-in choice
 
 void set_herk_variant(int tile, int pf) {
@@ -954,8 +954,8 @@ void herk_init() {
 
 int herk_prefetch(int mode, int tile) {
     if (g_pf_override) return g_pf_override;
-    if (!split_mode(mode)) return 1;
-    return tile == 128 ? 4 : 3;
+    (void)tile;
+    return split_mode(mode) ? 4 : 1;   // split modes: LDS ring on a persistent grid
 }
 
 hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {
