@@ -36,8 +36,9 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Operand bytes one serving job may hold at once (one batch of units).
-constexpr size_t SERVE_OPERAND_BUDGET = (size_t)3 << 30;
+// Operand bytes one serving job may hold at once (one batch of units): four
+// HERA-350 units of 2^20 sources per launch.
+constexpr size_t SERVE_OPERAND_BUDGET = (size_t)12 << 30;
 
 struct BucketKey {
     int mode, nant, beam_px, ntimes, nfreq, beam_f64;
@@ -106,6 +107,7 @@ struct hv_server {
     std::deque<SrvJob*> pending;
     std::deque<Slot*> to_copy;
     std::unordered_map<int64_t, SrvJob*> finished;   // completed, not yet collected
+    std::map<BucketKey, int> seen;     // jobs admitted per shape bucket
     std::deque<int64_t> finished_order;
     int64_t next_id = 1;
     bool stopping = false;
@@ -193,9 +195,17 @@ static int launch_job(hv_server* s, Slot& sl, SrvJob& j) {
     jd.amp_scale = j.key.scale;
     jd.dyn_kpad = (const int64_t*)w.buf[B_DYN].p;
 
+    // A bucket is captured the second time it is seen: a shape that comes
+    // once runs direct (capture and instantiation of a large job's graph
+    // costs more than its launches), a recurring one is replayed.
+    bool recurring = false;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        recurring = s->seen[j.key]++ > 0;
+    }
     StageTimer tm;
-    if (s->use_graphs) {
-        auto it = sl.graphs.find(j.key);
+    auto it = sl.graphs.find(j.key);
+    if (s->use_graphs && (it != sl.graphs.end() || recurring)) {
         if (it == sl.graphs.end()) {
             hipGraph_t g = nullptr;
             HV_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));

@@ -65,17 +65,21 @@ def test_mixed_results_equal_alone_bitwise(catalogs):
 
 def test_graph_replay_same_bucket(catalogs):
     antpos, beam, sky, eq, freqs = job_inputs(catalogs, 3, "disk128", 2, 2)
+    eq2 = configs.eq2tops_lst(2, lst0_deg=77.0).astype(np.float64)
+    eq3 = configs.eq2tops_lst(2, lst0_deg=151.0).astype(np.float64)
     with Server(nslots=1) as srv:
+        # one bucket (16384 < Kpad <= 32768): the first job runs direct, the
+        # second is captured, the third replays the graph
         a = srv.wait(srv.submit(sky, beam, antpos, freqs, eq, nsrc=20000), 120)
-        # same bucket (16384 < Kpad <= 32768), different source count and times
-        eq2 = configs.eq2tops_lst(2, lst0_deg=77.0).astype(np.float64)
         b = srv.wait(srv.submit(sky, beam, antpos, freqs, eq2, nsrc=31000), 120)
+        c = srv.wait(srv.submit(sky, beam, antpos, freqs, eq3, nsrc=17000), 120)
         st = srv.stats()
-    assert a.status == 0 and b.status == 0
-    assert not a.graph_hit and b.graph_hit
-    assert st["graph_captures"] == 1 and st["graph_launches"] == 2
-    alone, _ = stages.simulate(sky, beam, antpos, freqs, eq2, nsrc=31000, gemm="split_bf16")
-    np.testing.assert_array_equal(b.vis, alone)
+    assert a.status == 0 and b.status == 0 and c.status == 0
+    assert not a.graph_hit and not b.graph_hit and c.graph_hit
+    assert st["direct_launches"] == 1 and st["graph_captures"] == 1 and st["graph_launches"] == 2
+    for d, e, n in ((a, eq, 20000), (b, eq2, 31000), (c, eq3, 17000)):
+        alone, _ = stages.simulate(sky, beam, antpos, freqs, e, nsrc=n, gemm="split_bf16")
+        np.testing.assert_array_equal(d.vis, alone)
 
 
 def test_admission_by_footprint(catalogs):
