@@ -181,7 +181,8 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
         mode > HV_MODE_FP64)
         return fail(HV_ERR_ARG, "bad argument");
     if (tile == 0) tile = herk_tile(mode, nant);
-    if (pf == 0) pf = herk_prefetch(mode, tile);
+    if (pf == 5) return fail(HV_ERR_ARG, "pf 5 builds its operands in place: time it through hv_simulate");
+    if (pf == 0) pf = herk_prefetch(mode, tile) == 5 ? 4 : herk_prefetch(mode, tile);   // same MFMA loop
     std::lock_guard<std::mutex> lk(c->mu);
     HV_HIP(hipSetDevice(c->device));
     const int64_t kpad = round_up(nsrc, KALIGN);
@@ -236,8 +237,8 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
 }
 
 extern "C" int hv_set_herk_variant(int tile, int pf) {
-    if (!(tile == 0 || tile == 64 || tile == 128) || pf < 0 || pf > 4)
-        return fail(HV_ERR_ARG, "tile must be 0, 64 or 128 and pf 0..4");
+    if (!(tile == 0 || tile == 64 || tile == 128) || pf < 0 || pf > 5)
+        return fail(HV_ERR_ARG, "tile must be 0, 64 or 128 and pf 0..5");
     set_herk_variant(tile, pf);
     return HV_OK;
 }

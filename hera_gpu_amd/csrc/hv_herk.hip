@@ -33,6 +33,7 @@
 // reads) and double buffering.  Workgroups are ordered so the tiles of one
 // K split run back to back on one XCD (shared L2 lines).
 #include "hv_internal.h"
+#include "hv_phasor_math.h"
 
 namespace hv {
 
@@ -95,19 +96,22 @@ __device__ inline void decode_tile(int t, int nb, int& bi, int& bj) {
 // Work decode.  Per unit there are nsplit * ntiles workgroups; with nsplit a
 // multiple of 8 the hardware's round-robin XCD assignment (block id mod 8)
 // gives XCD x the splits x, x+8, ... with all their tiles consecutive.
-__device__ inline void decode_work(const HerkArgs& a, int64_t L, int& unit, int& split, int& tile) {
-    const int64_t per = (int64_t)a.nsplit * a.ntiles;
+__device__ inline void decode_work(int nsplit, int ntiles, int64_t L, int& unit, int& split, int& tile) {
+    const int64_t per = (int64_t)nsplit * ntiles;
     unit = (int)(L / per);
     const int64_t rem = L - (int64_t)unit * per;
-    if ((a.nsplit & 7) == 0) {
+    if ((nsplit & 7) == 0) {
         const int x = (int)(rem & 7);
         const int64_t p = rem >> 3;
-        split = x + 8 * (int)(p / a.ntiles);
-        tile = (int)(p % a.ntiles);
+        split = x + 8 * (int)(p / ntiles);
+        tile = (int)(p % ntiles);
     } else {
-        split = (int)(rem / a.ntiles);
-        tile = (int)(rem % a.ntiles);
+        split = (int)(rem / ntiles);
+        tile = (int)(rem % ntiles);
     }
+}
+__device__ inline void decode_work(const HerkArgs& a, int64_t L, int& unit, int& split, int& tile) {
+    decode_work(a.nsplit, a.ntiles, L, unit, split, tile);
 }
 
 // Compile-time shape of one kernel variant.
@@ -910,6 +914,336 @@ static hipError_t launch_herk_ring_t(const HerkArgs& a, bool persistent, hipStre
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Split-mode contraction that builds its operands in place.  Instead of
+// reading phasor stages written by phasor_kernel, each workgroup generates
+// the hi/lo operand rows of its two antenna blocks for every stage from the
+// beam amplitude A sqrt(I) (amp_kernel, one per time) and the FP64 source
+// directions, with the same arithmetic as phasor_kernel (bit for bit the same
+// operands), and contracts them as herk_ring_kernel does (the same MFMA
+// sequence, so the same partial sums).  The 8 B per (antenna, source, unit)
+// operand never goes to HBM; the VALU work of the build runs beside the
+// matrix pipe of the same waves.
+//
+// LDS: an input ring of FRING slots (amplitude rows of both antenna blocks,
+// 64 B per row, and the stage's x, y, z) filled by global_load_lds, and two
+// operand buffers (A then B rows, 128-byte swizzled rows as in the ring
+// kernel).  Per stage k, after one barrier: issue the inputs of stage
+// k + FRING, build operand stage k+1 (wave w builds source group w of every
+// row, lane l rows l, l+64, ...) and run the MFMAs of stage k.
+#define HV_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+constexpr int FRING = 4;
+
+template <int TILE>
+struct FusedLds {
+    static constexpr int ROWS = 2 * TILE;                // A rows, then B rows
+    static constexpr int AMP = ROWS * 64;                // amplitude bytes per stage
+    static constexpr int CRD = 4 * 1024;                 // one KB per wave; x, y, z in waves 0-2
+    static constexpr int IN = AMP + CRD;                 // one input slot
+    static constexpr int OP = TILE * STAGE_BYTES;        // one operand tile
+    static constexpr int NAMP = ROWS * 4 / 256;          // amplitude loads per thread per stage
+    static constexpr int PER_STAGE = NAMP + 1;           // loads per thread per stage
+};
+
+// Issue the input loads of one stage into an input slot.  amp_st / crd_st
+// point at the stage in global memory; ampoff[i] (floats) and crdoff
+// (doubles) are this lane's offsets from there.
+template <int TILE>
+__device__ __forceinline__ void fused_issue(char* slot, const float* amp_st, const double* crd_st, const int* ampoff,
+                                            int crdoff, int wave) {
+    using L = FusedLds<TILE>;
+#pragma unroll
+    for (int i = 0; i < L::NAMP; ++i)
+        __builtin_amdgcn_global_load_lds(amp_st + ampoff[i], HV_LDS(slot + (i * 4 + wave) * 1024), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(crd_st + crdoff, HV_LDS(slot + L::AMP + wave * 1024), 16, 0, 0);
+}
+
+// Build operand stage rows for source group g (= wave) from an input slot:
+// lane rows rr = lane + 64 i, i < RPT (rows < TILE go to the A tile, the
+// rest to the B tile).
+template <int MODE, int TILE, int RPT>
+__device__ __forceinline__ void fused_build(const char* in, char* op, int lane, int g, const double (&ax)[RPT],
+                                            const double (&ay)[RPT], const double (&az)[RPT], double il4) {
+    using L = FusedLds<TILE>;
+    const f64x2* cx = reinterpret_cast<const f64x2*>(in + L::AMP + g * 32);
+    const f64x2* cy = reinterpret_cast<const f64x2*>(in + L::AMP + 1024 + g * 32);
+    const f64x2* cz = reinterpret_cast<const f64x2*>(in + L::AMP + 2048 + g * 32);
+    const f64x2 x01 = cx[0], x23 = cx[1], y01 = cy[0], y23 = cy[1], z01 = cz[0], z23 = cz[1];
+    const double xs[4] = {x01[0], x01[1], x23[0], x23[1]};
+    const double ys[4] = {y01[0], y01[1], y23[0], y23[1]};
+    const double zs[4] = {z01[0], z01[1], z23[0], z23[1]};
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int rr = lane + 64 * i;
+        const f32x4 am = *reinterpret_cast<const f32x4*>(in + rr * 64 + g * 16);
+        u32x4 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double d = fma(ax[i], xs[j], fma(ay[i], ys[j], az[i] * zs[j]));
+            float sn, cs;
+            sincos_quarter_turns(d * il4, &sn, &cs);
+            uint32_t h, l;
+            split_pair<MODE>(am[j] * cs, am[j] * sn, h, l);
+            hi[j] = h;
+            lo[j] = l;
+        }
+        char* base = (rr < TILE) ? op : op + L::OP;
+        const int r = (rr < TILE) ? rr : rr - TILE;
+        *reinterpret_cast<u32x4*>(base + lds_off<true>(r, 2 * g)) = hi;
+        *reinterpret_cast<u32x4*>(base + lds_off<true>(r, 2 * g + 1)) = lo;
+    }
+}
+
+// One split: stages [0, n_st).  RPT rows per lane (TILE/64 on diagonal
+// tiles, whose B tile is the A tile; TILE/32 otherwise).  KIND 0: the wave
+// builds but runs no MFMAs; 1: it runs the MFMAs of its whole wave tile.
+template <int MODE, int TILE, int RPT, int KIND>
+__device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, char* in3, char* op0, char* op1,
+                                            const float* amp_t, const double* crd_t, size_t amp_stage, int s_begin,
+                                            int n_st, const int* ampoff, int crdoff, int lane, int wave, int wr,
+                                            int wc, const double (&ax)[RPT], const double (&ay)[RPT],
+                                            const double (&az)[RPT], double il4, bool diag,
+                                            f32x16 (&tre)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
+                                            f32x16 (&tim)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS]) {
+    using C = Cfg<MODE, TILE>;
+    using L = FusedLds<TILE>;
+    constexpr int NS = C::NS, NACC = C::NACC;
+    constexpr int NMF = 6 * NS * NS;
+    constexpr int PER = L::PER_STAGE;
+    constexpr int VPM = (RPT * 4 * 36) / (2 * NMF);     // build VALU ops per MFMA (~36 per element)
+    constexpr int RFOLD = (TILE == 64) ? FOLD_SRC / C::SRC_STAGE : C::STAGES_SPLIT;
+    char* ins[FRING] = {in0, in1, in2, in3};
+    char* ops[2] = {op0, op1};
+    const int lrow = lane & 31, lh = lane >> 5;
+    const int ra0 = wr * C::WT + lrow, rb0 = wc * C::WT + lrow;
+    const int boff = diag ? 0 : L::OP;                  // B tile of a diagonal tile is its A tile
+#define HV_FISSUE(slot, st)                                                                         \
+    fused_issue<TILE>(slot, amp_t + (size_t)(s_begin + min(st, n_st - 1)) * amp_stage,               \
+                      crd_t + (size_t)(s_begin + min(st, n_st - 1)) * 16, ampoff, crdoff, wave)
+
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+#pragma unroll
+            for (int e = 0; e < NACC; ++e) { tre[i][j][e] = 0; tim[i][j][e] = 0; }
+
+    HV_FISSUE(in0, 0);
+    HV_FISSUE(in1, 1);
+    HV_FISSUE(in2, 2);
+    HV_FISSUE(in3, 3);
+    wait_vmcnt<(FRING - 1) * PER>();
+    __builtin_amdgcn_s_barrier();
+    fused_build<MODE, TILE, RPT>(in0, op0, lane, wave, ax, ay, az, il4);
+
+    for (int st = 0; st < n_st;) {
+        const int g_end = min(st + RFOLD, n_st);
+        f32x16 gre[NS][NS], gim[NS][NS];
+        if constexpr (TILE == 64) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j)
+#pragma unroll
+                    for (int e = 0; e < NACC; ++e) { gre[i][j][e] = 0; gim[i][j][e] = 0; }
+        }
+        f32x16 (&re)[NS][NS] = (TILE == 64) ? gre : tre;
+        f32x16 (&im)[NS][NS] = (TILE == 64) ? gim : tim;
+        for (; st < g_end; st += FRING) {
+#pragma unroll
+            for (int k = 0; k < FRING; ++k) {
+                char* cur = ops[k & 1];
+                char* nxt = ops[(k + 1) & 1];
+                // inputs of stage st+k+1 have landed (this wave); every
+                // wave's LDS writes and reads of the previous stage are done
+                wait_vm_lgkm<(FRING - 2) * PER, 0>();
+                __builtin_amdgcn_s_barrier();
+                HV_FISSUE(ins[k], st + k + FRING);           // slot of stage st+k, consumed
+                SplitFrag<NS> fa, fb;
+                read_frag<NS>(fa, cur, cur + boff, ra0, rb0, lh);
+                read_frag<NS>(fb, cur, cur + boff, ra0, rb0, 2 + lh);
+                // operand stage st+k+1 (past the end: rebuilt from a re-read, unused)
+                fused_build<MODE, TILE, RPT>(ins[(k + 1) % FRING], nxt, lane, wave, ax, ay, az, il4);
+                if constexpr (KIND != 0) {
+                    frag_mma<MODE, NS, true>(fa, 0u, re, im);
+                    frag_mma<MODE, NS, true>(fb, 0u, re, im);
+                    // spread the build's VALU work between the MFMAs so
+                    // the matrix pipe and the VALU run side by side
+#pragma unroll
+                    for (int m = 0; m < 2 * NMF; ++m) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x002, VPM, 0);
+                    }
+                }
+            }
+        }
+        if constexpr (KIND != 0 && TILE == 64) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    tre[i][j] += gre[i][j];
+                    tim[i][j] += gim[i][j];
+                }
+        }
+    }
+#undef HV_FISSUE
+    wait_vm_lgkm<0, 0>();
+    __builtin_amdgcn_s_barrier();
+}
+
+template <int MODE, int TILE>
+__global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
+    using C = Cfg<MODE, TILE>;
+    using L = FusedLds<TILE>;
+    constexpr int NS = C::NS, BLK = C::BLK, WT = C::WT, NACC = C::NACC;
+    constexpr int RD = TILE / 64, RO = TILE / 32;       // rows per lane: diagonal / other tiles
+
+    __shared__ __attribute__((aligned(1024))) char in0[L::IN];
+    __shared__ __attribute__((aligned(1024))) char in1[L::IN];
+    __shared__ __attribute__((aligned(1024))) char in2[L::IN];
+    __shared__ __attribute__((aligned(1024))) char in3[L::IN];
+    __shared__ __attribute__((aligned(1024))) char op0[2 * L::OP];
+    __shared__ __attribute__((aligned(1024))) char op1[2 * L::OP];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / C::SRC_STAGE);
+    const int64_t nitems = (int64_t)a.nunits * a.nsplit * a.ntiles;
+    const size_t amp_stage = (size_t)a.nant * 16;                 // floats per amplitude stage
+    const size_t amp_time = (size_t)a.kpad * a.nant;              // floats per time
+    const unsigned full = (1u << (NS * NS)) - 1;
+    (void)full;
+
+    for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
+        int unit, split, tile, bi, bj;
+        decode_work(a.nsplit, a.ntiles, Lw, unit, split, tile);
+        decode_tile(tile, a.nb, bi, bj);
+        const bool diag = (bi == bj);
+        const int s_begin = split * C::STAGES_SPLIT;
+        const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
+        if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
+        const int n_st = s_end - s_begin;
+        const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
+        const double il4 = 4.0 * a.inv_lambda[fl];
+        const float* amp_t = a.amp + (size_t)tb * amp_time;
+        const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
+
+        unsigned smask = 0;
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                const int r0 = wr * WT + i * BLK, c0 = wc * WT + j * BLK;
+                const bool on = (bi * TILE + r0 < a.nant) && (bj * TILE + c0 < a.nant) && !(diag && r0 > c0);
+                smask |= (unsigned)on << (i * NS + j);
+            }
+        smask = __builtin_amdgcn_readfirstlane(smask);
+
+        // input loads: amplitude chunk c = (i*4 + wave)*64 + lane is row
+        // c/4 (A rows, then B rows; past the last antenna: the last one),
+        // 16-byte part c%4; coordinate chunk: wave w < 3 loads coordinate w,
+        // lanes past 8 repeat lane & 7
+        int ampoff[L::NAMP];
+#pragma unroll
+        for (int i = 0; i < L::NAMP; ++i) {
+            const int c = (i * 4 + wave) * 64 + lane;
+            const int rr = c >> 2;
+            const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
+            ampoff[i] = ant * 16 + (c & 3) * 4;
+        }
+        const int crdoff = min(wave, 2) * (int)a.kpad + 2 * (lane & 7);
+
+        f32x16 tre[NS][NS], tim[NS][NS];
+        if (diag) {
+            double ax[RD], ay[RD], az[RD];
+#pragma unroll
+            for (int i = 0; i < RD; ++i) {
+                const int ant = min(bi * TILE + lane + 64 * i, a.nant - 1);
+                ax[i] = a.antpos[3 * ant]; ay[i] = a.antpos[3 * ant + 1]; az[i] = a.antpos[3 * ant + 2];
+            }
+            if (smask)
+                fused_split<MODE, TILE, RD, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, true, tre, tim);
+            else
+                fused_split<MODE, TILE, RD, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, true, tre, tim);
+        } else {
+            double ax[RO], ay[RO], az[RO];
+#pragma unroll
+            for (int i = 0; i < RO; ++i) {
+                const int rr = lane + 64 * i;
+                const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
+                ax[i] = a.antpos[3 * ant]; ay[i] = a.antpos[3 * ant + 1]; az[i] = a.antpos[3 * ant + 2];
+            }
+            if (smask)
+                fused_split<MODE, TILE, RO, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, false, tre, tim);
+            else
+                fused_split<MODE, TILE, RO, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, false, tre, tim);
+        }
+        if (!smask) continue;
+
+        constexpr size_t TSZ = (size_t)TILE * TILE;
+        float* out = reinterpret_cast<float*>(a.partials) +
+                     (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ;
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                if (!(smask & (1u << (i * NS + j)))) continue;
+#pragma unroll
+                for (int e = 0; e < NACC; ++e) {
+                    const int r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                    const int c = lane & 31;
+                    const int row = wr * WT + i * BLK + r;
+                    const int col = wc * WT + j * BLK + c;
+                    out[(size_t)row * TILE + col] = tre[i][j][e];
+                    out[TSZ + (size_t)row * TILE + col] = tim[i][j][e];
+                }
+            }
+    }
+}
+
+#undef HV_LDS
+
+// Resident workgroups of a fused kernel on the whole device (a multiple of 8).
+template <int MODE, int TILE>
+static int64_t fused_resident() {
+    static const int64_t cached = [] {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_fused_kernel<MODE, TILE>, 256, 0) !=
+                hipSuccess)
+            return (int64_t)0;
+        return (int64_t)ncu * per_cu / 8 * 8;
+    }();
+    return cached;
+}
+
+template <int MODE, int TILE>
+static hipError_t launch_fused_t(const FusedArgs& a, hipStream_t s) {
+    const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
+    if (nblk <= 0) return hipSuccess;
+    if (a.kpad % 16) return hipErrorInvalidValue;
+    const int64_t res = fused_resident<MODE, TILE>();
+    const int64_t grid = (res > 0 && res < nblk) ? res : nblk;
+    hipLaunchKernelGGL((herk_fused_kernel<MODE, TILE>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s) {
+    if (mode == MODE_SPLIT_BF16 && tile == 64) return launch_fused_t<MODE_SPLIT_BF16, 64>(a, s);
+    if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_fused_t<MODE_SPLIT_BF16, 128>(a, s);
+    if (mode == MODE_SPLIT_FP16 && tile == 64) return launch_fused_t<MODE_SPLIT_FP16, 64>(a, s);
+    if (mode == MODE_SPLIT_FP16 && tile == 128) return launch_fused_t<MODE_SPLIT_FP16, 128>(a, s);
+    return hipErrorInvalidValue;
+}
+
 template <int MODE, int TILE, int PF>
 static hipError_t launch_herk_t(const HerkArgs& a, hipStream_t s) {
     const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
@@ -950,6 +1284,10 @@ void herk_init() {
     (void)ring_resident<MODE_SPLIT_FP16, 64>();
     (void)ring_resident<MODE_SPLIT_FP16, 128>();
     (void)ring_resident<MODE_FP64, 64>();
+    (void)fused_resident<MODE_SPLIT_BF16, 64>();
+    (void)fused_resident<MODE_SPLIT_BF16, 128>();
+    (void)fused_resident<MODE_SPLIT_FP16, 64>();
+    (void)fused_resident<MODE_SPLIT_FP16, 128>();
 }
 
 int herk_prefetch(int mode, int tile) {
@@ -958,8 +1296,15 @@ int herk_prefetch(int mode, int tile) {
     return split_mode(mode) ? 4 : 1;   // split modes: LDS ring on a persistent grid
 }
 
+// Contraction of operands already in HBM (the built-in variant, or with
+// pf 5, which builds operands in place, the LDS ring on a persistent grid).
 hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {
-    return launch_herk_variant(mode, tile, herk_prefetch(mode, tile), a, s);
+    const int pf = herk_prefetch(mode, tile);
+    return launch_herk_variant(mode, tile, pf == 5 ? 4 : pf, a, s);
+}
+
+bool herk_fused(int mode, int nant) {
+    return split_mode(mode) && herk_prefetch(mode, herk_tile(mode, nant)) == 5;
 }
 
 }  // namespace hv

@@ -97,6 +97,8 @@ struct Plan {
     int tile = 0, nb = 0, ntiles = 0, nsplit = 0;
     size_t row_bytes = 0, unit_bytes = 0, part_unit = 0;
     int fb = 1, tb = 1;            // frequencies / times per batch
+    bool fused = false;            // operands built inside the contraction (B_W holds A sqrt(I))
+    size_t amp_time = 0;           // bytes of A sqrt(I) per time (fused)
     // device bytes per workspace buffer
     size_t crd_top_bytes = 0, w_bytes = 0, part_bytes = 0, out_bytes = 0;
     size_t footprint() const { return crd_top_bytes + w_bytes + part_bytes + out_bytes; }

@@ -140,6 +140,37 @@ struct SplitArgs {           // complex phasors -> operands (hv_k_herk)
     float scale;
 };
 
+// Beam amplitude A sqrt(I) (times the FP16 operand scale) per (antenna,
+// source, time): the input of the contraction that builds its operands in
+// place.  amp is [ntime][stage][N][16] FP32 (stage = 16 sources).
+struct AmpArgs {
+    const double* crd_top;   // [3][Kpad] of the first time; time tb at + tb*3*Kpad
+    const double* isqrt;     // [Kpad]
+    const void* beam;        // [N][P][P] float or double (in_f64)
+    float* amp;              // [ntime][Kpad/16][N][16]
+    int64_t kpad;
+    int nant, beam_px, ntime, in_f64;
+    float amp_scale;
+    const int64_t* dyn_kpad;
+};
+
+// Contraction of split-mode operands generated in place (see hv_herk.hip).
+// Unit u of the launch is time u / nfreq, frequency u % nfreq.
+struct FusedArgs {
+    const float* amp;         // AmpArgs::amp of the launch's times
+    const double* crd_top;    // [3][Kpad] of the first time; time tb at + tb*3*Kpad
+    const double* antpos;     // [N][3]
+    const double* inv_lambda; // [nfreq] nu/c of the launch's frequencies
+    void* partials;           // as HerkArgs::partials
+    int64_t kpad;
+    int nant, nb, ntiles, nsplit, nunits, nfreq;
+    const int64_t* dyn_kpad;
+};
+
+hipError_t launch_amp(const AmpArgs& a, hipStream_t s);
+hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s);
+bool herk_fused(int mode, int nant);   // does the built-in choice build operands in place?
+
 int herk_tile(int mode, int nant);
 
 hipError_t launch_coords(const CoordsArgs& a, hipStream_t s);

@@ -229,16 +229,22 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.unit_bytes = (size_t)nant * p.row_bytes;
     const size_t acc_bytes = mode_is_f64(mode) ? 8 : 4;
     p.part_unit = (size_t)p.nsplit * p.ntiles * 2 * p.tile * p.tile * acc_bytes;
-    // batch: all frequencies of a time if they fit, then as many times as fit
-    p.fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nfreq, wbudget / std::max<size_t>(p.unit_bytes, 1)));
+    p.fused = herk_fused(mode, nant);
+    // batch: all frequencies of a time if they fit, then as many times as fit.
+    // Operands per unit bound a batch, or, when the contraction builds them,
+    // the partial tiles per unit and A sqrt(I) per time
+    p.amp_time = p.fused ? (size_t)nant * kpad * sizeof(float) : 0;
+    const size_t per_unit = p.fused ? p.part_unit : p.unit_bytes;
+    p.fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nfreq, wbudget / std::max<size_t>(per_unit, 1)));
     p.tb = 1;
     if (p.fb == nfreq)
-        p.tb = (int)std::max<size_t>(1, std::min<size_t>((size_t)ntimes, wbudget / (p.unit_bytes * nfreq)));
-    // keep one contraction launch within ~64k workgroups
+        p.tb = (int)std::max<size_t>(
+            1, std::min<size_t>((size_t)ntimes, wbudget / std::max<size_t>(per_unit * nfreq + p.amp_time, 1)));
+    // keep one contraction launch within ~64k work items
     while (p.tb > 1 && (int64_t)p.tb * p.fb * p.nsplit * p.ntiles > 65536) --p.tb;
     const int units = p.fb * p.tb;
     p.crd_top_bytes = (size_t)ntimes * 3 * kpad * sizeof(double);
-    p.w_bytes = p.unit_bytes * units;
+    p.w_bytes = p.fused ? p.amp_time * p.tb : p.unit_bytes * units;
     p.part_bytes = p.part_unit * units;
     p.out_bytes = (size_t)nfreq * ntimes * nant * nant * (mode_is_f64(mode) ? 16 : 8);
     return p;
@@ -283,6 +289,9 @@ static int enqueue_phasor(const JobDev& j, const Plan& p, Workspace& ws, void* W
     return HV_OK;
 }
 
+static int enqueue_reduce(const JobDev& j, const Plan& p, void* part, int t0, int nt, int f0, int nf,
+                          hipStream_t stream, StageTimer& tm);
+
 static int enqueue_contract(const JobDev& j, const Plan& p, const void* W, void* part, int t0, int nt, int f0,
                             int nf, hipStream_t stream, StageTimer& tm) {
     HerkArgs ha{};
@@ -299,7 +308,11 @@ static int enqueue_contract(const JobDev& j, const Plan& p, const void* W, void*
     HV_CHECK(tm.begin(2, stream));
     HV_HIP(launch_herk(j.mode, p.tile, ha, stream));
     HV_CHECK(tm.end(2, stream));
+    return enqueue_reduce(j, p, part, t0, nt, f0, nf, stream, tm);
+}
 
+static int enqueue_reduce(const JobDev& j, const Plan& p, void* part, int t0, int nt, int f0, int nf,
+                          hipStream_t stream, StageTimer& tm) {
     ReduceArgs ra{};
     ra.partials = part;
     ra.vis = j.vis;
@@ -321,8 +334,68 @@ static int enqueue_contract(const JobDev& j, const Plan& p, const void* W, void*
     return HV_OK;
 }
 
+// Fused plans: A sqrt(I) of the batch's times (stage 1), then the
+// contraction that builds its operands from it (stage 2).
+static int enqueue_amp(const JobDev& j, const Plan& p, Workspace& ws, int t0, int nt, hipStream_t stream,
+                       StageTimer& tm) {
+    AmpArgs aa{};
+    aa.crd_top = static_cast<double*>(ws.buf[B_CRD_TOP].p) + (size_t)t0 * 3 * p.kpad;
+    aa.isqrt = j.isqrt;
+    aa.beam = j.beam;
+    aa.amp = static_cast<float*>(ws.buf[B_W].p);
+    aa.kpad = p.kpad;
+    aa.nant = j.nant;
+    aa.beam_px = j.beam_px;
+    aa.ntime = nt;
+    aa.in_f64 = j.beam_f64;
+    aa.amp_scale = j.amp_scale;
+    aa.dyn_kpad = j.dyn_kpad;
+    HV_CHECK(tm.begin(1, stream));
+    HV_HIP(launch_amp(aa, stream));
+    HV_CHECK(tm.end(1, stream));
+    return HV_OK;
+}
+
+static int enqueue_fused(const JobDev& j, const Plan& p, Workspace& ws, int t0, int nt, int f0, int nf,
+                         hipStream_t stream, StageTimer& tm) {
+    FusedArgs fa{};
+    fa.amp = static_cast<const float*>(ws.buf[B_W].p);
+    fa.crd_top = static_cast<double*>(ws.buf[B_CRD_TOP].p) + (size_t)t0 * 3 * p.kpad;
+    fa.antpos = j.antpos;
+    fa.inv_lambda = j.inv_lambda + f0;
+    fa.partials = ws.buf[B_PART].p;
+    fa.kpad = p.kpad;
+    fa.nant = j.nant;
+    fa.nb = p.nb;
+    fa.ntiles = p.ntiles;
+    fa.nsplit = p.nsplit;
+    fa.nunits = nt * nf;
+    fa.nfreq = nf;
+    fa.dyn_kpad = j.dyn_kpad;
+    HV_CHECK(tm.begin(2, stream));
+    HV_HIP(launch_fused(j.mode, p.tile, fa, stream));
+    HV_CHECK(tm.end(2, stream));
+    return HV_OK;
+}
+
 int hv::enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t stream, StageTimer& tm) {
     HV_CHECK(enqueue_coords(j, p, ws, stream, tm));
+    if (p.fused) {
+        int amp_t0 = -1;
+        for (int t0 = 0; t0 < j.ntimes; t0 += p.tb) {
+            const int nt = std::min(p.tb, j.ntimes - t0);
+            for (int f0 = 0; f0 < j.nfreq; f0 += p.fb) {
+                const int nf = std::min(p.fb, j.nfreq - f0);
+                if (amp_t0 != t0) {
+                    HV_CHECK(enqueue_amp(j, p, ws, t0, nt, stream, tm));
+                    amp_t0 = t0;
+                }
+                HV_CHECK(enqueue_fused(j, p, ws, t0, nt, f0, nf, stream, tm));
+                HV_CHECK(enqueue_reduce(j, p, ws.buf[B_PART].p, t0, nt, f0, nf, stream, tm));
+            }
+        }
+        return HV_OK;
+    }
     for (int t0 = 0; t0 < j.ntimes; t0 += p.tb) {
         const int nt = std::min(p.tb, j.ntimes - t0);
         for (int f0 = 0; f0 < j.nfreq; f0 += p.fb) {
@@ -373,7 +446,7 @@ static bool overlap_batches() {
 static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
     Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c));
     const int64_t batches = (int64_t)((j.ntimes + p.tb - 1) / p.tb) * ((j.nfreq + p.fb - 1) / p.fb);
-    if (batches < 2 || !overlap_batches()) {
+    if (batches < 2 || !overlap_batches() || p.fused) {
         HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
         return enqueue_job(j, p, c->ws, c->stream, tm);
     }

@@ -12,106 +12,9 @@
 // 4 nu / c, keeps only the fractional part (exact in FP64 for |turns| < 2^49)
 // and the quadrant, and evaluates sin/cos of that small angle.
 #include "hv_internal.h"
+#include "hv_phasor_math.h"
 
 namespace hv {
-
-// --------------------------------------------------------------- helpers
-// fmin/fmax return the non-NaN operand, so a NaN coordinate clamps to 0.
-template <typename R>
-__device__ inline R clampr(R v, R lo, R hi) { return fmin(fmax(v, lo), hi); }
-
-// Clamp-to-edge bilinear interpolation of beam[y][x] (P x P) at direction
-// cosines (x, y).  The texel coordinate and its integer part are formed in
-// FP64 and only the fraction is rounded to R (the reference forms the
-// coordinate in double and stores it in DTYPE, vis.py:63,69; keeping it in
-// FP64 removes the (P-1) 2^-24 quantisation of the interpolation weight).
-template <typename R>
-__device__ inline R beam_bilinear(const R* __restrict__ b, int P, double x, double y) {
-    const double bx = (double)(P - 1) * (0.5 * x + 0.5);
-    const double by = (double)(P - 1) * (0.5 * y + 0.5);
-    const double px = floor(bx), py = floor(by);
-    const R fx = (R)(bx - px), fy = (R)(by - py);
-    const double pm = (double)(P - 1);
-    const int ix0 = (int)clampr(px, 0.0, pm), ix1 = (int)clampr(px + 1.0, 0.0, pm);
-    const int iy0 = (int)clampr(py, 0.0, pm), iy1 = (int)clampr(py + 1.0, 0.0, pm);
-    const R v00 = b[iy0 * P + ix0], v01 = b[iy0 * P + ix1];
-    const R v10 = b[iy1 * P + ix0], v11 = b[iy1 * P + ix1];
-    const R top = fma(fx, v01 - v00, v00);
-    const R bot = fma(fx, v11 - v10, v10);
-    return fma(fy, bot - top, top);
-}
-
-// sin and cos of a phase given in quarter turns t4 (FP32 result, FP64
-// reduction): t4 = q + u with q = rint(t4) and |u| <= 1/2 exact in FP64;
-// sin/cos(pi/2 u) come from odd/even polynomials in u fitted here by
-// iteratively reweighted least squares (Lawson's minimax iteration) on
-// [-1/2, 1/2] (max error 8.9e-8 / 7.4e-8 when evaluated in FP32), and the
-// quadrant q mod 4 rotates the pair.  q mod 4 is read from the low mantissa
-// bits of q + 1.5 * 2^52, exact for |q| < 2^51.
-__device__ inline void sincos_quarter_turns(double t4, float* s, float* c) {
-    const double q = rint(t4);
-    const float u = (float)(t4 - q);
-    const int k = (int)(__double2loint(q + 6755399441055744.0) & 3);
-    const float u2 = u * u;
-    const float ps = u * fmaf(u2, fmaf(u2, fmaf(u2, -0.0046021766029298306f, 0.07968022674322128f),
-                                      -0.6459634900093079f), 1.5707963705062866f);
-    const float pc = fmaf(u2, fmaf(u2, fmaf(u2, fmaf(u2, 0.0008997500990517437f, -0.020859001204371452f),
-                                            0.2536691725254059f), -1.2337005138397217f), 1.0f);
-    const bool swap = k & 1;
-    const float sv = swap ? pc : ps;
-    const float cv = swap ? ps : pc;
-    // quadrants 2, 3 negate sin; quadrants 1, 2 negate cos
-    *s = __int_as_float(__float_as_int(sv) ^ ((k & 2) << 30));
-    *c = __int_as_float(__float_as_int(cv) ^ (((k + 1) & 2) << 30));
-}
-__device__ inline void sincos_quarter_turns(double t4, double* s, double* c) {
-    const double r = t4 - rint(t4);            // [-1/2, 1/2] quarter turns
-    sincospi(0.5 * r, s, c);                   // angle pi/2 r, then rotate by the quadrant
-    const int k = (int)(__double2loint(rint(t4) + 6755399441055744.0) & 3);
-    const double sv = (k & 1) ? *c : *s;
-    const double cv = (k & 1) ? *s : *c;
-    *s = (k & 2) ? -sv : sv;
-    *c = ((k + 1) & 2) ? -cv : cv;
-}
-
-// Round-to-nearest-even float -> bf16 / fp16 bit patterns and back.
-__device__ inline uint16_t to_bf16(float x) {
-    __bf16 h = (__bf16)x;
-    return __builtin_bit_cast(uint16_t, h);
-}
-__device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ inline uint16_t to_f16(float x) {
-    _Float16 h = (_Float16)x;
-    return __builtin_bit_cast(uint16_t, h);
-}
-__device__ inline float from_f16(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
-
-// Split 4 complex values into a 32-byte group [8 hi | 8 lo] of 16-bit halves.
-template <int MODE>
-__device__ inline void split_group(const float* re, const float* im, uint4* out) {
-    uint16_t hi[8], lo[8];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const float x = c ? im[j] : re[j];
-            uint16_t h;
-            float back;
-            if constexpr (MODE == MODE_SPLIT_BF16) { h = to_bf16(x); back = from_bf16(h); }
-            else { h = to_f16(x); back = from_f16(h); }
-            hi[2 * j + c] = h;
-            lo[2 * j + c] = (MODE == MODE_SPLIT_BF16) ? to_bf16(x - back) : to_f16(x - back);
-        }
-    }
-    uint32_t w[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        w[i] = (uint32_t)hi[2 * i] | ((uint32_t)hi[2 * i + 1] << 16);
-        w[4 + i] = (uint32_t)lo[2 * i] | ((uint32_t)lo[2 * i + 1] << 16);
-    }
-    out[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    out[1] = make_uint4(w[4], w[5], w[6], w[7]);
-}
 
 // Write 8 consecutive phasors (one thread's share) of one antenna row.
 template <int MODE, typename R>
@@ -252,6 +155,64 @@ hipError_t launch_phasor(int mode, const PhasorArgs& a, hipStream_t s) {
             return f64 ? launch_phasor_t<MODE_FP64, double>(a, s) : launch_phasor_t<MODE_FP64, float>(a, s);
     }
     return hipErrorInvalidValue;
+}
+
+// ------------------------------------------- beam amplitude per time
+// A sqrt(I) (times the FP16 operand scale) per (antenna, source, time) in
+// FP32, for the contraction that builds its operands in place: the same
+// arithmetic, in the same order, as the first half of phasor_kernel, so the
+// operands built from it are bit for bit those phasor_kernel writes.  Grid
+// (antenna, source chunk, time) with the antenna fastest, as phasor_kernel.
+template <typename BT>
+__global__ __launch_bounds__(PH_THREADS) void amp_kernel(AmpArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    BT* sb = reinterpret_cast<BT*>(smem);
+    const int64_t kact = eff_kpad(a.kpad, a.dyn_kpad);
+    const int64_t wg0 = (int64_t)blockIdx.y * PH_SRC_PER_WG;
+    if (wg0 >= kact) return;                       // uniform per workgroup
+    const int ant = blockIdx.x;
+    const int tb = blockIdx.z;
+    const int P = a.beam_px;
+    const BT* gb = reinterpret_cast<const BT*>(a.beam) + (size_t)ant * P * P;
+    for (int i = threadIdx.x; i < P * P; i += PH_THREADS) sb[i] = gb[i];
+    __syncthreads();
+    const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
+    const double* cy = cx + a.kpad;
+    const double* cz = cx + 2 * a.kpad;
+    const int64_t wg1 = min(wg0 + (int64_t)PH_SRC_PER_WG, kact);
+    float* out = a.amp + (size_t)tb * a.kpad * a.nant;
+    for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
+         s0 += PH_THREADS * PH_SRC_PER_THREAD) {
+        float amp[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t s = s0 + j;
+            const double x = cx[s], y = cy[s], z = cz[s];
+            float A = 0.f;
+            if (z > 0.0) A = (float)beam_bilinear<BT>(sb, P, x, y);
+            amp[j] = A * (float)a.isqrt[s] * a.amp_scale;
+        }
+        float4* d = reinterpret_cast<float4*>(out + ((size_t)(s0 / 16) * a.nant + ant) * 16 + (s0 % 16));
+        d[0] = make_float4(amp[0], amp[1], amp[2], amp[3]);
+        d[1] = make_float4(amp[4], amp[5], amp[6], amp[7]);
+    }
+}
+
+hipError_t launch_amp(const AmpArgs& a, hipStream_t s) {
+    const int64_t nchunk = (a.kpad + PH_SRC_PER_WG - 1) / PH_SRC_PER_WG;
+    if (nchunk > 65535 || a.ntime > 65535 || a.kpad % 16) return hipErrorInvalidConfiguration;
+    dim3 grid((unsigned)a.nant, (unsigned)nchunk, (unsigned)a.ntime);
+    const size_t lds = (size_t)a.beam_px * a.beam_px * (a.in_f64 ? 8 : 4);
+    const void* k = a.in_f64 ? (const void*)amp_kernel<double> : (const void*)amp_kernel<float>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (a.in_f64)
+        hipLaunchKernelGGL(amp_kernel<double>, grid, dim3(PH_THREADS), lds, s, a);
+    else
+        hipLaunchKernelGGL(amp_kernel<float>, grid, dim3(PH_THREADS), lds, s, a);
+    return hipGetLastError();
 }
 
 // ----------------------------------------------------- standalone beam stage

@@ -163,8 +163,10 @@ int hv_bench_herk(hv_ctx* ctx, int mode, int nant, int64_t nsrc, int units, int 
                   double* ms_per_launch);
 /* Process-wide choice of contraction variant for later calls (tuning and
  * tests): tile 64/128, pf 1/2 (register-staged prefetch depth), 3 (LDS ring,
- * one workgroup per work item) or 4 (LDS ring on a persistent grid); 0
- * restores the built-in choice.  Not safe while other calls run. */
+ * one workgroup per work item), 4 (LDS ring on a persistent grid) or 5 (split
+ * modes: operands built inside the contraction from A sqrt(I) per time, no
+ * operand buffer; hv_k_herk and hv_bench_herk, which take operands, use 4);
+ * 0 restores the built-in choice.  Not safe while other calls run. */
 int hv_set_herk_variant(int tile, int pf);
 /* The contraction variant (tile edge, pf as above) that calls of this mode and
  * antenna count use now. */

@@ -210,3 +210,27 @@ def test_batch_schedules_agree_bitwise(gemm, monkeypatch):
     np.testing.assert_array_equal(outs[0], outs[1])
     ref = vis_oracle(w.antpos, freqs[:2], w.eq2tops[:1], w.crd_eq, w.I_sky, w.bm_cube)
     assert rel_err(outs[0][:2, :1], ref) < VIS_TOL[gemm]
+
+
+@pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
+@pytest.mark.parametrize("tile", [64, 128])
+def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile):
+    # pf 5 builds the operand stages inside the contraction from A sqrt(I);
+    # pf 4 contracts the stages phasor_kernel wrote: same operand bits, same
+    # MFMA sequence, so the same visibilities bit for bit.  N = 300 leaves a
+    # partial last tile; K spans several 8192-source splits; 3 times x 4
+    # frequencies
+    from hera_gpu_amd import _lib
+    lib = _lib.load()
+    w = configs.make_hera350(nsrc=3 * 8192 + 1000, ntimes=3, nfreq=4)
+    ap, bm = w.antpos[:300], w.bm_cube[:300]
+    outs = []
+    try:
+        for pf in (4, 5):
+            _lib.check(lib.hv_set_herk_variant(tile, pf))
+            outs.append(vis_gpu(ap, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, bm, gemm=gemm))
+    finally:
+        _lib.check(lib.hv_set_herk_variant(0, 0))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    ref = vis_oracle(ap, w.freqs[-1:], w.eq2tops[:1], w.crd_eq, w.I_sky, bm)[0, 0]
+    assert rel_err(outs[1][-1, 0], ref) < VIS_TOL[gemm]
