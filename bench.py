@@ -33,7 +33,8 @@ MEASURED_TRAFFIC = {
                                    "write_scale": 1.0},
 }
 PF_NAMES = {1: "register-staged, 1 stage ahead", 2: "register-staged, 2 stages ahead",
-            3: "LDS ring, one workgroup per work item", 4: "LDS ring, persistent grid"}
+            3: "LDS ring, one workgroup per work item", 4: "LDS ring, persistent grid",
+            5: "operands built in place from A sqrt(I), 8 waves, persistent grid"}
 
 
 def log(*a):
@@ -154,6 +155,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrivals per second (default: 70%% of the measured serving capacity)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--herk-variant", default=None, metavar="TILE,PF",
+                    help="contraction variant for every run (as hv_set_herk_variant; default: built-in)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling target: copy calibration, then the workload's first time (all "
                          "frequencies) through the production kernels; prints per-kernel figures")
@@ -165,7 +168,11 @@ def main():
               "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from hera_gpu_amd import configs, stages
+    from hera_gpu_amd import _lib, configs, stages
+
+    if args.herk_variant:
+        tile, pf = (int(x) for x in args.herk_variant.split(","))
+        _lib.check(_lib.load().hv_set_herk_variant(tile, pf), "hv_set_herk_variant")
 
     t0 = time.perf_counter()
     w = configs.WORKLOADS[args.workload]()
@@ -204,8 +211,15 @@ def main():
         f"herk {herk_tflops:.1f} TFLOP/s ({st['herk_n']} launches, {herk_avg_ms:.3f} ms avg)")
     phasor_avg_ms = st["phasor_ms"] / st["phasor_n"]
     elem = 16 if args.gemm == "fp64" else 8
-    # phasor launch writes units_per_launch * N * K phasors of `elem` bytes
-    phasor_bytes = elem * N * K * units * args.steps / st["phasor_n"]
+    tile, pf = stages.herk_choice(args.gemm, N)
+    if pf == 5:
+        # stage 1 is the beam amplitude A sqrt(I): 4 B per (antenna, source, time)
+        stage1 = "amp_kernel: A sqrt(I) per time, FP32"
+        phasor_bytes = 4 * N * K * T * args.steps / st["phasor_n"]
+    else:
+        # phasor launch writes units_per_launch * N * K phasors of `elem` bytes
+        stage1 = "phasor_kernel: operand stages"
+        phasor_bytes = elem * N * K * units * args.steps / st["phasor_n"]
     stage_ms = {k: st[k] / args.steps for k in ("coords_ms", "phasor_ms", "herk_ms", "reduce_ms")}
 
     fp32 = None
@@ -272,7 +286,6 @@ def main():
     tr = MEASURED_TRAFFIC.get((args.workload, args.gemm, int(round(upl))))
     traffic = ((tr["fetch_kib"] * tr["fetch_scale"] + tr["write_kib"] * tr["write_scale"]) * 1024
                if tr and abs(upl - round(upl)) < 1e-9 else None)
-    tile, pf = stages.herk_choice(args.gemm, N)
     out = {
         "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -287,11 +300,11 @@ def main():
                      "kernel": f"Hermitian contraction, {args.gemm}, {tile}x{tile} tiles, {PF_NAMES[pf]}",
                      "avg_launch_ms": herk_avg_ms, "launches": st["herk_n"], "units_per_launch": upl,
                      "flops_per_launch": flops_per_launch,
-                     "operand_bytes_per_launch": elem * N * K * upl,
+                     "operand_bytes_per_launch": (elem * N * K * upl) if pf != 5 else 0,
                      "flop_count": "useful 4 N (N+1) K per unit; peak = published dense MFMA rate of the "
                                    "arithmetic (split-BF16 spends 3 BF16 products per useful product)"},
         "stages_ms_per_step": stage_ms,
-        "phasor": {"avg_launch_ms": phasor_avg_ms, "bytes_per_launch": phasor_bytes,
+        "phasor": {"kernel": stage1, "avg_launch_ms": phasor_avg_ms, "bytes_per_launch": phasor_bytes,
                    "GBps": phasor_bytes / (phasor_avg_ms / 1e3) / 1e9, "hbm_frac": phasor_bytes /
                    (phasor_avg_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
         "fp32": fp32,

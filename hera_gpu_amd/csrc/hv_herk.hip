@@ -47,8 +47,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDS_STRIDE = 144;
 
-// Built-in variants (measured on MI355X, 8 units x 2^18 sources): split
-// modes take the LDS-ring kernel on a persistent grid, with 128-tiles at 256+
+// Built-in variants (measured on MI355X): split modes take 128-tiles at 256+
 // antennas (the tiles of a split share their operand stages in L2: HBM reads
 // ~1.1x the operand bytes against ~3.5x for 64-tiles) and 64-tiles below,
 // which waste less on the triangle's edge.  FP32 and FP64 keep the
@@ -62,8 +61,10 @@ void set_herk_variant(int tile, int pf) {
 
 static bool split_mode(int mode) { return mode == MODE_SPLIT_BF16 || mode == MODE_SPLIT_FP16; }
 
+// Overrides apply where the variant exists for the mode (128-tiles and
+// in-place operand builds are split-mode kernels).
 int herk_tile(int mode, int nant) {
-    if (g_tile_override) return g_tile_override;
+    if (g_tile_override && (g_tile_override == 64 || split_mode(mode))) return g_tile_override;
     return (split_mode(mode) && nant >= 256) ? 128 : 64;
 }
 
@@ -1208,6 +1209,290 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// 128-tile build-in-place contraction with 8 waves (two per SIMD): each wave
+// owns a 64x32 wave tile (two 32x32 MFMA blocks, 64 accumulator registers),
+// so two waves share a SIMD and one's MFMAs run while the other builds
+// operands or waits on LDS.  Operand build, input ring and per-block MFMA
+// chains are those of herk_fused_kernel (same bits).
+//
+// Waves: MFMA tile rows wr = wave / 4 (64 rows), columns wc = wave % 4 (32);
+// build group g = wave % 4 (4 sources), row half h = wave / 4.
+struct Frag21 {
+    u32x4 ahi[2], alo[2], bhi, blo;
+};
+
+__device__ __forceinline__ void read_frag21(Frag21& f, const char* As, const char* Bs, int ra0, int rb0, int g) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        f.ahi[i] = *reinterpret_cast<const u32x4*>(As + lds_off<true>(ra0 + 32 * i, 2 * g));
+        f.alo[i] = *reinterpret_cast<const u32x4*>(As + lds_off<true>(ra0 + 32 * i, 2 * g + 1));
+    }
+    f.bhi = *reinterpret_cast<const u32x4*>(Bs + lds_off<true>(rb0, 2 * g));
+    f.blo = *reinterpret_cast<const u32x4*>(Bs + lds_off<true>(rb0, 2 * g + 1));
+}
+
+// Same chain per 32x32 block as frag_mma: Re += hi.hi + hi.lo + lo.hi and
+// Im += a'.b with the same operand order.
+template <int MODE>
+__device__ __forceinline__ void frag21_mma(const Frag21& f, f32x16 (&re)[2], f32x16 (&im)[2]) {
+    u32x4 phi[2], plo[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        phi[i] = rot_neg16(f.ahi[i]);
+        plo[i] = rot_neg16(f.alo[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        re[i] = mfma16<MODE>(f.ahi[i], f.bhi, re[i]);
+        im[i] = mfma16<MODE>(phi[i], f.bhi, im[i]);
+        re[i] = mfma16<MODE>(f.ahi[i], f.blo, re[i]);
+        im[i] = mfma16<MODE>(phi[i], f.blo, im[i]);
+        re[i] = mfma16<MODE>(f.alo[i], f.bhi, re[i]);
+        im[i] = mfma16<MODE>(plo[i], f.bhi, im[i]);
+    }
+}
+
+struct Fused8Lds {
+    static constexpr int TILE = 128;
+    static constexpr int ROWS = 2 * TILE;
+    static constexpr int AMP = ROWS * 64;                // 16 KB
+    static constexpr int CRD = 4 * 1024;                 // waves w and w+4 fill region w % 4
+    static constexpr int IN = AMP + CRD;
+    static constexpr int OP = TILE * STAGE_BYTES;
+    static constexpr int NAMP = ROWS * 4 / 512;          // amplitude loads per thread per stage
+    static constexpr int PER_STAGE = NAMP + 1;
+};
+
+__device__ __forceinline__ void fused8_issue(char* slot, const float* amp_st, const double* crd_st,
+                                             const int* ampoff, int crdoff, int wave) {
+    using L = Fused8Lds;
+#pragma unroll
+    for (int i = 0; i < L::NAMP; ++i)
+        __builtin_amdgcn_global_load_lds(amp_st + ampoff[i], HV_LDS(slot + (i * 8 + wave) * 1024), 16, 0, 0);
+    // waves w and w + 4 write the same bytes of coordinate w % 4 (w % 4 = 3:
+    // a copy of z nobody reads)
+    __builtin_amdgcn_global_load_lds(crd_st + crdoff, HV_LDS(slot + L::AMP + (wave & 3) * 1024), 16, 0, 0);
+}
+
+// Build operand rows rbase + 64 i (i < RPT) of source group g from an
+// input slot (as fused_build).
+template <int MODE, int RPT>
+__device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase, int g, const double (&ax)[RPT],
+                                             const double (&ay)[RPT], const double (&az)[RPT], double il4) {
+    using L = Fused8Lds;
+    constexpr int TILE = 128;
+    const double* cx = reinterpret_cast<const double*>(in + L::AMP + g * 32);
+    const double* cy = reinterpret_cast<const double*>(in + L::AMP + 1024 + g * 32);
+    const double* cz = reinterpret_cast<const double*>(in + L::AMP + 2048 + g * 32);
+    f32x4 am[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) am[i] = *reinterpret_cast<const f32x4*>(in + (rbase + 64 * i) * 64 + g * 16);
+    u32x4 hi[RPT], lo[RPT];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double x = cx[j], y = cy[j], z = cz[j];      // one source, every row of the lane
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const double d = fma(ax[i], x, fma(ay[i], y, az[i] * z));
+            float sn, cs;
+            sincos_quarter_turns(d * il4, &sn, &cs);
+            uint32_t hw, lw;
+            split_pair<MODE>(am[i][j] * cs, am[i][j] * sn, hw, lw);
+            hi[i][j] = hw;
+            lo[i][j] = lw;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int rr = rbase + 64 * i;
+        char* base = (rr < TILE) ? op : op + L::OP;
+        const int r = (rr < TILE) ? rr : rr - TILE;
+        *reinterpret_cast<u32x4*>(base + lds_off<true>(r, 2 * g)) = hi[i];
+        *reinterpret_cast<u32x4*>(base + lds_off<true>(r, 2 * g + 1)) = lo[i];
+    }
+}
+
+// The MFMAs of one stage for one wave: both 8-source steps, fragments read
+// just before use.
+template <int MODE>
+__device__ __forceinline__ void fused8_mma(const char* As, const char* Bs, int ra0, int rb0, int lh,
+                                           f32x16 (&re)[2], f32x16 (&im)[2]) {
+    Frag21 f;
+    read_frag21(f, As, Bs, ra0, rb0, lh);
+    frag21_mma<MODE>(f, re, im);
+    read_frag21(f, As, Bs, ra0, rb0, 2 + lh);
+    frag21_mma<MODE>(f, re, im);
+}
+
+template <int MODE, int RPT, int KIND>
+__device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, char* in3, char* op0, char* op1,
+                                             const float* amp_t, const double* crd_t, size_t amp_stage,
+                                             int s_begin, int n_st, const int* ampoff, int crdoff, int lane,
+                                             int wave, const double (&ax)[RPT], const double (&ay)[RPT],
+                                             const double (&az)[RPT], double il4, bool diag, f32x16 (&re)[2],
+                                             f32x16 (&im)[2]) {
+    using L = Fused8Lds;
+    constexpr int TILE = 128;
+    constexpr int PER = L::PER_STAGE;
+    char* ins[FRING] = {in0, in1, in2, in3};
+    char* ops[2] = {op0, op1};
+    const int g = wave & 3, h = wave >> 2;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int lrow = lane & 31, lh = lane >> 5;
+    const int ra0 = wr * 64 + lrow, rb0 = wc * 32 + lrow;
+    const int boff = diag ? 0 : L::OP;
+    // rows this lane builds: RPT = 2 (other tiles): h*128 + lane + 64 i;
+    // RPT = 1 (diagonal tiles, A rows only): h*64 + lane
+    const int rbase = (RPT == 2) ? h * 128 + lane : h * 64 + lane;
+#define HV_F8ISSUE(slot, st)                                                                         \
+    fused8_issue(slot, amp_t + (size_t)(s_begin + min(st, n_st - 1)) * amp_stage,                     \
+                 crd_t + (size_t)(s_begin + min(st, n_st - 1)) * 16, ampoff, crdoff, wave)
+
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) { re[i][e] = 0; im[i][e] = 0; }
+
+    HV_F8ISSUE(in0, 0);
+    HV_F8ISSUE(in1, 1);
+    HV_F8ISSUE(in2, 2);
+    HV_F8ISSUE(in3, 3);
+    wait_vmcnt<(FRING - 1) * PER>();
+    __builtin_amdgcn_s_barrier();
+    fused8_build<MODE, RPT>(in0, op0, rbase, g, ax, ay, az, il4);
+
+    for (int st = 0; st < n_st; st += FRING) {
+#pragma unroll
+        for (int k = 0; k < FRING; ++k) {
+            char* cur = ops[k & 1];
+            char* nxt = ops[(k + 1) & 1];
+            wait_vm_lgkm<(FRING - 2) * PER, 0>();
+            __builtin_amdgcn_s_barrier();
+            HV_F8ISSUE(ins[k], st + k + FRING);
+            // the two waves of a SIMD (h = 0 and 1) take the build and the
+            // MFMAs in opposite order, so one's matrix work runs while the
+            // other's VALU work does
+            if (h == 0) {
+                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il4);
+                __builtin_amdgcn_sched_barrier(0);
+                if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
+            } else {
+                if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
+                __builtin_amdgcn_sched_barrier(0);
+                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il4);
+            }
+        }
+    }
+#undef HV_F8ISSUE
+    wait_vm_lgkm<0, 0>();
+    __builtin_amdgcn_s_barrier();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
+    using L = Fused8Lds;
+    constexpr int TILE = 128;
+    __shared__ __attribute__((aligned(1024))) char in0[L::IN];
+    __shared__ __attribute__((aligned(1024))) char in1[L::IN];
+    __shared__ __attribute__((aligned(1024))) char in2[L::IN];
+    __shared__ __attribute__((aligned(1024))) char in3[L::IN];
+    __shared__ __attribute__((aligned(1024))) char op0[2 * L::OP];
+    __shared__ __attribute__((aligned(1024))) char op1[2 * L::OP];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / 16);
+    const int64_t nitems = (int64_t)a.nunits * a.nsplit * a.ntiles;
+    const size_t amp_stage = (size_t)a.nant * 16;
+    const size_t amp_time = (size_t)a.kpad * a.nant;
+    constexpr int STAGES_SPLIT = KSPLIT / 16;
+
+    for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
+        int unit, split, tile, bi, bj;
+        decode_work(a.nsplit, a.ntiles, Lw, unit, split, tile);
+        decode_tile(tile, a.nb, bi, bj);
+        const bool diag = (bi == bj);
+        const int s_begin = split * STAGES_SPLIT;
+        const int s_end = min(s_begin + STAGES_SPLIT, kst);
+        if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
+        const int n_st = s_end - s_begin;
+        const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
+        const double il4 = 4.0 * a.inv_lambda[fl];
+        const float* amp_t = a.amp + (size_t)tb * amp_time;
+        const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
+
+        unsigned smask = 0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int r0 = wr * 64 + i * 32, c0 = wc * 32;
+            const bool on = (bi * TILE + r0 < a.nant) && (bj * TILE + c0 < a.nant) && !(diag && r0 > c0);
+            smask |= (unsigned)on << i;
+        }
+        smask = __builtin_amdgcn_readfirstlane(smask);
+
+        // amplitude chunk c = (i*8 + wave)*64 + lane: row c/4, part c%4
+        int ampoff[L::NAMP];
+#pragma unroll
+        for (int i = 0; i < L::NAMP; ++i) {
+            const int c = (i * 8 + wave) * 64 + lane;
+            const int rr = c >> 2;
+            const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
+            ampoff[i] = ant * 16 + (c & 3) * 4;
+        }
+        const int crdoff = min(wave & 3, 2) * (int)a.kpad + 2 * (lane & 7);
+
+        f32x16 re[2], im[2];
+        const int h = wave >> 2;
+        if (diag) {
+            double ax[1], ay[1], az[1];
+            const int ant = min(bi * TILE + h * 64 + lane, a.nant - 1);
+            ax[0] = a.antpos[3 * ant]; ay[0] = a.antpos[3 * ant + 1]; az[0] = a.antpos[3 * ant + 2];
+            if (smask)
+                fused8_split<MODE, 1, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, true, re, im);
+            else
+                fused8_split<MODE, 1, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, true, re, im);
+        } else {
+            double ax[2], ay[2], az[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int rr = h * 128 + lane + 64 * i;
+                const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
+                ax[i] = a.antpos[3 * ant]; ay[i] = a.antpos[3 * ant + 1]; az[i] = a.antpos[3 * ant + 2];
+            }
+            if (smask)
+                fused8_split<MODE, 2, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, false, re, im);
+            else
+                fused8_split<MODE, 2, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, false, re, im);
+        }
+        if (!smask) continue;
+
+        constexpr size_t TSZ = (size_t)TILE * TILE;
+        float* out = reinterpret_cast<float*>(a.partials) +
+                     (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (!(smask & (1u << i))) continue;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                const int c = lane & 31;
+                const int row = wr * 64 + i * 32 + r;
+                const int col = wc * 32 + c;
+                out[(size_t)row * TILE + col] = re[i][e];
+                out[TSZ + (size_t)row * TILE + col] = im[i][e];
+            }
+        }
+    }
+}
+
 #undef HV_LDS
 
 // Resident workgroups of a fused kernel on the whole device (a multiple of 8).
@@ -1225,6 +1510,30 @@ static int64_t fused_resident() {
     return cached;
 }
 
+template <int MODE>
+static int64_t fused8_resident() {
+    static const int64_t cached = [] {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_fused8_kernel<MODE>, 512, 0) != hipSuccess)
+            return (int64_t)0;
+        return (int64_t)ncu * per_cu / 8 * 8;
+    }();
+    return cached;
+}
+
+template <int MODE>
+static hipError_t launch_fused8_t(const FusedArgs& a, hipStream_t s) {
+    const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
+    if (nblk <= 0) return hipSuccess;
+    if (a.kpad % 16) return hipErrorInvalidValue;
+    const int64_t res = fused8_resident<MODE>();
+    const int64_t grid = (res > 0 && res < nblk) ? res : nblk;
+    hipLaunchKernelGGL((herk_fused8_kernel<MODE>), dim3((unsigned)grid), dim3(512), 0, s, a);
+    return hipGetLastError();
+}
+
 template <int MODE, int TILE>
 static hipError_t launch_fused_t(const FusedArgs& a, hipStream_t s) {
     const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
@@ -1238,9 +1547,9 @@ static hipError_t launch_fused_t(const FusedArgs& a, hipStream_t s) {
 
 hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s) {
     if (mode == MODE_SPLIT_BF16 && tile == 64) return launch_fused_t<MODE_SPLIT_BF16, 64>(a, s);
-    if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_fused_t<MODE_SPLIT_BF16, 128>(a, s);
+    if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_fused8_t<MODE_SPLIT_BF16>(a, s);
     if (mode == MODE_SPLIT_FP16 && tile == 64) return launch_fused_t<MODE_SPLIT_FP16, 64>(a, s);
-    if (mode == MODE_SPLIT_FP16 && tile == 128) return launch_fused_t<MODE_SPLIT_FP16, 128>(a, s);
+    if (mode == MODE_SPLIT_FP16 && tile == 128) return launch_fused8_t<MODE_SPLIT_FP16>(a, s);
     return hipErrorInvalidValue;
 }
 
@@ -1285,15 +1594,21 @@ void herk_init() {
     (void)ring_resident<MODE_SPLIT_FP16, 128>();
     (void)ring_resident<MODE_FP64, 64>();
     (void)fused_resident<MODE_SPLIT_BF16, 64>();
-    (void)fused_resident<MODE_SPLIT_BF16, 128>();
+    (void)fused8_resident<MODE_SPLIT_BF16>();
     (void)fused_resident<MODE_SPLIT_FP16, 64>();
-    (void)fused_resident<MODE_SPLIT_FP16, 128>();
+    (void)fused8_resident<MODE_SPLIT_FP16>();
 }
 
 int herk_prefetch(int mode, int tile) {
-    if (g_pf_override) return g_pf_override;
-    (void)tile;
-    return split_mode(mode) ? 4 : 1;   // split modes: LDS ring on a persistent grid
+    if (g_pf_override && (g_pf_override != 5 || split_mode(mode)) && !(mode == MODE_FP64 && g_pf_override == 2))
+        return g_pf_override;
+    if (!split_mode(mode)) return 1;
+    // split modes: 128-tiles build their operands in place (measured 3.5%
+    // faster on the whole HERA-350 job than the phasor build + LDS ring, and
+    // no operand buffer); 64-tiles, whose in-place build would be repeated
+    // by six tiles per antenna block, contract phasor-built operands on the
+    // persistent LDS-ring grid
+    return tile == 128 ? 5 : 4;
 }
 
 // Contraction of operands already in HBM (the built-in variant, or with
