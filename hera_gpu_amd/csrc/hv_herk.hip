@@ -964,7 +964,7 @@ __device__ __forceinline__ void fused_issue(char* slot, const float* amp_st, con
 // rest to the B tile).
 template <int MODE, int TILE, int RPT>
 __device__ __forceinline__ void fused_build(const char* in, char* op, int lane, int g, const double (&ax)[RPT],
-                                            const double (&ay)[RPT], const double (&az)[RPT], double il4) {
+                                            const double (&ay)[RPT], const double (&az)[RPT], double il) {
     using L = FusedLds<TILE>;
     const f64x2* cx = reinterpret_cast<const f64x2*>(in + L::AMP + g * 32);
     const f64x2* cy = reinterpret_cast<const f64x2*>(in + L::AMP + 1024 + g * 32);
@@ -982,9 +982,9 @@ __device__ __forceinline__ void fused_build(const char* in, char* op, int lane, 
         for (int j = 0; j < 4; ++j) {
             const double d = fma(ax[i], xs[j], fma(ay[i], ys[j], az[i] * zs[j]));
             float sn, cs;
-            sincos_quarter_turns(d * il4, &sn, &cs);
+            sincos_turns_hw(d * il, &sn, &cs);
             uint32_t h, l;
-            split_pair<MODE>(am[j] * cs, am[j] * sn, h, l);
+            split_phasor<MODE>(am[j], cs, sn, h, l);
             hi[j] = h;
             lo[j] = l;
         }
@@ -1003,7 +1003,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
                                             const float* amp_t, const double* crd_t, size_t amp_stage, int s_begin,
                                             int n_st, const int* ampoff, int crdoff, int lane, int wave, int wr,
                                             int wc, const double (&ax)[RPT], const double (&ay)[RPT],
-                                            const double (&az)[RPT], double il4, bool diag,
+                                            const double (&az)[RPT], double il, bool diag,
                                             f32x16 (&tre)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
                                             f32x16 (&tim)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS]) {
     using C = Cfg<MODE, TILE>;
@@ -1035,7 +1035,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
     HV_FISSUE(in3, 3);
     wait_vmcnt<(FRING - 1) * PER>();
     __builtin_amdgcn_s_barrier();
-    fused_build<MODE, TILE, RPT>(in0, op0, lane, wave, ax, ay, az, il4);
+    fused_build<MODE, TILE, RPT>(in0, op0, lane, wave, ax, ay, az, il);
 
     for (int st = 0; st < n_st;) {
         const int g_end = min(st + RFOLD, n_st);
@@ -1064,7 +1064,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
                 read_frag<NS>(fa, cur, cur + boff, ra0, rb0, lh);
                 read_frag<NS>(fb, cur, cur + boff, ra0, rb0, 2 + lh);
                 // operand stage st+k+1 (past the end: rebuilt from a re-read, unused)
-                fused_build<MODE, TILE, RPT>(ins[(k + 1) % FRING], nxt, lane, wave, ax, ay, az, il4);
+                fused_build<MODE, TILE, RPT>(ins[(k + 1) % FRING], nxt, lane, wave, ax, ay, az, il);
                 if constexpr (KIND != 0) {
                     frag_mma<MODE, NS, true>(fa, 0u, re, im);
                     frag_mma<MODE, NS, true>(fb, 0u, re, im);
@@ -1128,7 +1128,7 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
         if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
         const int n_st = s_end - s_begin;
         const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
-        const double il4 = 4.0 * a.inv_lambda[fl];
+        const double il = a.inv_lambda[fl];   // turns per metre
         const float* amp_t = a.amp + (size_t)tb * amp_time;
         const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
 
@@ -1167,10 +1167,10 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
             }
             if (smask)
                 fused_split<MODE, TILE, RD, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, true, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, true, tre, tim);
             else
                 fused_split<MODE, TILE, RD, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, true, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, true, tre, tim);
         } else {
             double ax[RO], ay[RO], az[RO];
 #pragma unroll
@@ -1181,10 +1181,10 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
             }
             if (smask)
                 fused_split<MODE, TILE, RO, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, false, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, false, tre, tim);
             else
                 fused_split<MODE, TILE, RO, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il4, false, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, false, tre, tim);
         }
         if (!smask) continue;
 
@@ -1279,7 +1279,7 @@ __device__ __forceinline__ void fused8_issue(char* slot, const float* amp_st, co
 // input slot (as fused_build).
 template <int MODE, int RPT>
 __device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase, int g, const double (&ax)[RPT],
-                                             const double (&ay)[RPT], const double (&az)[RPT], double il4) {
+                                             const double (&ay)[RPT], const double (&az)[RPT], double il) {
     using L = Fused8Lds;
     constexpr int TILE = 128;
     const double* cx = reinterpret_cast<const double*>(in + L::AMP + g * 32);
@@ -1296,9 +1296,9 @@ __device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase
         for (int i = 0; i < RPT; ++i) {
             const double d = fma(ax[i], x, fma(ay[i], y, az[i] * z));
             float sn, cs;
-            sincos_quarter_turns(d * il4, &sn, &cs);
+            sincos_turns_hw(d * il, &sn, &cs);
             uint32_t hw, lw;
-            split_pair<MODE>(am[i][j] * cs, am[i][j] * sn, hw, lw);
+            split_phasor<MODE>(am[i][j], cs, sn, hw, lw);
             hi[i][j] = hw;
             lo[i][j] = lw;
         }
@@ -1330,7 +1330,7 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
                                              const float* amp_t, const double* crd_t, size_t amp_stage,
                                              int s_begin, int n_st, const int* ampoff, int crdoff, int lane,
                                              int wave, const double (&ax)[RPT], const double (&ay)[RPT],
-                                             const double (&az)[RPT], double il4, bool diag, f32x16 (&re)[2],
+                                             const double (&az)[RPT], double il, bool diag, f32x16 (&re)[2],
                                              f32x16 (&im)[2]) {
     using L = Fused8Lds;
     constexpr int TILE = 128;
@@ -1361,7 +1361,7 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
     HV_F8ISSUE(in3, 3);
     wait_vmcnt<(FRING - 1) * PER>();
     __builtin_amdgcn_s_barrier();
-    fused8_build<MODE, RPT>(in0, op0, rbase, g, ax, ay, az, il4);
+    fused8_build<MODE, RPT>(in0, op0, rbase, g, ax, ay, az, il);
 
     for (int st = 0; st < n_st; st += FRING) {
 #pragma unroll
@@ -1375,13 +1375,13 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
             // MFMAs in opposite order, so one's matrix work runs while the
             // other's VALU work does
             if (h == 0) {
-                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il4);
+                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il);
                 __builtin_amdgcn_sched_barrier(0);
                 if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
             } else {
                 if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
                 __builtin_amdgcn_sched_barrier(0);
-                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il4);
+                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il);
             }
         }
     }
@@ -1421,7 +1421,7 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
         if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
         const int n_st = s_end - s_begin;
         const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
-        const double il4 = 4.0 * a.inv_lambda[fl];
+        const double il = a.inv_lambda[fl];   // turns per metre
         const float* amp_t = a.amp + (size_t)tb * amp_time;
         const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
 
@@ -1453,10 +1453,10 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
             ax[0] = a.antpos[3 * ant]; ay[0] = a.antpos[3 * ant + 1]; az[0] = a.antpos[3 * ant + 2];
             if (smask)
                 fused8_split<MODE, 1, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, true, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il, true, re, im);
             else
                 fused8_split<MODE, 1, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, true, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il, true, re, im);
         } else {
             double ax[2], ay[2], az[2];
 #pragma unroll
@@ -1467,10 +1467,10 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
             }
             if (smask)
                 fused8_split<MODE, 2, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, false, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il, false, re, im);
             else
                 fused8_split<MODE, 2, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il4, false, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, il, false, re, im);
         }
         if (!smask) continue;
 

@@ -64,6 +64,20 @@ __device__ inline void sincos_quarter_turns(double t4, double* s, double* c) {
     *c = ((k + 1) & 2) ? -cv : cv;
 }
 
+// sin and cos of 2 pi t for a phase t in turns (split-mode operands): the
+// remainder r = t - rint(t) in [-1/2, 1/2] is exact in FP64 and rounded once
+// to FP32, then the hardware sine and cosine, which take revolutions
+// (measured max error 1.25e-7 on [-1/2, 1/2] against FP64 sincospi; with the
+// FP32 rounding of r, < 3e-7 in all, far below the 2^-17 of a hi+lo
+// split).  Two instructions replace the polynomials and the quadrant
+// rotation.
+__device__ inline void sincos_turns_hw(double t, float* s, float* c) {
+#pragma clang fp contract(off)
+    const float r = (float)(t - rint(t));
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
+
 // Round-to-nearest-even float -> bf16 / fp16 bit patterns and back.
 __device__ inline uint16_t to_bf16(float x) {
     __bf16 h = (__bf16)x;
@@ -95,6 +109,30 @@ __device__ inline void split_pair(float re, float im, uint32_t& hi, uint32_t& lo
         hi = __builtin_bit_cast(uint32_t, h);
         const hv_f32x2 b = __builtin_convertvector(h, hv_f32x2);
         lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((hv_f32x2){re - b[0], im - b[1]}, hv_f16x2));
+    }
+}
+
+// hi and lo words of the phasor am (cs, sn), fixed arithmetic (no
+// contraction): hi = RNE(round(am cs), round(am sn)) and lo =
+// RNE(fma(am, cs, -hi_x), fma(am, sn, -hi_y)), so every kernel that builds
+// operands produces the same bits.
+template <int MODE>
+__device__ inline void split_phasor(float am, float cs, float sn, uint32_t& hi, uint32_t& lo) {
+#pragma clang fp contract(off)
+    const float re = am * cs, im = am * sn;
+    float br, bm;
+    if constexpr (MODE == MODE_SPLIT_BF16) {
+        hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((hv_f32x2){re, im}, hv_bf16x2));
+        br = __uint_as_float(hi << 16);
+        bm = __uint_as_float(hi & 0xffff0000u);
+        lo = __builtin_bit_cast(uint32_t,
+                                __builtin_convertvector((hv_f32x2){fmaf(am, cs, -br), fmaf(am, sn, -bm)}, hv_bf16x2));
+    } else {
+        const hv_f16x2 h = __builtin_convertvector((hv_f32x2){re, im}, hv_f16x2);
+        hi = __builtin_bit_cast(uint32_t, h);
+        const hv_f32x2 b = __builtin_convertvector(h, hv_f32x2);
+        lo = __builtin_bit_cast(uint32_t,
+                                __builtin_convertvector((hv_f32x2){fmaf(am, cs, -b[0]), fmaf(am, sn, -b[1])}, hv_f16x2));
     }
 }
 

@@ -110,17 +110,35 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
             d[j] = fma(ax, x, fma(ay, y, az * z));
         }
         for (int u = 0; u < a.nfreq; ++u) {
-            const double il4 = 4.0 * a.inv_lambda[u];   // quarter turns per metre (exact scaling)
-            R re[8], im[8];
+            const double il = a.inv_lambda[u];          // turns per metre
+            const double il4 = 4.0 * il;                // quarter turns per metre (exact scaling)
+            char* dst = wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
+                        (size_t)(s0 % SST) * elem;
+            if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
+                // two 32-byte groups [4 hi words | 4 lo words]
+                uint32_t hw[8], lw[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                R sn, cs;
-                sincos_quarter_turns(d[j] * il4, &sn, &cs);
-                re[j] = amp[j] * cs;
-                im[j] = amp[j] * sn;
+                for (int j = 0; j < 8; ++j) {
+                    float sn, cs;
+                    sincos_turns_hw(d[j] * il, &sn, &cs);
+                    split_phasor<MODE>(amp[j], cs, sn, hw[j], lw[j]);
+                }
+                uint4* o = reinterpret_cast<uint4*>(dst);
+                o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+                o[1] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+                o[2] = make_uint4(hw[4], hw[5], hw[6], hw[7]);
+                o[3] = make_uint4(lw[4], lw[5], lw[6], lw[7]);
+            } else {
+                R re[8], im[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    R sn, cs;
+                    sincos_quarter_turns(d[j] * il4, &sn, &cs);
+                    re[j] = amp[j] * cs;
+                    im[j] = amp[j] * sn;
+                }
+                store_phasors8<MODE, R>(dst, re, im);
             }
-            store_phasors8<MODE, R>(wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
-                                        (size_t)(s0 % SST) * elem, re, im);
         }
     }
 }
