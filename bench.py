@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # same counters read for a 1 GiB device copy: FETCH_SIZE x2.0, WRITE_SIZE x1.0.
 MEASURED_TRAFFIC = {
     # operands built in place (herk_fused8_kernel), one launch per time
-    ("hera350", "split_bf16", 64): {"fetch_kib": 133064700, "write_kib": 4325376, "fetch_scale": 2.0,
+    ("hera350", "split_bf16", 64): {"fetch_kib": 138366362, "write_kib": 4325376, "fetch_scale": 2.0,
                                     "write_scale": 1.0},
     # phasor build + LDS ring (--herk-variant 128,4), 8 units per launch
     ("hera350", "split_bf16", 8): {"fetch_kib": 12448316, "write_kib": 540672, "fetch_scale": 2.0,
