@@ -47,11 +47,11 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDS_STRIDE = 144;
 
-// Built-in variants (measured on MI355X): split modes take 128-tiles at 256+
-// antennas (the tiles of a split share their operand stages in L2: HBM reads
-// ~1.1x the operand bytes against ~3.5x for 64-tiles) and 64-tiles below,
-// which waste less on the triangle's edge.  FP32 and FP64 keep the
-// register-staged 64-tile kernel, their fastest.
+// Built-in variants (measured on MI355X): split modes take 128-tiles above
+// 64 antennas (fewer tiles share, and rebuild, each antenna block; for
+// operands in HBM the tiles of a split share their stages in L2: reads ~1.1x
+// the operand bytes against ~3.5x for 64-tiles) and one 64-tile below.  FP32
+// and FP64 keep the register-staged 64-tile kernel, their fastest.
 static int g_tile_override = 0, g_pf_override = 0;   // 0 = built-in choice
 
 void set_herk_variant(int tile, int pf) {
@@ -65,7 +65,7 @@ static bool split_mode(int mode) { return mode == MODE_SPLIT_BF16 || mode == MOD
 // in-place operand builds are split-mode kernels).
 int herk_tile(int mode, int nant) {
     if (g_tile_override && (g_tile_override == 64 || split_mode(mode))) return g_tile_override;
-    return (split_mode(mode) && nant >= 256) ? 128 : 64;
+    return (split_mode(mode) && nant > 64) ? 128 : 64;
 }
 
 // (x, y) pairs of 16-bit values -> (y, -x): rotate each 32-bit word by 16 and
@@ -1602,13 +1602,11 @@ void herk_init() {
 int herk_prefetch(int mode, int tile) {
     if (g_pf_override && (g_pf_override != 5 || split_mode(mode)) && !(mode == MODE_FP64 && g_pf_override == 2))
         return g_pf_override;
-    if (!split_mode(mode)) return 1;
-    // split modes: 128-tiles build their operands in place (measured 3.5%
-    // faster on the whole HERA-350 job than the phasor build + LDS ring, and
-    // no operand buffer); 64-tiles, whose in-place build would be repeated
-    // by six tiles per antenna block, contract phasor-built operands on the
-    // persistent LDS-ring grid
-    return tile == 128 ? 5 : 4;
+    (void)tile;
+    // split modes build their operands in place (whole simulations, 2^18
+    // sources: faster than the phasor build + LDS ring at every array size
+    // measured, 1.2x at N = 37 and 350, 1.6x at N = 128, and no operand buffer)
+    return split_mode(mode) ? 5 : 1;
 }
 
 // Contraction of operands already in HBM (the built-in variant, or with

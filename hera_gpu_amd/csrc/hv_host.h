@@ -37,6 +37,7 @@ enum Buf {
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    size_t cap_seen = 0;   // largest capacity held so far (growth base)
 };
 
 // A set of grow-only device buffers.  The context owns one; every serving

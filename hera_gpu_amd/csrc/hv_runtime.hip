@@ -81,9 +81,18 @@ int hv::ensure(hv_ctx* c, Workspace& ws, Buf b, size_t bytes, hipStream_t stream
         d.p = nullptr;
         d.cap = 0;
     }
+    // the context's own workspace (direct calls) grows geometrically (x1.5)
+    // so a stream of growing requests reallocates O(log) times, falling back
+    // to the exact size when the budget is tight; serving slots take exact
+    // sizes, which their admission accounts for
     size_t want = std::max(bytes, (size_t)256);
+    const size_t grown = (&ws == &c->ws) ? std::max(want, d.cap_seen + d.cap_seen / 2) : want;
+    if (grown > want && dev_alloc(c, &d.p, grown) == HV_OK) {
+        d.cap = d.cap_seen = grown;
+        return HV_OK;
+    }
     HV_CHECK(dev_alloc(c, &d.p, want));
-    d.cap = want;
+    d.cap = d.cap_seen = want;
     return HV_OK;
 }
 

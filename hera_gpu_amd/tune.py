@@ -62,6 +62,41 @@ def sweep_accuracy(gemms):
             print(json.dumps({"accuracy": gemm, "tile": tile, "pf": pf, "rel_err": err}), flush=True)
 
 
+PIPELINE_VARIANTS = ((64, 4), (128, 4), (128, 5), (64, 5))
+
+
+def sweep_pipeline(nants, nsrc, ntimes, nfreq, gemm):
+    """Whole simulation (rotation, operand build, contraction, reduce) per
+    variant on a synthetic sky: ms per (time, frequency) unit."""
+    lib = _lib.load()
+    rng = np.random.default_rng(1)
+    z = rng.uniform(-1, 1, nsrc)
+    ph = rng.uniform(0, 2 * np.pi, nsrc)
+    crd = np.stack([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z]).astype(np.float32)
+    flux = rng.lognormal(0, 1, nsrc).astype(np.float32)
+    sky = stages.Sky(crd, flux)
+    eq = configs.eq2tops_lst(ntimes)
+    freqs = np.linspace(100e6, 200e6, nfreq)
+    for nant in nants:
+        ap = configs.hera350_layout(np.random.default_rng(3))[:nant].astype(np.float32)
+        beam = stages.Beam(configs.gaussian_beams(np.random.default_rng(4), nant, 63))
+        for tile, pf in PIPELINE_VARIANTS:
+            _lib.check(lib.hv_set_herk_variant(tile, pf), "hv_set_herk_variant")
+            try:
+                stages.simulate(sky, beam, ap, freqs, eq, gemm=gemm, want_output=False, dtype=np.float32)
+                _, t = stages.simulate(sky, beam, ap, freqs, eq, gemm=gemm, want_output=False, timing=True,
+                                       dtype=np.float32)
+            finally:
+                _lib.check(lib.hv_set_herk_variant(0, 0), "hv_set_herk_variant")
+            units = ntimes * nfreq
+            print(json.dumps({"pipeline": gemm, "nant": nant, "nsrc": nsrc, "tile": tile, "pf": pf,
+                              "ms_per_unit": t["total_ms"] / units,
+                              "tflops": 4.0 * nant * (nant + 1) * nsrc * units / (t["total_ms"] * 1e-3) / 1e12}),
+                  flush=True)
+        beam.close()
+    sky.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nant", type=int, nargs="+", default=[350, 128])
@@ -69,7 +104,12 @@ def main():
     ap.add_argument("--units", type=int, default=8)
     ap.add_argument("--gemm", nargs="+", default=list(VARIANTS))
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--pipeline", action="store_true",
+                    help="time whole simulations per variant (split-BF16) instead of the contraction alone")
     args = ap.parse_args()
+    if args.pipeline:
+        sweep_pipeline(args.nant, args.nsrc, 2, 16, "split_bf16")
+        return
     sweep_speed(args.nant, args.nsrc, args.units, args.gemm)
     if not args.no_accuracy:
         sweep_accuracy(args.gemm)
