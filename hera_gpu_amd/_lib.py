@@ -75,6 +75,7 @@ SIGNATURES = {
     "hv_ctx_create": (_i, [_i, _sz, C.POINTER(_vp)]),
     "hv_ctx_destroy": (_i, [_vp]),
     "hv_vis": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _vp]),
+    "hv_vis_ex": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _i, _vp]),
     "hv_k_rotate": (_i, [_vp, _i, _vp, _vp, _i64, _vp]),
     "hv_k_beam": (_i, [_vp, _i, _vp, _i, _i, _vp, _i64, _vp]),
     "hv_k_phasor": (_i, [_vp, _i, _i, _vp, _i, _i, _vp, _vp, _i64, _vp, _d, _vp]),

@@ -105,7 +105,7 @@ struct Plan {
     size_t footprint() const { return crd_top_bytes + w_bytes + part_bytes + out_bytes; }
 };
 
-Plan plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t operand_budget);
+Plan plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t operand_budget, int layout = 0);
 
 struct JobDev {
     int mode;
@@ -120,6 +120,7 @@ struct JobDev {
     void* vis;                  // [F][T][N][N] complex, device
     float amp_scale;
     const int64_t* dyn_kpad;    // optional device-side source count
+    int layout = 0;             // output layout (Layout)
 };
 
 struct StageTimer {

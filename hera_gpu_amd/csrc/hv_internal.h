@@ -104,9 +104,18 @@ struct HerkArgs {
     const int64_t* dyn_kpad;
 };
 
+// Output layouts of the visibilities (HV_LAYOUT_* in heravis.h).
+enum Layout { LAYOUT_FULL = 0, LAYOUT_FTB = 1, LAYOUT_TBF = 2 };
+// Complex elements of an output: full N x N per unit, or the N (N+1) / 2
+// baselines a <= b (index a N - a (a-1) / 2 + (b - a)).
+inline size_t out_elems(int layout, int nfreq, int ntimes, int nant) {
+    const size_t per = (layout == LAYOUT_FULL) ? (size_t)nant * nant : (size_t)nant * (nant + 1) / 2;
+    return (size_t)nfreq * ntimes * per;
+}
+
 struct ReduceArgs {
     const void* partials;
-    void* vis;               // complex[F][T][N][N]
+    void* vis;               // complex[F][T][N][N], [F][T][Nbl] or [T][Nbl][F] (layout)
     int t0, f0;              // first time / frequency of the launch
     int nfreq_launch;        // unit u of the launch = (t0 + u / nfreq_launch,
                              //                        f0 + u % nfreq_launch)
@@ -119,6 +128,8 @@ struct ReduceArgs {
     int tile;
     double scale;            // 1 / amp_scale^2
     const int64_t* dyn_kpad; // splits summed = ceil(*dyn_kpad / KSPLIT)
+    int layout;              // Layout
+    int nfreq_total;         // F of the output array
 };
 
 struct BeamArgs {

@@ -223,7 +223,7 @@ static size_t operand_budget(hv_ctx* c) {
     return std::min((size_t)24 << 30, c->budget / 3);
 }
 
-Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t wbudget) {
+Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t wbudget, int layout) {
     Plan p;
     p.mode = mode;
     p.nant = nant;
@@ -255,7 +255,7 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.crd_top_bytes = (size_t)ntimes * 3 * kpad * sizeof(double);
     p.w_bytes = p.fused ? p.amp_time * p.tb : p.unit_bytes * units;
     p.part_bytes = p.part_unit * units;
-    p.out_bytes = (size_t)nfreq * ntimes * nant * nant * (mode_is_f64(mode) ? 16 : 8);
+    p.out_bytes = out_elems(layout, nfreq, ntimes, nant) * (mode_is_f64(mode) ? 16 : 8);
     return p;
 }
 
@@ -337,6 +337,8 @@ static int enqueue_reduce(const JobDev& j, const Plan& p, void* part, int t0, in
     ra.tile = p.tile;
     ra.scale = 1.0 / ((double)j.amp_scale * (double)j.amp_scale);
     ra.dyn_kpad = j.dyn_kpad;
+    ra.layout = j.layout;
+    ra.nfreq_total = j.nfreq;
     HV_CHECK(tm.begin(3, stream));
     HV_HIP(launch_reduce(j.mode, ra, stream));
     HV_CHECK(tm.end(3, stream));
@@ -453,14 +455,14 @@ static bool overlap_batches() {
 }
 
 static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
-    Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c));
+    Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c), j.layout);
     const int64_t batches = (int64_t)((j.ntimes + p.tb - 1) / p.tb) * ((j.nfreq + p.fb - 1) / p.fb);
     if (batches < 2 || !overlap_batches() || p.fused) {
         HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
         return enqueue_job(j, p, c->ws, c->stream, tm);
     }
     // several batches: double-buffer operands and partials at half the budget each
-    p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c) / 2);
+    p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c) / 2, j.layout);
     HV_CHECK(ensure(c, c->ws, B_CRD_TOP, p.crd_top_bytes, c->stream));
     HV_CHECK(ensure(c, c->ws, B_W, 2 * p.w_bytes, c->stream));
     HV_CHECK(ensure(c, c->ws, B_PART, 2 * p.part_bytes, c->stream));
@@ -521,7 +523,15 @@ static double max_abs(const void* p, int dtype, size_t n) {
 extern "C" int hv_vis(hv_ctx* c, int mode, int dtype, const void* antpos, int nant, const double* freqs,
                       int nfreq, const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt,
                       int64_t nsrc, const void* bm_cube, int beam_px, void* vis_out) {
+    return hv_vis_ex(c, mode, dtype, antpos, nant, freqs, nfreq, eq2tops, ntimes, crd_eq, isqrt, nsrc, bm_cube,
+                     beam_px, HV_LAYOUT_FULL, vis_out);
+}
+
+extern "C" int hv_vis_ex(hv_ctx* c, int mode, int dtype, const void* antpos, int nant, const double* freqs,
+                         int nfreq, const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt,
+                         int64_t nsrc, const void* bm_cube, int beam_px, int layout, void* vis_out) {
     if (!c) return fail(HV_ERR_ARG, "ctx is NULL");
+    if (layout < HV_LAYOUT_FULL || layout > HV_LAYOUT_TBF) return fail(HV_ERR_ARG, "unknown output layout");
     HV_CHECK(check_job_args(mode, dtype, nant, nfreq, ntimes, nsrc, beam_px));
     if (!antpos || !freqs || !eq2tops || !bm_cube || !vis_out || (nsrc > 0 && (!crd_eq || !isqrt)))
         return fail(HV_ERR_ARG, "NULL array argument");
@@ -543,10 +553,11 @@ extern "C" int hv_vis(hv_ctx* c, int mode, int dtype, const void* antpos, int na
     HV_CHECK(upload(c, B_ISQRT, isq.data(), isq.size() * 8));
     HV_CHECK(upload(c, B_BEAM, bm_cube, beam_bytes));
     const size_t cplx = mode_is_f64(mode) ? 16 : 8;
-    const size_t out_bytes = (size_t)nfreq * ntimes * nant * nant * cplx;
+    const size_t out_bytes = out_elems(layout, nfreq, ntimes, nant) * cplx;
     HV_CHECK(ensure(c, B_OUT, out_bytes));
 
     JobDev j{};
+    j.layout = layout;
     j.mode = mode;
     j.nant = nant; j.ntimes = ntimes; j.nfreq = nfreq; j.beam_px = beam_px; j.beam_f64 = dtype == HV_F64;
     j.kpad = kpad;

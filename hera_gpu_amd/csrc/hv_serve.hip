@@ -45,9 +45,10 @@ struct BucketKey {
     int mode, nant, beam_px, ntimes, nfreq, beam_f64;
     float scale;
     int64_t kbucket;
+    int layout;
     bool operator<(const BucketKey& o) const {
-        return std::tie(mode, nant, beam_px, ntimes, nfreq, beam_f64, scale, kbucket) <
-               std::tie(o.mode, o.nant, o.beam_px, o.ntimes, o.nfreq, o.beam_f64, o.scale, o.kbucket);
+        return std::tie(mode, nant, beam_px, ntimes, nfreq, beam_f64, scale, kbucket, layout) <
+               std::tie(o.mode, o.nant, o.beam_px, o.ntimes, o.nfreq, o.beam_f64, o.scale, o.kbucket, o.layout);
     }
 };
 
@@ -58,6 +59,7 @@ struct SrvJob {
     const hv_beam* beam = nullptr;
     int64_t nsrc = 0, kpad = 0;
     int nant = 0, ntimes = 0, nfreq = 0;
+    int layout = 0;            // output layout from the job flags
     std::vector<double> antpos, invlam, eq2tops;
     void* out = nullptr;
     BucketKey key{};
@@ -195,6 +197,7 @@ static int launch_job(hv_server* s, Slot& sl, SrvJob& j) {
     jd.vis = w.buf[B_OUT].p;
     jd.amp_scale = j.key.scale;
     jd.dyn_kpad = (const int64_t*)w.buf[B_DYN].p;
+    jd.layout = j.layout;
 
     // A bucket is captured the second time it is seen: a shape that comes
     // once runs direct (capture and instantiation of a large job's graph
@@ -455,8 +458,9 @@ extern "C" int hv_server_submit(hv_server* s, const hv_job* job, int64_t* id) {
     int64_t kb = 1024;
     while (kb < j->kpad) kb <<= 1;
     const float scale = (j->mode == HV_MODE_SPLIT_FP16) ? fp16_scale(beam->max_abs * sky->max_isqrt) : 1.0f;
-    j->key = BucketKey{j->mode, j->nant, beam->beam_px, j->ntimes, j->nfreq, beam->f64, scale, kb};
-    j->plan = plan_job(j->mode, j->nant, kb, j->ntimes, j->nfreq, SERVE_OPERAND_BUDGET);
+    j->layout = (job->flags & HV_JOB_TBF) ? LAYOUT_TBF : (job->flags & HV_JOB_FTB) ? LAYOUT_FTB : LAYOUT_FULL;
+    j->key = BucketKey{j->mode, j->nant, beam->beam_px, j->ntimes, j->nfreq, beam->f64, scale, kb, j->layout};
+    j->plan = plan_job(j->mode, j->nant, kb, j->ntimes, j->nfreq, SERVE_OPERAND_BUDGET, j->layout);
     j->out_bytes = j->plan.out_bytes;
     j->footprint = slot_bytes_needed(*j);
     if (j->footprint > s->budget)

@@ -311,6 +311,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
     const int a = blockIdx.y * 8 + (threadIdx.x >> 5);
     const int u = blockIdx.z;
     if (a >= r.nant || b >= r.nant) return;
+    if (r.layout != LAYOUT_FULL && a > b) return;        // baseline layouts hold a <= b only
     const int T = r.tile;
     const int ta = a / T, tb = b / T;
     const bool mirror = (ta > tb) || (ta == tb && a > b);
@@ -329,8 +330,16 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
     if (mirror) im = -im;
     if (a == b) im = 0.0;
     const int64_t t = r.t0 + u / r.nfreq_launch, f = r.f0 + u % r.nfreq_launch;
-    const int64_t unit_off = (f * r.ntimes_total + t) * (int64_t)r.nant * r.nant;
-    OUT* o = reinterpret_cast<OUT*>(r.vis) + 2 * (unit_off + (int64_t)a * r.nant + b);
+    int64_t idx;
+    if (r.layout == LAYOUT_FULL) {
+        idx = (f * r.ntimes_total + t) * (int64_t)r.nant * r.nant + (int64_t)a * r.nant + b;
+    } else {
+        const int64_t nbl = (int64_t)r.nant * (r.nant + 1) / 2;
+        const int64_t bl = (int64_t)a * r.nant - (int64_t)a * (a - 1) / 2 + (b - a);
+        idx = (r.layout == LAYOUT_FTB) ? (f * r.ntimes_total + t) * nbl + bl
+                                       : (t * nbl + bl) * r.nfreq_total + f;
+    }
+    OUT* o = reinterpret_cast<OUT*>(r.vis) + 2 * idx;
     o[0] = (OUT)(re * r.scale);
     o[1] = (OUT)(im * r.scale);
 }

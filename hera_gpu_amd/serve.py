@@ -25,7 +25,9 @@ from . import _lib, stages
 JobInfo = _lib.JobInfo
 Job = _lib.Job
 ServerStats = _lib.ServerStats
-FAIL_TEST = 1
+FAIL_TEST = 1     # HV_JOB_FAIL_TEST
+FTB = 2           # HV_JOB_FTB: result (F, T, Nbl), baselines a <= b
+TBF = 4           # HV_JOB_TBF: result (T, Nbl, F)
 
 
 def _bind():
@@ -65,7 +67,14 @@ class Server:
         eq = np.ascontiguousarray(eq2tops, dtype=np.float64)
         n, t, f = ap.shape[0], eq.shape[0], fr.shape[0]
         cd = np.complex128 if mode == _lib.MODES["fp64"] else np.complex64
-        out = np.empty((f, t, n, n), dtype=cd)
+        nbl = n * (n + 1) // 2
+        if flags & TBF:
+            shape = (t, nbl, f)
+        elif flags & FTB:
+            shape = (f, t, nbl)
+        else:
+            shape = (f, t, n, n)
+        out = np.empty(shape, dtype=cd)
         job = Job(mode, flags, sky.handle, sky.nsrc if nsrc is None else int(nsrc), beam.handle,
                   ap.ctypes.data, n, fr.ctypes.data, f, eq.ctypes.data, t, out.ctypes.data)
         jid = C.c_int64()

@@ -58,6 +58,16 @@ def _resolve(precision, real_dtype, complex_dtype, gemm):
     return np.float64, np.complex128, _lib.MODES["fp64"], _lib.HV_F64
 
 
+LAYOUTS = {"full": 0, "ftb": 1, "tbf": 2}
+
+
+def baseline_pairs(nant):
+    """(ant1, ant2) of the baseline layouts: every a <= b, a-major, autos included
+    (baseline index a N - a (a-1) / 2 + (b - a))."""
+    ant1, ant2 = np.triu_indices(nant)
+    return ant1.astype(np.int32), ant2.astype(np.int32)
+
+
 def vis_gpu(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube,
             nthreads=NTHREADS, max_memory=MAX_MEMORY, precision=1, verbose=False,
             *, real_dtype=None, complex_dtype=None, gemm=None, device=0):
@@ -66,6 +76,32 @@ def vis_gpu(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube,
     v_a(s) = A_a(s) sqrt(I(s)) exp(+2 pi i nu antpos_a . crd_top(s) / c), with
     A the clamp-to-edge bilinear beam, zero unless crd_top z > 0.
     """
+    out, scalar_freq = _vis(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube, precision, real_dtype, complex_dtype,
+                            gemm, device, "full")
+    return out[0] if scalar_freq else out
+
+
+def vis_gpu_baselines(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube, precision=1, *, order="tbf",
+                      real_dtype=None, complex_dtype=None, gemm=None, device=0):
+    """The visibilities of the baselines a <= b only, written on the device in
+    baseline order (half the bytes of the (T, N, N) matrices).
+
+    order "tbf": data (ntimes, nbl, nfreq) -- time-major baseline-times
+    (Nblts = ntimes * nbl) by frequency, the order of a UVData data array;
+    order "ftb": data (nfreq, ntimes, nbl).  Returns (data, ant1, ant2) with
+    ant1 <= ant2 the antenna indices of the nbl = N (N+1) / 2 baselines.
+    Same arguments and arithmetic as vis_gpu (data[..., k] equals
+    vis_gpu(...)[..., ant1[k], ant2[k]] bit for bit).
+    """
+    assert order in ("tbf", "ftb"), "order must be 'tbf' or 'ftb'"
+    out, _ = _vis(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube, precision, real_dtype, complex_dtype, gemm,
+                  device, order)
+    ant1, ant2 = baseline_pairs(np.asarray(antpos).shape[0])
+    return out, ant1, ant2
+
+
+def _vis(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube, precision, real_dtype, complex_dtype, gemm, device,
+         layout):
     real_dtype, complex_dtype, mode, dtype = _resolve(precision, real_dtype, complex_dtype, gemm)
     antpos = np.asarray(antpos)
     eq2tops = np.asarray(eq2tops)
@@ -95,10 +131,13 @@ def vis_gpu(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube,
     isqrt = np.ascontiguousarray(np.sqrt(I_sky).astype(real_dtype))
     bm_cube = np.ascontiguousarray(bm_cube, dtype=real_dtype)
 
-    out = np.empty((freqs.size, ntimes, nant, nant), dtype=complex_dtype)
+    nbl = nant * (nant + 1) // 2
+    shape = {"full": (freqs.size, ntimes, nant, nant), "ftb": (freqs.size, ntimes, nbl),
+             "tbf": (ntimes, nbl, freqs.size)}[layout]
+    out = np.empty(shape, dtype=complex_dtype)
     ctx = _lib.context(device)
-    rc = _lib.load().hv_vis(ctx.handle, mode, dtype, _lib.ptr(antpos), nant, _lib.ptr(freqs), freqs.size,
-                            _lib.ptr(eq2tops), ntimes, _lib.ptr(crd_eq), _lib.ptr(isqrt), npix,
-                            _lib.ptr(bm_cube), beam_px, _lib.ptr(out))
+    rc = _lib.load().hv_vis_ex(ctx.handle, mode, dtype, _lib.ptr(antpos), nant, _lib.ptr(freqs), freqs.size,
+                               _lib.ptr(eq2tops), ntimes, _lib.ptr(crd_eq), _lib.ptr(isqrt), npix,
+                               _lib.ptr(bm_cube), beam_px, LAYOUTS[layout], _lib.ptr(out))
     _lib.check(rc, "hv_vis")
-    return out[0] if scalar_freq else out
+    return out, scalar_freq

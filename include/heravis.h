@@ -82,6 +82,25 @@ int hv_vis(hv_ctx* ctx, int mode, int dtype,
            const void* bm_cube, int beam_px,
            void* vis_out);
 
+/* Output layouts.  The baseline layouts hold the N (N+1) / 2 baselines
+ * (a, b), a <= b (autos included), at index a N - a (a-1) / 2 + (b - a), i.e.
+ * V[a][b] = sum v_a conj(v_b) of the full layout; half the bytes of the full
+ * matrix.  No counterpart in the reference (which returns (T,N,N),
+ * vis.py:212,262): the output writer that turns (T,N,N) into the baseline-
+ * major, time-major (Nblts, Nfreqs) order of a UVData data array. */
+#define HV_LAYOUT_FULL 0   /* (F, T, N, N)                 */
+#define HV_LAYOUT_FTB 1    /* (F, T, Nbl)                  */
+#define HV_LAYOUT_TBF 2    /* (T, Nbl, F): UVData blt order */
+
+/* hv_vis with an output layout (hv_vis = HV_LAYOUT_FULL). */
+int hv_vis_ex(hv_ctx* ctx, int mode, int dtype,
+              const void* antpos, int nant,
+              const double* freqs, int nfreq,
+              const void* eq2tops, int ntimes,
+              const void* crd_eq, const void* isqrt, int64_t nsrc,
+              const void* bm_cube, int beam_px,
+              int layout, void* vis_out);
+
 /* ---------------------------------------------------------------------------
  * Kernel-level entry points, one device stage each, host arrays in and out.
  * ------------------------------------------------------------------------- */
@@ -182,10 +201,12 @@ int hv_herk_choice(int mode, int nant, int* tile, int* pf);
 typedef struct hv_server hv_server;
 
 #define HV_JOB_FAIL_TEST 1   /* flags: fail this job at admission (host-side test hook) */
+#define HV_JOB_FTB 2         /* flags: write vis_out in HV_LAYOUT_FTB                  */
+#define HV_JOB_TBF 4         /* flags: write vis_out in HV_LAYOUT_TBF                  */
 
 typedef struct hv_job {
     int mode;                  /* HV_MODE_*                                        */
-    int flags;                 /* 0 or HV_JOB_FAIL_TEST                            */
+    int flags;                 /* HV_JOB_* bits                                    */
     const hv_sky* sky;         /* resident catalog; the first nsrc sources are used */
     int64_t nsrc;
     const hv_beam* beam;       /* resident beam, nant x P x P                      */
@@ -196,7 +217,8 @@ typedef struct hv_job {
     const double* eq2tops;     /* host (ntimes,3,3), copied at submit              */
     int ntimes;
     void* vis_out;             /* host (F,T,N,N) complex64 (complex128 for FP64),
-                                  written before the job is reported finished      */
+                                  or a baseline layout (flags), written before the
+                                  job is reported finished                         */
 } hv_job;
 
 typedef struct hv_job_info {

@@ -113,3 +113,16 @@ def test_failed_job_is_isolated(catalogs):
     with Server(nslots=1) as srv:
         with pytest.raises(AssertionError):
             srv.submit(sky, beam, antpos, freqs, eq, nsrc=10 ** 7)
+
+
+def test_baseline_layout_jobs_equal_full_jobs(catalogs):
+    from hera_gpu_amd import baseline_pairs
+    from hera_gpu_amd.serve import FTB, TBF
+    antpos, beam, sky, eq, freqs = job_inputs(catalogs, 7, "disk128", 2, 5)
+    with Server(nslots=2) as srv:
+        ids = [srv.submit(sky, beam, antpos, freqs, eq, nsrc=12345, flags=fl) for fl in (0, FTB, TBF)]
+        full, ftb, tbf = (srv.wait(i, 120) for i in ids)
+    assert full.status == 0 and ftb.status == 0 and tbf.status == 0
+    a1, a2 = baseline_pairs(128)
+    np.testing.assert_array_equal(ftb.vis, full.vis[:, :, a1, a2])
+    np.testing.assert_array_equal(tbf.vis, full.vis[:, :, a1, a2].transpose(1, 2, 0))

@@ -234,3 +234,23 @@ def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile):
     np.testing.assert_array_equal(outs[0], outs[1])
     ref = vis_oracle(ap, w.freqs[-1:], w.eq2tops[:1], w.crd_eq, w.I_sky, bm)[0, 0]
     assert rel_err(outs[1][-1, 0], ref) < VIS_TOL[gemm]
+
+
+@pytest.mark.parametrize("gemm", ["fp32", "split_bf16", "fp64"])
+def test_baseline_layouts_equal_full_matrix_bitwise(gemm):
+    # the reduce writes only a <= b, in (T, Nbl, F) or (F, T, Nbl) order; the
+    # values are those of the full (F, T, N, N) output bit for bit
+    from hera_gpu_amd import vis_gpu_baselines
+    w = configs.make_disk128(nsrc=20000, ntimes=3)
+    freqs = np.array([110e6, 150e6, 190e6])
+    p = 2 if gemm == "fp64" else 1
+    I = w.I_sky.astype(np.float64) if p == 2 else w.I_sky
+    full = vis_gpu(w.antpos, freqs, w.eq2tops, w.crd_eq, I, w.bm_cube, precision=p, gemm=gemm)
+    tbf, a1, a2 = vis_gpu_baselines(w.antpos, freqs, w.eq2tops, w.crd_eq, I, w.bm_cube, precision=p, gemm=gemm)
+    ftb, b1, b2 = vis_gpu_baselines(w.antpos, freqs, w.eq2tops, w.crd_eq, I, w.bm_cube, precision=p, gemm=gemm,
+                                    order="ftb")
+    nbl = 128 * 129 // 2
+    assert tbf.shape == (3, nbl, 3) and ftb.shape == (3, 3, nbl) and a1.shape == (nbl,)
+    assert np.array_equal(a1, b1) and np.array_equal(a2, b2) and np.all(a1 <= a2)
+    np.testing.assert_array_equal(ftb, full[:, :, a1, a2])
+    np.testing.assert_array_equal(tbf, full[:, :, a1, a2].transpose(1, 2, 0))
