@@ -75,7 +75,7 @@ SIGNATURES = {
     "hv_ctx_create": (_i, [_i, _sz, C.POINTER(_vp)]),
     "hv_ctx_destroy": (_i, [_vp]),
     "hv_vis": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _vp]),
-    "hv_vis_ex": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _i, _vp]),
+    "hv_vis_ex": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _i, _i, _vp]),
     "hv_k_rotate": (_i, [_vp, _i, _vp, _vp, _i64, _vp]),
     "hv_k_beam": (_i, [_vp, _i, _vp, _i, _i, _vp, _i64, _vp]),
     "hv_k_phasor": (_i, [_vp, _i, _i, _vp, _i, _i, _vp, _vp, _i64, _vp, _d, _vp]),
@@ -83,6 +83,7 @@ SIGNATURES = {
     "hv_sky_upload": (_i, [_vp, _i, _vp, _vp, _i64, C.POINTER(_vp)]),
     "hv_sky_free": (_i, [_vp]),
     "hv_beam_upload": (_i, [_vp, _i, _vp, _i, _i, C.POINTER(_vp)]),
+    "hv_beam_upload_freq": (_i, [_vp, _i, _vp, _i, _i, _i, C.POINTER(_vp)]),
     "hv_beam_free": (_i, [_vp]),
     "hv_simulate": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _i,
                          C.POINTER(StageTimes)]),

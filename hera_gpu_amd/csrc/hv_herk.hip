@@ -1129,7 +1129,7 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
         const int n_st = s_end - s_begin;
         const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
         const double il = a.inv_lambda[fl];   // turns per metre
-        const float* amp_t = a.amp + (size_t)tb * amp_time;
+        const float* amp_t = a.amp + (size_t)(a.amp_per_unit ? unit : tb) * amp_time;
         const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
 
         unsigned smask = 0;
@@ -1422,7 +1422,7 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
         const int n_st = s_end - s_begin;
         const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
         const double il = a.inv_lambda[fl];   // turns per metre
-        const float* amp_t = a.amp + (size_t)tb * amp_time;
+        const float* amp_t = a.amp + (size_t)(a.amp_per_unit ? unit : tb) * amp_time;
         const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
 
         unsigned smask = 0;

@@ -75,8 +75,9 @@ struct hv_sky {
 
 struct hv_beam {
     hv_ctx* ctx;
-    void* data;       // [N][P][P] float or double
+    void* data;       // [nfreq][N][P][P] float or double
     int nant, beam_px, f64;
+    int nfreq;        // 1: one beam for every frequency
     double max_abs;
 };
 
@@ -99,13 +100,15 @@ struct Plan {
     size_t row_bytes = 0, unit_bytes = 0, part_unit = 0;
     int fb = 1, tb = 1;            // frequencies / times per batch
     bool fused = false;            // operands built inside the contraction (B_W holds A sqrt(I))
-    size_t amp_time = 0;           // bytes of A sqrt(I) per time (fused)
+    size_t amp_time = 0;           // bytes of A sqrt(I) per time or per unit (fused)
+    bool amp_unit = false;         // A sqrt(I) per unit (frequency-dependent beams)
     // device bytes per workspace buffer
     size_t crd_top_bytes = 0, w_bytes = 0, part_bytes = 0, out_bytes = 0;
     size_t footprint() const { return crd_top_bytes + w_bytes + part_bytes + out_bytes; }
 };
 
-Plan plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t operand_budget, int layout = 0);
+Plan plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t operand_budget, int layout = 0,
+              bool beam_per_freq = false);
 
 struct JobDev {
     int mode;
@@ -121,6 +124,7 @@ struct JobDev {
     float amp_scale;
     const int64_t* dyn_kpad;    // optional device-side source count
     int layout = 0;             // output layout (Layout)
+    size_t beam_fstride = 0;    // elements between per-frequency beam planes (0: one beam)
 };
 
 struct StageTimer {

@@ -89,6 +89,9 @@ struct PhasorArgs {
     float amp_scale;         // multiplies A*sqrt(I) (power of two; FP16 range)
     int in_f64;              // beam element type is double
     const int64_t* dyn_kpad;
+    size_t beam_fstride;     // elements between the beam planes of consecutive
+                             // frequencies (0: one beam for all frequencies);
+                             // beam points at the launch's first frequency
 };
 
 struct HerkArgs {
@@ -157,12 +160,15 @@ struct SplitArgs {           // complex phasors -> operands (hv_k_herk)
 struct AmpArgs {
     const double* crd_top;   // [3][Kpad] of the first time; time tb at + tb*3*Kpad
     const double* isqrt;     // [Kpad]
-    const void* beam;        // [N][P][P] float or double (in_f64)
-    float* amp;              // [ntime][Kpad/16][N][16]
+    const void* beam;        // [N][P][P] float or double (in_f64), first frequency's plane
+    float* amp;              // [ntime][Kpad/16][N][16], or [ntime*nfreq][...] per unit
     int64_t kpad;
     int nant, beam_px, ntime, in_f64;
     float amp_scale;
     const int64_t* dyn_kpad;
+    size_t beam_fstride;     // 0: one beam for all frequencies (amp per time);
+                             // else elements between frequency planes (amp per unit)
+    int nfreq;               // frequencies per time of the launch (per-unit amp)
 };
 
 // Contraction of split-mode operands generated in place (see hv_herk.hip).
@@ -176,6 +182,7 @@ struct FusedArgs {
     int64_t kpad;
     int nant, nb, ntiles, nsplit, nunits, nfreq;
     const int64_t* dyn_kpad;
+    int amp_per_unit;         // amp holds one block per unit (frequency-dependent beams)
 };
 
 hipError_t launch_amp(const AmpArgs& a, hipStream_t s);

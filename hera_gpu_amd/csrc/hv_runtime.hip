@@ -223,7 +223,8 @@ static size_t operand_budget(hv_ctx* c) {
     return std::min((size_t)24 << 30, c->budget / 3);
 }
 
-Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t wbudget, int layout) {
+Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t wbudget, int layout,
+                  bool beam_per_freq) {
     Plan p;
     p.mode = mode;
     p.nant = nant;
@@ -241,19 +242,22 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.fused = herk_fused(mode, nant);
     // batch: all frequencies of a time if they fit, then as many times as fit.
     // Operands per unit bound a batch, or, when the contraction builds them,
-    // the partial tiles per unit and A sqrt(I) per time
+    // the partial tiles per unit and A sqrt(I) per time (per unit with a beam
+    // per frequency: then the frequencies of a time do not share it)
+    p.amp_unit = p.fused && beam_per_freq;
     p.amp_time = p.fused ? (size_t)nant * kpad * sizeof(float) : 0;
-    const size_t per_unit = p.fused ? p.part_unit : p.unit_bytes;
+    const size_t per_unit = p.fused ? p.part_unit + (p.amp_unit ? p.amp_time : 0) : p.unit_bytes;
+    const size_t per_time = p.amp_unit ? 0 : p.amp_time;
     p.fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nfreq, wbudget / std::max<size_t>(per_unit, 1)));
     p.tb = 1;
     if (p.fb == nfreq)
         p.tb = (int)std::max<size_t>(
-            1, std::min<size_t>((size_t)ntimes, wbudget / std::max<size_t>(per_unit * nfreq + p.amp_time, 1)));
+            1, std::min<size_t>((size_t)ntimes, wbudget / std::max<size_t>(per_unit * nfreq + per_time, 1)));
     // keep one contraction launch within ~64k work items
     while (p.tb > 1 && (int64_t)p.tb * p.fb * p.nsplit * p.ntiles > 65536) --p.tb;
     const int units = p.fb * p.tb;
     p.crd_top_bytes = (size_t)ntimes * 3 * kpad * sizeof(double);
-    p.w_bytes = p.fused ? p.amp_time * p.tb : p.unit_bytes * units;
+    p.w_bytes = p.fused ? p.amp_time * (p.amp_unit ? units : p.tb) : p.unit_bytes * units;
     p.part_bytes = p.part_unit * units;
     p.out_bytes = out_elems(layout, nfreq, ntimes, nant) * (mode_is_f64(mode) ? 16 : 8);
     return p;
@@ -279,7 +283,8 @@ static int enqueue_phasor(const JobDev& j, const Plan& p, Workspace& ws, void* W
     PhasorArgs pa{};
     pa.crd_top = static_cast<double*>(ws.buf[B_CRD_TOP].p) + (size_t)t0 * 3 * p.kpad;
     pa.isqrt = j.isqrt;
-    pa.beam = j.beam;
+    pa.beam = static_cast<const char*>(j.beam) + (size_t)f0 * j.beam_fstride * (j.beam_f64 ? 8 : 4);
+    pa.beam_fstride = j.beam_fstride;
     pa.antpos = j.antpos;
     pa.inv_lambda = j.inv_lambda + f0;
     pa.W = W;
@@ -347,12 +352,14 @@ static int enqueue_reduce(const JobDev& j, const Plan& p, void* part, int t0, in
 
 // Fused plans: A sqrt(I) of the batch's times (stage 1), then the
 // contraction that builds its operands from it (stage 2).
-static int enqueue_amp(const JobDev& j, const Plan& p, Workspace& ws, int t0, int nt, hipStream_t stream,
-                       StageTimer& tm) {
+static int enqueue_amp(const JobDev& j, const Plan& p, Workspace& ws, int t0, int nt, int f0, int nf,
+                       hipStream_t stream, StageTimer& tm) {
     AmpArgs aa{};
     aa.crd_top = static_cast<double*>(ws.buf[B_CRD_TOP].p) + (size_t)t0 * 3 * p.kpad;
     aa.isqrt = j.isqrt;
-    aa.beam = j.beam;
+    aa.beam = static_cast<const char*>(j.beam) + (size_t)f0 * j.beam_fstride * (j.beam_f64 ? 8 : 4);
+    aa.beam_fstride = j.beam_fstride;
+    aa.nfreq = nf;
     aa.amp = static_cast<float*>(ws.buf[B_W].p);
     aa.kpad = p.kpad;
     aa.nant = j.nant;
@@ -383,6 +390,7 @@ static int enqueue_fused(const JobDev& j, const Plan& p, Workspace& ws, int t0, 
     fa.nunits = nt * nf;
     fa.nfreq = nf;
     fa.dyn_kpad = j.dyn_kpad;
+    fa.amp_per_unit = p.amp_unit ? 1 : 0;
     HV_CHECK(tm.begin(2, stream));
     HV_HIP(launch_fused(j.mode, p.tile, fa, stream));
     HV_CHECK(tm.end(2, stream));
@@ -397,8 +405,8 @@ int hv::enqueue_job(const JobDev& j, const Plan& p, Workspace& ws, hipStream_t s
             const int nt = std::min(p.tb, j.ntimes - t0);
             for (int f0 = 0; f0 < j.nfreq; f0 += p.fb) {
                 const int nf = std::min(p.fb, j.nfreq - f0);
-                if (amp_t0 != t0) {
-                    HV_CHECK(enqueue_amp(j, p, ws, t0, nt, stream, tm));
+                if (amp_t0 != t0 || p.amp_unit) {          // per time, or per batch of units
+                    HV_CHECK(enqueue_amp(j, p, ws, t0, nt, f0, nf, stream, tm));
                     amp_t0 = t0;
                 }
                 HV_CHECK(enqueue_fused(j, p, ws, t0, nt, f0, nf, stream, tm));
@@ -455,14 +463,14 @@ static bool overlap_batches() {
 }
 
 static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
-    Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c), j.layout);
+    Plan p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c), j.layout, j.beam_fstride != 0);
     const int64_t batches = (int64_t)((j.ntimes + p.tb - 1) / p.tb) * ((j.nfreq + p.fb - 1) / p.fb);
     if (batches < 2 || !overlap_batches() || p.fused) {
         HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
         return enqueue_job(j, p, c->ws, c->stream, tm);
     }
     // several batches: double-buffer operands and partials at half the budget each
-    p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c) / 2, j.layout);
+    p = plan_job(j.mode, j.nant, j.kpad, j.ntimes, j.nfreq, operand_budget(c) / 2, j.layout, j.beam_fstride != 0);
     HV_CHECK(ensure(c, c->ws, B_CRD_TOP, p.crd_top_bytes, c->stream));
     HV_CHECK(ensure(c, c->ws, B_W, 2 * p.w_bytes, c->stream));
     HV_CHECK(ensure(c, c->ws, B_PART, 2 * p.part_bytes, c->stream));
@@ -524,14 +532,17 @@ extern "C" int hv_vis(hv_ctx* c, int mode, int dtype, const void* antpos, int na
                       int nfreq, const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt,
                       int64_t nsrc, const void* bm_cube, int beam_px, void* vis_out) {
     return hv_vis_ex(c, mode, dtype, antpos, nant, freqs, nfreq, eq2tops, ntimes, crd_eq, isqrt, nsrc, bm_cube,
-                     beam_px, HV_LAYOUT_FULL, vis_out);
+                     beam_px, 1, HV_LAYOUT_FULL, vis_out);
 }
 
 extern "C" int hv_vis_ex(hv_ctx* c, int mode, int dtype, const void* antpos, int nant, const double* freqs,
                          int nfreq, const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt,
-                         int64_t nsrc, const void* bm_cube, int beam_px, int layout, void* vis_out) {
+                         int64_t nsrc, const void* bm_cube, int beam_px, int beam_nfreq, int layout,
+                         void* vis_out) {
     if (!c) return fail(HV_ERR_ARG, "ctx is NULL");
     if (layout < HV_LAYOUT_FULL || layout > HV_LAYOUT_TBF) return fail(HV_ERR_ARG, "unknown output layout");
+    if (beam_nfreq != 1 && beam_nfreq != nfreq)
+        return fail(HV_ERR_ARG, "beam_nfreq must be 1 (one beam) or nfreq (a beam per frequency)");
     HV_CHECK(check_job_args(mode, dtype, nant, nfreq, ntimes, nsrc, beam_px));
     if (!antpos || !freqs || !eq2tops || !bm_cube || !vis_out || (nsrc > 0 && (!crd_eq || !isqrt)))
         return fail(HV_ERR_ARG, "NULL array argument");
@@ -544,7 +555,8 @@ extern "C" int hv_vis_ex(hv_ctx* c, int mode, int dtype, const void* antpos, int
     to_f64(eq2tops, dtype, (size_t)ntimes * 9, eq);
     inv_lambdas(freqs, nfreq, il);
     pad_sky(crd_eq, isqrt, dtype, nsrc, kpad, crd, isq);
-    const size_t beam_bytes = (size_t)nant * beam_px * beam_px * (dtype == HV_F64 ? 8 : 4);
+    const size_t beam_plane = (size_t)nant * beam_px * beam_px;
+    const size_t beam_bytes = beam_plane * beam_nfreq * (dtype == HV_F64 ? 8 : 4);
 
     HV_CHECK(upload(c, B_ANTPOS, ap.data(), ap.size() * 8));
     HV_CHECK(upload(c, B_EQ2TOPS, eq.data(), eq.size() * 8));
@@ -558,6 +570,7 @@ extern "C" int hv_vis_ex(hv_ctx* c, int mode, int dtype, const void* antpos, int
 
     JobDev j{};
     j.layout = layout;
+    j.beam_fstride = beam_nfreq > 1 ? beam_plane : 0;
     j.mode = mode;
     j.nant = nant; j.ntimes = ntimes; j.nfreq = nfreq; j.beam_px = beam_px; j.beam_f64 = dtype == HV_F64;
     j.kpad = kpad;
@@ -621,17 +634,23 @@ extern "C" int hv_sky_free(hv_sky* s) {
 }
 
 extern "C" int hv_beam_upload(hv_ctx* c, int dtype, const void* bm_cube, int nant, int beam_px, hv_beam** out) {
+    return hv_beam_upload_freq(c, dtype, bm_cube, 1, nant, beam_px, out);
+}
+
+extern "C" int hv_beam_upload_freq(hv_ctx* c, int dtype, const void* bm_cube, int nfreq, int nant, int beam_px,
+                                   hv_beam** out) {
     if (!c || !out || !bm_cube) return fail(HV_ERR_ARG, "NULL argument");
     *out = nullptr;
     HV_CHECK(check_dtype(dtype));
-    if (nant <= 0 || beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "bad beam shape");
+    if (nant <= 0 || nfreq <= 0 || beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "bad beam shape");
     std::lock_guard<std::mutex> lk(c->mu);
     HV_CHECK(set_device(c));
-    const size_t n = (size_t)nant * beam_px * beam_px;
+    const size_t n = (size_t)nfreq * nant * beam_px * beam_px;
     const size_t bytes = n * (dtype == HV_F64 ? 8 : 4);
     hv_beam* b = new hv_beam();
     b->ctx = c;
     b->nant = nant;
+    b->nfreq = nfreq;
     b->beam_px = beam_px;
     b->f64 = dtype == HV_F64;
     b->max_abs = max_abs(bm_cube, dtype, n);
@@ -650,7 +669,7 @@ extern "C" int hv_beam_free(hv_beam* b) {
     std::lock_guard<std::mutex> lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    dev_free(b->ctx, b->data, (size_t)b->nant * b->beam_px * b->beam_px * (b->f64 ? 8 : 4));
+    dev_free(b->ctx, b->data, (size_t)b->nfreq * b->nant * b->beam_px * b->beam_px * (b->f64 ? 8 : 4));
     delete b;
     return HV_OK;
 }
@@ -662,6 +681,8 @@ extern "C" int hv_simulate(hv_ctx* c, int mode, const hv_sky* sky, int64_t nsrc_
     if (sky->ctx != c || beam->ctx != c) return fail(HV_ERR_ARG, "sky/beam belong to another context");
     HV_CHECK(check_job_args(mode, dtype, nant, nfreq, ntimes, nsrc_used, beam->beam_px));
     if (beam->nant != nant) return fail(HV_ERR_ARG, "beam antenna count does not match antpos");
+    if (beam->nfreq != 1 && beam->nfreq != nfreq)
+        return fail(HV_ERR_ARG, "a per-frequency beam needs one plane per frequency of the call");
     if (nsrc_used > sky->nsrc) return fail(HV_ERR_ARG, "nsrc_used exceeds the sky catalog");
     if (mode == HV_MODE_FP64 && !beam->f64) return fail(HV_ERR_ARG, "FP64 mode needs a float64 beam");
     if (reps < 1) reps = 1;
@@ -707,6 +728,7 @@ extern "C" int hv_simulate(hv_ctx* c, int mode, const hv_sky* sky, int64_t nsrc_
     j.crd_eq = crd_eq;
     j.isqrt = isq;
     j.beam = beam->data;
+    j.beam_fstride = beam->nfreq > 1 ? (size_t)nant * beam->beam_px * beam->beam_px : 0;
     j.vis = c->ws.buf[B_OUT].p;
     j.amp_scale = (mode == HV_MODE_SPLIT_FP16) ? fp16_scale(beam->max_abs * sky->max_isqrt) : 1.0f;
 

@@ -46,9 +46,11 @@ struct BucketKey {
     float scale;
     int64_t kbucket;
     int layout;
+    bool beam_per_freq;
     bool operator<(const BucketKey& o) const {
-        return std::tie(mode, nant, beam_px, ntimes, nfreq, beam_f64, scale, kbucket, layout) <
-               std::tie(o.mode, o.nant, o.beam_px, o.ntimes, o.nfreq, o.beam_f64, o.scale, o.kbucket, o.layout);
+        return std::tie(mode, nant, beam_px, ntimes, nfreq, beam_f64, scale, kbucket, layout, beam_per_freq) <
+               std::tie(o.mode, o.nant, o.beam_px, o.ntimes, o.nfreq, o.beam_f64, o.scale, o.kbucket, o.layout,
+                        o.beam_per_freq);
     }
 };
 
@@ -91,7 +93,7 @@ struct Slot {
 
 size_t slot_bytes_needed(const SrvJob& j) {
     const Plan& p = j.plan;
-    const size_t beam_bytes = (size_t)j.nant * j.key.beam_px * j.key.beam_px * (j.key.beam_f64 ? 8 : 4);
+    const size_t beam_bytes = (size_t)j.beam->nfreq * j.nant * j.key.beam_px * j.key.beam_px * (j.key.beam_f64 ? 8 : 4);
     return p.footprint() + (size_t)4 * p.kpad * 8 + beam_bytes + (size_t)j.nant * 24 + (size_t)j.ntimes * 72 +
            (size_t)j.nfreq * 8 + 256 * 6;
 }
@@ -139,7 +141,7 @@ static int launch_job(hv_server* s, Slot& sl, SrvJob& j) {
     bool moved = false;
     HV_CHECK(prepare_workspace(c, sl.ws, p, sl.stream, &moved));
     const size_t es = j.key.beam_f64 ? 8 : 4;
-    const size_t beam_bytes = (size_t)j.nant * j.key.beam_px * j.key.beam_px * es;
+    const size_t beam_bytes = (size_t)j.beam->nfreq * j.nant * j.key.beam_px * j.key.beam_px * es;
     HV_CHECK(ensure(c, sl.ws, B_CRD_EQ, (size_t)3 * p.kpad * 8, sl.stream, &moved));
     HV_CHECK(ensure(c, sl.ws, B_ISQRT, (size_t)p.kpad * 8, sl.stream, &moved));
     HV_CHECK(ensure(c, sl.ws, B_BEAM, beam_bytes, sl.stream, &moved));
@@ -198,6 +200,7 @@ static int launch_job(hv_server* s, Slot& sl, SrvJob& j) {
     jd.amp_scale = j.key.scale;
     jd.dyn_kpad = (const int64_t*)w.buf[B_DYN].p;
     jd.layout = j.layout;
+    jd.beam_fstride = j.beam->nfreq > 1 ? (size_t)j.nant * j.key.beam_px * j.key.beam_px : 0;
 
     // A bucket is captured the second time it is seen: a shape that comes
     // once runs direct (capture and instantiation of a large job's graph
@@ -439,6 +442,8 @@ extern "C" int hv_server_submit(hv_server* s, const hv_job* job, int64_t* id) {
     if (job->nant <= 0 || job->nfreq <= 0 || job->ntimes <= 0) return fail(HV_ERR_ARG, "bad job shape");
     if (job->nsrc < 0 || job->nsrc > sky->nsrc) return fail(HV_ERR_ARG, "nsrc outside the sky catalog");
     if (beam->nant != job->nant) return fail(HV_ERR_ARG, "beam antenna count does not match antpos");
+    if (beam->nfreq != 1 && beam->nfreq != job->nfreq)
+        return fail(HV_ERR_ARG, "a per-frequency beam needs one plane per frequency of the job");
     if (job->mode == HV_MODE_FP64 && !beam->f64) return fail(HV_ERR_ARG, "FP64 mode needs a float64 beam");
     auto j = std::make_unique<SrvJob>();
     j->mode = job->mode;
@@ -459,8 +464,9 @@ extern "C" int hv_server_submit(hv_server* s, const hv_job* job, int64_t* id) {
     while (kb < j->kpad) kb <<= 1;
     const float scale = (j->mode == HV_MODE_SPLIT_FP16) ? fp16_scale(beam->max_abs * sky->max_isqrt) : 1.0f;
     j->layout = (job->flags & HV_JOB_TBF) ? LAYOUT_TBF : (job->flags & HV_JOB_FTB) ? LAYOUT_FTB : LAYOUT_FULL;
-    j->key = BucketKey{j->mode, j->nant, beam->beam_px, j->ntimes, j->nfreq, beam->f64, scale, kb, j->layout};
-    j->plan = plan_job(j->mode, j->nant, kb, j->ntimes, j->nfreq, SERVE_OPERAND_BUDGET, j->layout);
+    j->key = BucketKey{j->mode, j->nant, beam->beam_px, j->ntimes, j->nfreq, beam->f64, scale, kb, j->layout,
+                       beam->nfreq > 1};
+    j->plan = plan_job(j->mode, j->nant, kb, j->ntimes, j->nfreq, SERVE_OPERAND_BUDGET, j->layout, beam->nfreq > 1);
     j->out_bytes = j->plan.out_bytes;
     j->footprint = slot_bytes_needed(*j);
     if (j->footprint > s->budget)

@@ -69,6 +69,37 @@ constexpr int PH_THREADS = 256;
 constexpr int PH_SRC_PER_THREAD = 8;
 constexpr int PH_SRC_PER_WG = 8192;
 
+// Phasors of 8 consecutive sources of one antenna at one frequency (path
+// differences d in metres, amplitudes amp) written to their operand slots.
+template <int MODE, typename R>
+__device__ __forceinline__ void phasor_store(char* dst, const R* amp, const double* d, double il, double il4) {
+    if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
+        // two 32-byte groups [4 hi words | 4 lo words]
+        uint32_t hw[8], lw[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float sn, cs;
+            sincos_turns_hw(d[j] * il, &sn, &cs);
+            split_phasor<MODE>(amp[j], cs, sn, hw[j], lw[j]);
+        }
+        uint4* o = reinterpret_cast<uint4*>(dst);
+        o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        o[1] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+        o[2] = make_uint4(hw[4], hw[5], hw[6], hw[7]);
+        o[3] = make_uint4(lw[4], lw[5], lw[6], lw[7]);
+    } else {
+        R re[8], im[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            R sn, cs;
+            sincos_quarter_turns(d[j] * il4, &sn, &cs);
+            re[j] = amp[j] * cs;
+            im[j] = amp[j] * sn;
+        }
+        store_phasors8<MODE, R>(dst, re, im);
+    }
+}
+
 template <int MODE, typename BT>
 __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     using R = typename std::conditional<MODE == MODE_FP64, double, float>::type;
@@ -96,6 +127,39 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     constexpr int SST = stage_src(MODE == MODE_FP64);
     constexpr size_t elem = (MODE == MODE_FP64) ? 16 : 8;
 
+    if (a.beam_fstride) {
+        // a beam per frequency: frequency-outer, each frequency's plane in
+        // LDS in turn, the same arithmetic per element as below
+        for (int u = 0; u < a.nfreq; ++u) {
+            if (u > 0) {
+                __syncthreads();                       // every thread is done with plane u-1
+                const BT* gu = reinterpret_cast<const BT*>(a.beam) + u * a.beam_fstride + (size_t)ant * P * P;
+                for (int i = threadIdx.x; i < P * P; i += PH_THREADS) sb[i] = gu[i];
+                __syncthreads();
+            }
+            const double il = a.inv_lambda[u];
+            const double il4 = 4.0 * il;
+            for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
+                 s0 += PH_THREADS * PH_SRC_PER_THREAD) {
+                R amp[8];
+                double d[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int64_t s = s0 + j;
+                    const double x = cx[s], y = cy[s], z = cz[s];
+                    R A = (R)0;
+                    if (z > 0.0) A = (R)beam_bilinear<BT>(sb, P, x, y);
+                    amp[j] = A * (R)isq[s] * (R)a.amp_scale;
+                    d[j] = fma(ax, x, fma(ay, y, az * z));
+                }
+                phasor_store<MODE, R>(wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
+                                          (size_t)(s0 % SST) * elem,
+                                      amp, d, il, il4);
+            }
+        }
+        return;
+    }
+
     for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
          s0 += PH_THREADS * PH_SRC_PER_THREAD) {
         R amp[8];
@@ -112,33 +176,9 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
         for (int u = 0; u < a.nfreq; ++u) {
             const double il = a.inv_lambda[u];          // turns per metre
             const double il4 = 4.0 * il;                // quarter turns per metre (exact scaling)
-            char* dst = wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
-                        (size_t)(s0 % SST) * elem;
-            if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
-                // two 32-byte groups [4 hi words | 4 lo words]
-                uint32_t hw[8], lw[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    float sn, cs;
-                    sincos_turns_hw(d[j] * il, &sn, &cs);
-                    split_phasor<MODE>(amp[j], cs, sn, hw[j], lw[j]);
-                }
-                uint4* o = reinterpret_cast<uint4*>(dst);
-                o[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-                o[1] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-                o[2] = make_uint4(hw[4], hw[5], hw[6], hw[7]);
-                o[3] = make_uint4(lw[4], lw[5], lw[6], lw[7]);
-            } else {
-                R re[8], im[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    R sn, cs;
-                    sincos_quarter_turns(d[j] * il4, &sn, &cs);
-                    re[j] = amp[j] * cs;
-                    im[j] = amp[j] * sn;
-                }
-                store_phasors8<MODE, R>(dst, re, im);
-            }
+            phasor_store<MODE, R>(wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
+                                      (size_t)(s0 % SST) * elem,
+                                  amp, d, il, il4);
         }
     }
 }
@@ -189,16 +229,19 @@ __global__ __launch_bounds__(PH_THREADS) void amp_kernel(AmpArgs a) {
     const int64_t wg0 = (int64_t)blockIdx.y * PH_SRC_PER_WG;
     if (wg0 >= kact) return;                       // uniform per workgroup
     const int ant = blockIdx.x;
-    const int tb = blockIdx.z;
+    // z: the time of the launch, or (frequency-dependent beams) its unit
+    const int z = blockIdx.z;
+    const int tb = a.beam_fstride ? z / a.nfreq : z;
+    const int fu = a.beam_fstride ? z - tb * a.nfreq : 0;
     const int P = a.beam_px;
-    const BT* gb = reinterpret_cast<const BT*>(a.beam) + (size_t)ant * P * P;
+    const BT* gb = reinterpret_cast<const BT*>(a.beam) + fu * a.beam_fstride + (size_t)ant * P * P;
     for (int i = threadIdx.x; i < P * P; i += PH_THREADS) sb[i] = gb[i];
     __syncthreads();
     const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
     const double* cy = cx + a.kpad;
     const double* cz = cx + 2 * a.kpad;
     const int64_t wg1 = min(wg0 + (int64_t)PH_SRC_PER_WG, kact);
-    float* out = a.amp + (size_t)tb * a.kpad * a.nant;
+    float* out = a.amp + (size_t)z * a.kpad * a.nant;
     for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
          s0 += PH_THREADS * PH_SRC_PER_THREAD) {
         float amp[8];
@@ -218,8 +261,9 @@ __global__ __launch_bounds__(PH_THREADS) void amp_kernel(AmpArgs a) {
 
 hipError_t launch_amp(const AmpArgs& a, hipStream_t s) {
     const int64_t nchunk = (a.kpad + PH_SRC_PER_WG - 1) / PH_SRC_PER_WG;
-    if (nchunk > 65535 || a.ntime > 65535 || a.kpad % 16) return hipErrorInvalidConfiguration;
-    dim3 grid((unsigned)a.nant, (unsigned)nchunk, (unsigned)a.ntime);
+    const int64_t nz = a.beam_fstride ? (int64_t)a.ntime * a.nfreq : a.ntime;
+    if (nchunk > 65535 || nz > 65535 || a.kpad % 16) return hipErrorInvalidConfiguration;
+    dim3 grid((unsigned)a.nant, (unsigned)nchunk, (unsigned)nz);
     const size_t lds = (size_t)a.beam_px * a.beam_px * (a.in_f64 ? 8 : 4);
     const void* k = a.in_f64 ? (const void*)amp_kernel<double> : (const void*)amp_kernel<float>;
     if (lds > 64 * 1024) {

@@ -123,17 +123,20 @@ class Sky:
 
 
 class Beam:
-    """Beam cube (N, P, P) resident in HBM."""
+    """Beam cube (N, P, P) resident in HBM, or (F, N, P, P): a beam per
+    frequency, for calls and jobs with exactly F frequencies."""
 
     def __init__(self, bm_cube, device=0, dtype=np.float32):
         self.ctx = _lib.context(device)
         bm = np.ascontiguousarray(bm_cube, dtype=dtype)
-        self.nant, self.beam_px, p2 = bm.shape
+        assert bm.ndim in (3, 4)
+        self.nfreq = bm.shape[0] if bm.ndim == 4 else 1
+        self.nant, self.beam_px, p2 = bm.shape[-3:]
         assert self.beam_px == p2
         self.dtype = np.dtype(dtype)
         h = C.c_void_p()
-        _lib.check(_lib.load().hv_beam_upload(self.ctx.handle, _dtype_code(dtype), _lib.ptr(bm), self.nant,
-                                              self.beam_px, C.byref(h)), "hv_beam_upload")
+        _lib.check(_lib.load().hv_beam_upload_freq(self.ctx.handle, _dtype_code(dtype), _lib.ptr(bm), self.nfreq,
+                                                   self.nant, self.beam_px, C.byref(h)), "hv_beam_upload")
         self.handle = h
 
     def close(self):

@@ -13,7 +13,8 @@ Differences a caller may notice:
 * every source is summed (the reference drops ``npix mod chunk`` tail
   sources, vis.py:176-177);
 * ``freq`` may also be a 1-D array of frequencies; the result is then
-  (nfreq, ntimes, nant, nant);
+  (nfreq, ntimes, nant, nant), and ``bm_cube`` may then be (nfreq, nant, P, P),
+  a beam per frequency;
 * the delay phase is reduced in FP64 (the reference evaluates it in FP32 for
   precision=1, vis.py:106-108), so realistic phases of ~10^3 rad keep their
   accuracy;
@@ -115,15 +116,20 @@ def _vis(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube, precision, real_dtype, c
     npix = crd_eq.shape[1]
     assert crd_eq.shape == (3, npix)
     assert I_sky.shape == (npix,)
-    assert bm_cube.ndim == 3
-    beam_px = bm_cube.shape[1]
-    assert bm_cube.shape == (nant, beam_px, beam_px)
     assert eq2tops.ndim == 3
     ntimes = eq2tops.shape[0]
     assert eq2tops.shape == (ntimes, 3, 3)
     scalar_freq = np.ndim(freq) == 0
     freqs = np.ascontiguousarray(np.atleast_1d(np.asarray(freq, dtype=np.float64)))
     assert freqs.ndim == 1 and freqs.size >= 1
+    # (N, P, P) as the reference, or (F, N, P, P): a beam per frequency
+    assert bm_cube.ndim in (3, 4)
+    beam_px = bm_cube.shape[-1]
+    if bm_cube.ndim == 3:
+        assert bm_cube.shape == (nant, beam_px, beam_px)
+    else:
+        assert bm_cube.shape == (freqs.size, nant, beam_px, beam_px)
+    beam_nfreq = 1 if bm_cube.ndim == 3 else freqs.size
     # casts of vis.py:171-175
     antpos = np.ascontiguousarray(antpos, dtype=real_dtype)
     eq2tops = np.ascontiguousarray(eq2tops, dtype=real_dtype)
@@ -138,6 +144,6 @@ def _vis(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube, precision, real_dtype, c
     ctx = _lib.context(device)
     rc = _lib.load().hv_vis_ex(ctx.handle, mode, dtype, _lib.ptr(antpos), nant, _lib.ptr(freqs), freqs.size,
                                _lib.ptr(eq2tops), ntimes, _lib.ptr(crd_eq), _lib.ptr(isqrt), npix,
-                               _lib.ptr(bm_cube), beam_px, LAYOUTS[layout], _lib.ptr(out))
+                               _lib.ptr(bm_cube), beam_px, beam_nfreq, LAYOUTS[layout], _lib.ptr(out))
     _lib.check(rc, "hv_vis")
     return out, scalar_freq

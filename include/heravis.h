@@ -92,13 +92,16 @@ int hv_vis(hv_ctx* ctx, int mode, int dtype,
 #define HV_LAYOUT_FTB 1    /* (F, T, Nbl)                  */
 #define HV_LAYOUT_TBF 2    /* (T, Nbl, F): UVData blt order */
 
-/* hv_vis with an output layout (hv_vis = HV_LAYOUT_FULL). */
+/* hv_vis with per-frequency beams and an output layout (hv_vis =
+ * beam_nfreq 1, HV_LAYOUT_FULL).  beam_nfreq = nfreq: bm_cube is
+ * (F, N, P, P), one beam per frequency; no counterpart in the reference,
+ * whose bm_cube is (N, P, P) for every frequency (vis.py:167,175). */
 int hv_vis_ex(hv_ctx* ctx, int mode, int dtype,
               const void* antpos, int nant,
               const double* freqs, int nfreq,
               const void* eq2tops, int ntimes,
               const void* crd_eq, const void* isqrt, int64_t nsrc,
-              const void* bm_cube, int beam_px,
+              const void* bm_cube, int beam_px, int beam_nfreq,
               int layout, void* vis_out);
 
 /* ---------------------------------------------------------------------------
@@ -149,6 +152,10 @@ int hv_sky_upload(hv_ctx* ctx, int dtype, const void* crd_eq, const void* isqrt,
 int hv_sky_free(hv_sky* sky);
 int hv_beam_upload(hv_ctx* ctx, int dtype, const void* bm_cube, int nant,
                    int beam_px, hv_beam** out);
+/* A beam per frequency, (nfreq, nant, P, P): calls and jobs using it must
+ * have exactly nfreq frequencies (no reference counterpart, as hv_vis_ex). */
+int hv_beam_upload_freq(hv_ctx* ctx, int dtype, const void* bm_cube, int nfreq,
+                        int nant, int beam_px, hv_beam** out);
 int hv_beam_free(hv_beam* beam);
 
 /* Per-stage device times of the last hv_simulate call with timing on, in

@@ -88,6 +88,10 @@ def vis_oracle(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube) -> np.ndarray:
 
     ``freqs`` may be a scalar (then F = 1).  Inputs are used at the values
     they hold (the caller casts to float32 first to mirror precision=1).
+    ``bm_cube`` is (N, P, P), the reference's one beam for every frequency
+    (vis.py:167), or (F, N, P, P), a beam per frequency (the extension the
+    reference leaves open at vis.py:175): frequency f then interpolates
+    plane f with the same bilinear rule.
     """
     antpos = np.asarray(antpos, dtype=np.float64)
     eq2tops = np.asarray(eq2tops, dtype=np.float64)
@@ -99,13 +103,17 @@ def vis_oracle(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube) -> np.ndarray:
     T = eq2tops.shape[0]
     F = freqs.shape[0]
     out = np.empty((F, T, N, N), dtype=np.complex128)
+    per_freq = bm_cube.ndim == 4
+    if per_freq:
+        assert bm_cube.shape[:2] == (F, N), "a per-frequency beam needs one plane per frequency"
     for t in range(T):
         crd_top = eq2tops[t] @ crd_eq
-        amp = beam_amplitude(bm_cube, crd_top, I_sky)
+        amp = None if per_freq else beam_amplitude(bm_cube, crd_top, I_sky)
         dist = antpos @ crd_top  # metres of path difference, (N, K)
         for f, nu in enumerate(freqs):
+            a_f = beam_amplitude(bm_cube[f], crd_top, I_sky) if per_freq else amp
             phase = (2.0 * np.pi * nu / C_LIGHT) * dist
-            v = amp * np.exp(1j * phase)
+            v = a_f * np.exp(1j * phase)
             out[f, t] = v @ v.conj().T
     return out
 
@@ -114,7 +122,8 @@ def vis_oracle_units(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube, units):
     """FP64 visibilities for a sample of (freq_index, time_index) units.
 
     Returns complex128 (len(units), N, N).  Used to check large
-    configurations on a stated sample of units.
+    configurations on a stated sample of units.  ``bm_cube`` is (N, P, P) or
+    (F, N, P, P) as in :func:`vis_oracle`.
     """
     antpos = np.asarray(antpos, dtype=np.float64)
     eq2tops = np.asarray(eq2tops, dtype=np.float64)
@@ -127,8 +136,11 @@ def vis_oracle_units(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube, units):
     for i, (f, t) in enumerate(units):
         if t not in cache:
             crd_top = eq2tops[t] @ crd_eq
-            cache = {t: (beam_amplitude(bm_cube, crd_top, I_sky), antpos @ crd_top)}
-        amp, dist = cache[t]
+            cache = {t: (crd_top, None if bm_cube.ndim == 4 else beam_amplitude(bm_cube, crd_top, I_sky),
+                         antpos @ crd_top)}
+        crd_top, amp, dist = cache[t]
+        if amp is None:
+            amp = beam_amplitude(bm_cube[f], crd_top, I_sky)
         v = amp * np.exp(1j * (2.0 * np.pi * freqs[f] / C_LIGHT) * dist)
         out[i] = v @ v.conj().T
     return out

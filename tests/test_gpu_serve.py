@@ -126,3 +126,29 @@ def test_baseline_layout_jobs_equal_full_jobs(catalogs):
     a1, a2 = baseline_pairs(128)
     np.testing.assert_array_equal(ftb.vis, full.vis[:, :, a1, a2])
     np.testing.assert_array_equal(tbf.vis, full.vis[:, :, a1, a2].transpose(1, 2, 0))
+
+
+def test_per_frequency_beam_jobs_equal_alone(catalogs):
+    # a resident (F, N, P, P) beam: served jobs (two of one bucket, so the
+    # second replays a graph) equal the direct simulation bit for bit, and a
+    # job whose frequency count differs from the beam's is refused at submit
+    antpos, beam3, sky, eq, freqs = job_inputs(catalogs, 9, "disk128", 2, 5)
+    P = beam3.beam_px
+    g = np.linspace(-1, 1, P)
+    r2 = g[None, :] ** 2 + g[:, None] ** 2
+    bm4 = np.stack([np.broadcast_to(np.exp(-r2 / (2 * (0.45 - 0.04 * f) ** 2)), (128, P, P))
+                    for f in range(5)]).astype(np.float32)
+    beam = stages.Beam(bm4)
+    try:
+        with Server(nslots=2) as srv:
+            ids = [srv.submit(sky, beam, antpos, freqs, e, nsrc=n)
+                   for e, n in ((eq, 15000), (configs.eq2tops_lst(2, lst0_deg=40.0), 16000))]
+            res = [srv.wait(i, 120) for i in ids]
+            with pytest.raises(AssertionError):
+                srv.submit(sky, beam, antpos, freqs[:3], eq, nsrc=1000)
+        for r, (e, n) in zip(res, ((eq, 15000), (configs.eq2tops_lst(2, lst0_deg=40.0), 16000))):
+            assert r.status == 0
+            alone, _ = stages.simulate(sky, beam, antpos, freqs, e, nsrc=n, gemm="split_bf16")
+            np.testing.assert_array_equal(r.vis, alone)
+    finally:
+        beam.close()

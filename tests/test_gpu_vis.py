@@ -254,3 +254,77 @@ def test_baseline_layouts_equal_full_matrix_bitwise(gemm):
     assert np.array_equal(a1, b1) and np.array_equal(a2, b2) and np.all(a1 <= a2)
     np.testing.assert_array_equal(ftb, full[:, :, a1, a2])
     np.testing.assert_array_equal(tbf, full[:, :, a1, a2].transpose(1, 2, 0))
+
+
+def _freq_beams(nant, nfreq, P, seed):
+    """Gaussian beams whose width scales as 1/nu: one (nant, P, P) plane per frequency."""
+    rng = np.random.default_rng(seed)
+    g = np.linspace(-1, 1, P)
+    r2 = g[None, :] ** 2 + g[:, None] ** 2
+    w0 = rng.uniform(0.3, 0.5, nant)
+    return np.stack([np.exp(-r2[None] / (2 * (w0[:, None, None] * (1.0 - 0.2 * f)) ** 2))
+                     for f in range(nfreq)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("gemm", P1_MODES + ["fp64"])
+def test_per_frequency_beams_vs_oracle(gemm):
+    # one beam plane per frequency (F, N, P, P): each frequency interpolates
+    # its own plane; N = 128 and K over several splits
+    w = configs.make_disk128(nsrc=20000, ntimes=2)
+    freqs = np.array([110e6, 150e6, 190e6])
+    bm4 = _freq_beams(128, 3, 31, 5)
+    p = 2 if gemm == "fp64" else 1
+    I = w.I_sky.astype(np.float64) if p == 2 else w.I_sky
+    bm = bm4.astype(np.float64) if p == 2 else bm4
+    v = vis_gpu(w.antpos, freqs, w.eq2tops, w.crd_eq, I, bm, precision=p, gemm=gemm)
+    ref = vis_oracle(w.antpos, freqs, w.eq2tops, w.crd_eq, effective_intensity(I, p), bm4)
+    assert rel_err(v, ref) < VIS_TOL[gemm]
+    # and each frequency's plane is the one used: plane 0 for every frequency is far off
+    wrong = vis_oracle(w.antpos, freqs, w.eq2tops, w.crd_eq, effective_intensity(I, p), bm4[[0, 0, 0]])
+    assert rel_err(v, wrong) > 1e-2
+
+
+@pytest.mark.parametrize("gemm", ["fp32", "split_bf16", "fp64"])
+@pytest.mark.parametrize("pf", [0, 4])
+def test_identical_frequency_planes_equal_one_beam_bitwise(gemm, pf):
+    # F copies of the one beam give the one-beam result bit for bit, through
+    # the built-in contraction (pf 0: operands built in place for split modes
+    # at N > 64) and through the phasor build (pf 4); N = 300 leaves a partial
+    # last tile
+    from hera_gpu_amd import _lib
+    if pf == 4 and gemm == "fp64":
+        pytest.skip("pf 4 is a split / FP32 variant")
+    lib = _lib.load()
+    w = configs.make_hera350(nsrc=2 * 8192 + 500, ntimes=2, nfreq=3)
+    ap, bm = w.antpos[:300], w.bm_cube[:300]
+    p = 2 if gemm == "fp64" else 1
+    if p == 2:
+        bm = bm.astype(np.float64)
+    bm4 = np.stack([bm] * 3)
+    try:
+        _lib.check(lib.hv_set_herk_variant(128 if pf else 0, pf))
+        one = vis_gpu(ap, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, bm, precision=p, gemm=gemm)
+        per = vis_gpu(ap, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, bm4, precision=p, gemm=gemm)
+    finally:
+        _lib.check(lib.hv_set_herk_variant(0, 0))
+    np.testing.assert_array_equal(per, one)
+
+
+def test_per_frequency_beam_resident_and_checks():
+    w = configs.make_hex37()
+    freqs = np.array([120e6, 160e6])
+    bm4 = _freq_beams(37, 2, w.bm_cube.shape[-1], 6)
+    sky = stages.Sky(w.crd_eq, w.I_sky)
+    beam = stages.Beam(bm4)
+    try:
+        a, _ = stages.simulate(sky, beam, w.antpos, freqs, w.eq2tops, gemm="split_bf16", dtype=np.float32)
+        b = vis_gpu(w.antpos, freqs, w.eq2tops, w.crd_eq, w.I_sky, bm4, gemm="split_bf16")
+        np.testing.assert_array_equal(a, b)
+        # a per-frequency beam needs exactly one plane per frequency
+        with pytest.raises(AssertionError):
+            stages.simulate(sky, beam, w.antpos, freqs[:1], w.eq2tops, gemm="split_bf16", dtype=np.float32)
+    finally:
+        beam.close()
+        sky.close()
+    with pytest.raises(AssertionError):
+        vis_gpu(w.antpos, freqs[:1], w.eq2tops, w.crd_eq, w.I_sky, bm4)

@@ -32,10 +32,11 @@ def brute_force(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube):
     K = crd_eq.shape[1]
     P = bm_cube.shape[-1]
     grid = np.linspace(-1, 1, P)
-    splines = [RectBivariateSpline(grid, grid, np.asarray(bm_cube[i], dtype=np.float64), kx=1, ky=1)
-               for i in range(N)]
+    planes = bm_cube if bm_cube.ndim == 4 else [bm_cube] * len(freqs)   # a beam per frequency
     out = np.zeros((len(freqs), len(eq2tops), N, N), dtype=np.complex128)
     for f, nu in enumerate(freqs):
+        splines = [RectBivariateSpline(grid, grid, np.asarray(planes[f][i], dtype=np.float64), kx=1, ky=1)
+                   for i in range(N)]
         for t, R in enumerate(eq2tops):
             v = [[0j] * K for _ in range(N)]
             for s in range(K):
@@ -164,6 +165,27 @@ def test_vis_cpu_fp32_agrees_to_single_precision_small_phase():
     assert vo.rel_err(v32, ref) < 1e-5
 
 
+def test_per_frequency_beam_matches_brute_force():
+    antpos, freqs, eq, crd, I, bm = tiny_case(7, N=3, K=9, T=2, F=2)
+    bm4 = np.stack([bm, np.random.default_rng(8).uniform(0, 1, bm.shape)])
+    ours = vo.vis_oracle(antpos, freqs, eq, crd, I, bm4)
+    ref = brute_force(antpos, freqs, eq, crd, I, bm4)
+    np.testing.assert_allclose(ours, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    # frequency f uses plane f: each frequency equals a one-beam run with that plane
+    for f in range(2):
+        np.testing.assert_array_equal(ours[f], vo.vis_oracle(antpos, freqs[f:f + 1], eq, crd, I, bm4[f])[0])
+    units = [(1, 0), (0, 1), (1, 1)]
+    part = vo.vis_oracle_units(antpos, freqs, eq, crd, I, bm4, units)
+    for i, (f, t) in enumerate(units):
+        np.testing.assert_array_equal(part[i], ours[f, t])
+
+
+def test_identical_frequency_planes_equal_one_beam():
+    antpos, freqs, eq, crd, I, bm = tiny_case(9, N=4, K=40, T=2, F=2)
+    np.testing.assert_array_equal(vo.vis_oracle(antpos, freqs, eq, crd, I, np.stack([bm, bm])),
+                                  vo.vis_oracle(antpos, freqs, eq, crd, I, bm))
+
+
 def test_units_sampler_matches_full():
     case = tiny_case(5, N=4, K=50, T=3, F=2)
     full = vo.vis_oracle(*case)
@@ -226,3 +248,13 @@ def test_pins_catch_planted_mistakes(mutation):
     ref = brute_force(*case)
     assert vo.rel_err(_mutant_vis(*case, mutation="none"), ref) < 1e-12
     assert vo.rel_err(_mutant_vis(*case, mutation=mutation), ref) > 1e-3
+
+
+def test_pins_catch_wrong_frequency_plane():
+    # a per-frequency beam read at plane 0 for every frequency (or planes in
+    # reverse order) must fail the brute-force pin
+    antpos, freqs, eq, crd, I, bm = tiny_case(10, N=3, K=9, T=2, F=2, P=7)
+    bm4 = np.stack([bm, np.random.default_rng(11).uniform(0, 1, bm.shape)])
+    ref = brute_force(antpos, freqs, eq, crd, I, bm4)
+    for wrong in (np.stack([bm4[0], bm4[0]]), bm4[::-1]):
+        assert vo.rel_err(vo.vis_oracle(antpos, freqs, eq, crd, I, wrong), ref) > 1e-3
