@@ -68,12 +68,15 @@ int herk_tile(int mode, int nant) {
     return (split_mode(mode) && nant > 64) ? 128 : 64;
 }
 
-// (x, y) pairs of 16-bit values -> (y, -x): rotate each 32-bit word by 16 and
-// flip the sign bit of the upper half.
+// (x, y) pairs of 16-bit values -> (-y, x): rotate each 32-bit word by 16 and
+// flip the sign bit of the lower half.  Applied to the B operand b = (x, y)
+// of a split-mode contraction, a . rot(b) = a_x (-b_y) + a_y b_x, the
+// imaginary part of a conj(b).  Every split-mode kernel rotates B (never A),
+// so all of them feed the matrix cores the same operands.
 __device__ inline u32x4 rot_neg16(u32x4 v) {
     u32x4 r;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(v[i], v[i], 16) ^ 0x80000000u;
+    for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(v[i], v[i], 16) ^ 0x00008000u;
     return r;
 }
 
@@ -186,16 +189,16 @@ __device__ __forceinline__ void stage_mma(const char* __restrict__ As, const cha
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
             const int g = 2 * jj + lh;
-            u32x4 ahi[NS], alo[NS], phi[NS], plo[NS], bhi[NS], blo[NS];
+            u32x4 ahi[NS], alo[NS], bhi[NS], blo[NS], bhr[NS], blr[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
                 const int ra = wr * WT + i * BLK + lrow, rb = wc * WT + i * BLK + lrow;
                 ahi[i] = *reinterpret_cast<const u32x4*>(As + lds_off<SWZ>(ra, 2 * g));
                 alo[i] = *reinterpret_cast<const u32x4*>(As + lds_off<SWZ>(ra, 2 * g + 1));
-                phi[i] = rot_neg16(ahi[i]);
-                plo[i] = rot_neg16(alo[i]);
                 bhi[i] = *reinterpret_cast<const u32x4*>(Bs + lds_off<SWZ>(rb, 2 * g));
                 blo[i] = *reinterpret_cast<const u32x4*>(Bs + lds_off<SWZ>(rb, 2 * g + 1));
+                bhr[i] = rot_neg16(bhi[i]);
+                blr[i] = rot_neg16(blo[i]);
             }
 #pragma unroll
             for (int i = 0; i < NS; ++i)
@@ -203,11 +206,11 @@ __device__ __forceinline__ void stage_mma(const char* __restrict__ As, const cha
                 for (int j = 0; j < NS; ++j)
                     if (ON(i, j)) {
                         re[i][j] = mfma16<MODE>(ahi[i], bhi[j], re[i][j]);
-                        im[i][j] = mfma16<MODE>(phi[i], bhi[j], im[i][j]);
+                        im[i][j] = mfma16<MODE>(ahi[i], bhr[j], im[i][j]);
                         re[i][j] = mfma16<MODE>(ahi[i], blo[j], re[i][j]);
-                        im[i][j] = mfma16<MODE>(phi[i], blo[j], im[i][j]);
+                        im[i][j] = mfma16<MODE>(ahi[i], blr[j], im[i][j]);
                         re[i][j] = mfma16<MODE>(alo[i], bhi[j], re[i][j]);
-                        im[i][j] = mfma16<MODE>(plo[i], bhi[j], im[i][j]);
+                        im[i][j] = mfma16<MODE>(alo[i], bhr[j], im[i][j]);
                     }
         }
     } else {  // FP64: lane quarter lh holds (x_s, y_s, x_s+1, y_s+1), s = 2 lh
@@ -556,15 +559,15 @@ __device__ __forceinline__ void read_frag(SplitFrag<NS>& f, const char* As, cons
     }
 }
 
-// Re += a.b and Im += a'.b with a' = (y, -x), as hi.hi + hi.lo + lo.hi.
+// Re += a.b and Im += a.b' with b' = (-y, x), as hi.hi + hi.lo + lo.hi.
 template <int MODE, int NS, bool ALLON>
 __device__ __forceinline__ void frag_mma(const SplitFrag<NS>& f, unsigned smask, f32x16 (&re)[NS][NS],
                                          f32x16 (&im)[NS][NS]) {
-    u32x4 phi[NS], plo[NS];
+    u32x4 bhr[NS], blr[NS];
 #pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        phi[i] = rot_neg16(f.ahi[i]);
-        plo[i] = rot_neg16(f.alo[i]);
+    for (int j = 0; j < NS; ++j) {
+        bhr[j] = rot_neg16(f.bhi[j]);
+        blr[j] = rot_neg16(f.blo[j]);
     }
 #pragma unroll
     for (int i = 0; i < NS; ++i)
@@ -572,11 +575,11 @@ __device__ __forceinline__ void frag_mma(const SplitFrag<NS>& f, unsigned smask,
         for (int j = 0; j < NS; ++j)
             if (ALLON || (smask & (1u << (i * NS + j)))) {
                 re[i][j] = mfma16<MODE>(f.ahi[i], f.bhi[j], re[i][j]);
-                im[i][j] = mfma16<MODE>(phi[i], f.bhi[j], im[i][j]);
+                im[i][j] = mfma16<MODE>(f.ahi[i], bhr[j], im[i][j]);
                 re[i][j] = mfma16<MODE>(f.ahi[i], f.blo[j], re[i][j]);
-                im[i][j] = mfma16<MODE>(phi[i], f.blo[j], im[i][j]);
+                im[i][j] = mfma16<MODE>(f.ahi[i], blr[j], im[i][j]);
                 re[i][j] = mfma16<MODE>(f.alo[i], f.bhi[j], re[i][j]);
-                im[i][j] = mfma16<MODE>(plo[i], f.bhi[j], im[i][j]);
+                im[i][j] = mfma16<MODE>(f.alo[i], bhr[j], im[i][j]);
             }
 }
 
@@ -1233,23 +1236,19 @@ __device__ __forceinline__ void read_frag21(Frag21& f, const char* As, const cha
 }
 
 // Same chain per 32x32 block as frag_mma: Re += hi.hi + hi.lo + lo.hi and
-// Im += a'.b with the same operand order.
+// Im += a.b' with the same operand order.
 template <int MODE>
 __device__ __forceinline__ void frag21_mma(const Frag21& f, f32x16 (&re)[2], f32x16 (&im)[2]) {
-    u32x4 phi[2], plo[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        phi[i] = rot_neg16(f.ahi[i]);
-        plo[i] = rot_neg16(f.alo[i]);
-    }
+    // one B fragment per step: rotating B costs half of rotating the two A fragments
+    const u32x4 bhr = rot_neg16(f.bhi), blr = rot_neg16(f.blo);
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         re[i] = mfma16<MODE>(f.ahi[i], f.bhi, re[i]);
-        im[i] = mfma16<MODE>(phi[i], f.bhi, im[i]);
+        im[i] = mfma16<MODE>(f.ahi[i], bhr, im[i]);
         re[i] = mfma16<MODE>(f.ahi[i], f.blo, re[i]);
-        im[i] = mfma16<MODE>(phi[i], f.blo, im[i]);
+        im[i] = mfma16<MODE>(f.ahi[i], blr, im[i]);
         re[i] = mfma16<MODE>(f.alo[i], f.bhi, re[i]);
-        im[i] = mfma16<MODE>(plo[i], f.bhi, im[i]);
+        im[i] = mfma16<MODE>(f.alo[i], bhr, im[i]);
     }
 }
 

@@ -65,15 +65,13 @@ __device__ inline void sincos_quarter_turns(double t4, double* s, double* c) {
 }
 
 // sin and cos of 2 pi t for a phase t in turns (split-mode operands): the
-// remainder r = t - rint(t) in [-1/2, 1/2] is exact in FP64 and rounded once
-// to FP32, then the hardware sine and cosine, which take revolutions
-// (measured max error 1.25e-7 on [-1/2, 1/2] against FP64 sincospi; with the
-// FP32 rounding of r, < 3e-7 in all, far below the 2^-17 of a hi+lo
-// split).  Two instructions replace the polynomials and the quadrant
-// rotation.
+// fraction r = t - floor(t) in [0, 1) is exact in FP64 (one v_fract_f64) and
+// rounded once to FP32 (error <= 2^-25 turns), then the hardware sine and
+// cosine, which take revolutions (measured max error 1.25e-7 against FP64
+// sincospi; < 4e-7 in all, far below the 2^-17 of a hi+lo split).  Three
+// instructions replace the polynomials and the quadrant rotation.
 __device__ inline void sincos_turns_hw(double t, float* s, float* c) {
-#pragma clang fp contract(off)
-    const float r = (float)(t - rint(t));
+    const float r = (float)__builtin_amdgcn_fract(t);
     *s = __builtin_amdgcn_sinf(r);
     *c = __builtin_amdgcn_cosf(r);
 }
