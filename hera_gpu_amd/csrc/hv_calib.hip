@@ -180,7 +180,8 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
     if (!c || !ms_per_launch || nant < 1 || nsrc < 1 || units < 1 || reps < 1 || mode < HV_MODE_FP32 ||
         mode > HV_MODE_FP64)
         return fail(HV_ERR_ARG, "bad argument");
-    if (tile == 0) tile = herk_tile(mode, nant);
+    if (tile == 192) return fail(HV_ERR_ARG, "192-tiles build their operands in place: time them through hv_simulate");
+    if (tile == 0) tile = herk_tile(mode, nant) == 192 ? 128 : herk_tile(mode, nant);   // the HBM-operand kernel
     if (pf == 5) return fail(HV_ERR_ARG, "pf 5 builds its operands in place: time it through hv_simulate");
     if (pf == 0) pf = herk_prefetch(mode, tile) == 5 ? 4 : herk_prefetch(mode, tile);   // same MFMA loop
     std::lock_guard<std::mutex> lk(c->mu);
@@ -237,8 +238,10 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
 }
 
 extern "C" int hv_set_herk_variant(int tile, int pf) {
-    if (!(tile == 0 || tile == 64 || tile == 128) || pf < 0 || pf > 5)
-        return fail(HV_ERR_ARG, "tile must be 0, 64 or 128 and pf 0..5");
+    if (!(tile == 0 || tile == 64 || tile == 128 || tile == 192) || pf < 0 || pf > 5)
+        return fail(HV_ERR_ARG, "tile must be 0, 64, 128 or 192 and pf 0..5");
+    if (tile == 192 && pf != 0 && pf != 5)
+        return fail(HV_ERR_ARG, "192-tiles exist only as in-place operand builds (pf 5)");
     set_herk_variant(tile, pf);
     return HV_OK;
 }

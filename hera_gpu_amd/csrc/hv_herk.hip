@@ -61,11 +61,25 @@ void set_herk_variant(int tile, int pf) {
 
 static bool split_mode(int mode) { return mode == MODE_SPLIT_BF16 || mode == MODE_SPLIT_FP16; }
 
-// Overrides apply where the variant exists for the mode (128-tiles and
-// in-place operand builds are split-mode kernels).
+// pf of the built-in choice or of the override where it exists for the mode
+// (pf 5, operands built in place, is a split-mode variant; FP64 has no pf 2).
+static int pf_choice(int mode) {
+    if (g_pf_override && (g_pf_override != 5 || split_mode(mode)) && !(mode == MODE_FP64 && g_pf_override == 2))
+        return g_pf_override;
+    // split modes build their operands in place (whole simulations, 2^18
+    // sources: faster than the phasor build + LDS ring at every array size
+    // measured, 1.2x at N = 37 and 350, 1.6x at N = 128, and no operand buffer)
+    return split_mode(mode) ? 5 : 1;
+}
+
+// Tile edge: 64 (FP32, FP64, small arrays), 128 (split modes above 64
+// antennas), or 192 (split modes, in-place builds only; selected by an
+// override).  Overrides apply where the variant exists for the mode.
 int herk_tile(int mode, int nant) {
-    if (g_tile_override && (g_tile_override == 64 || split_mode(mode))) return g_tile_override;
-    return (split_mode(mode) && nant > 64) ? 128 : 64;
+    int t = split_mode(mode) ? (nant > 64 ? 128 : 64) : 64;
+    if (g_tile_override && (g_tile_override == 64 || split_mode(mode))) t = g_tile_override;
+    if (t == 192 && pf_choice(mode) != 5) t = 128;
+    return t;
 }
 
 // (x, y) pairs of 16-bit values -> (-y, x): rotate each 32-bit word by 16 and
@@ -938,6 +952,11 @@ static hipError_t launch_herk_ring_t(const HerkArgs& a, bool persistent, hipStre
 #define HV_LDS(p) ((__attribute__((address_space(3))) void*)(p))
 constexpr int FRING = 4;
 
+// Amplitude rows in LDS: the 16-byte part p of row r (4 sources) sits at
+// chunk 4 r + (p ^ amp_swz(r)), so the 8 lanes of a read phase, which read
+// part g of 8 consecutive rows, hit 8 distinct bank groups.
+__device__ __forceinline__ int amp_swz(int r) { return (r >> 1) & 3; }
+
 template <int TILE>
 struct FusedLds {
     static constexpr int ROWS = 2 * TILE;                // A rows, then B rows
@@ -979,7 +998,7 @@ __device__ __forceinline__ void fused_build(const char* in, char* op, int lane, 
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
         const int rr = lane + 64 * i;
-        const f32x4 am = *reinterpret_cast<const f32x4*>(in + rr * 64 + g * 16);
+        const f32x4 am = *reinterpret_cast<const f32x4*>(in + rr * 64 + ((g ^ amp_swz(rr)) << 4));
         u32x4 hi, lo;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -1156,7 +1175,7 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
             const int c = (i * 4 + wave) * 64 + lane;
             const int rr = c >> 2;
             const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
-            ampoff[i] = ant * 16 + (c & 3) * 4;
+            ampoff[i] = ant * 16 + ((c & 3) ^ amp_swz(rr)) * 4;
         }
         const int crdoff = min(wave, 2) * (int)a.kpad + 2 * (lane & 7);
 
@@ -1286,7 +1305,8 @@ __device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase
     const double* cz = reinterpret_cast<const double*>(in + L::AMP + 2048 + g * 32);
     f32x4 am[RPT];
 #pragma unroll
-    for (int i = 0; i < RPT; ++i) am[i] = *reinterpret_cast<const f32x4*>(in + (rbase + 64 * i) * 64 + g * 16);
+    for (int i = 0; i < RPT; ++i)
+        am[i] = *reinterpret_cast<const f32x4*>(in + (rbase + 64 * i) * 64 + ((g ^ amp_swz(rbase + 64 * i)) << 4));
     u32x4 hi[RPT], lo[RPT];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -1312,16 +1332,16 @@ __device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase
     }
 }
 
-// The MFMAs of one stage for one wave: both 8-source steps, fragments read
-// just before use.
+// The MFMAs of one stage for one wave: both 8-source steps, the second
+// step's fragments read while the first step's MFMAs run.
 template <int MODE>
 __device__ __forceinline__ void fused8_mma(const char* As, const char* Bs, int ra0, int rb0, int lh,
                                            f32x16 (&re)[2], f32x16 (&im)[2]) {
-    Frag21 f;
-    read_frag21(f, As, Bs, ra0, rb0, lh);
-    frag21_mma<MODE>(f, re, im);
-    read_frag21(f, As, Bs, ra0, rb0, 2 + lh);
-    frag21_mma<MODE>(f, re, im);
+    Frag21 f0, f1;
+    read_frag21(f0, As, Bs, ra0, rb0, lh);
+    read_frag21(f1, As, Bs, ra0, rb0, 2 + lh);   // in flight during the first step's MFMAs
+    frag21_mma<MODE>(f0, re, im);
+    frag21_mma<MODE>(f1, re, im);
 }
 
 template <int MODE, int RPT, int KIND>
@@ -1440,7 +1460,7 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
             const int c = (i * 8 + wave) * 64 + lane;
             const int rr = c >> 2;
             const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
-            ampoff[i] = ant * 16 + (c & 3) * 4;
+            ampoff[i] = ant * 16 + ((c & 3) ^ amp_swz(rr)) * 4;
         }
         const int crdoff = min(wave & 3, 2) * (int)a.kpad + 2 * (lane & 7);
 
@@ -1492,6 +1512,369 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// 192-tile build-in-place contraction: 4 waves, one per SIMD (each with up to
+// 512 VGPRs), a 96x96 wave tile (3x3 MFMA blocks of 32x32, 288 accumulator
+// registers).  Against the 128-tile kernel a tile covers 2.25x the output for
+// 1.5x the operand rows it builds, so at N = 350 (two 192-blocks, three
+// tiles) each antenna's operands are built twice per unit instead of three
+// times, and a B fragment serves three MFMA blocks instead of two.  Each
+// 32x32 output block runs the MFMA chain of the 128-tile kernels (per stage:
+// source groups 0/1, then 2/3; hi.hi, hi.lo, lo.hi; one chain per split), so
+// the partial sums and the visibilities are those kernels' bit for bit.
+//
+// LDS: two input slots (amplitudes of the 384 rows and the stage's x, y, z)
+// filled by global_load_lds one stage ahead, and two operand buffers (A
+// rows, then B rows).  Per stage k, after one barrier: issue the inputs of
+// stage k+2, build operand stage k+1 (wave w builds source group w, lane l
+// rows l, l+64, ...) and run the MFMAs of stage k.
+//
+// Work order: the 32 workgroups resident on one XCD take the same (split,
+// tile) of up to 32 units (frequencies of one time) at once, so the
+// amplitude and coordinate rows they read come from that XCD's L2.
+struct T192 {
+    static constexpr int TILE = 192;
+    static constexpr int ROWS = 2 * TILE;                // A rows, then B rows
+    static constexpr int AMP = ROWS * 64;                // 24 KB of amplitudes per stage
+    static constexpr int CRD = 512;                      // x, y, z of 16 sources (128 B each)
+    static constexpr int IN = AMP + CRD;
+    static constexpr int OP = TILE * STAGE_BYTES;        // 24 KB per operand block
+    static constexpr int NAMP = ROWS * 4 / 256;          // amplitude loads per thread per stage
+    static constexpr int NAMP_DIAG = TILE * 4 / 256;     // diagonal tiles: A rows only
+};
+
+// Inputs of one stage: NA amplitude loads per thread, and coordinate w of
+// the stage's 16 sources by lanes 0-7 of wave w < 3 (waves 0-2 issue NA + 1
+// loads per stage, wave 3 NA).
+// LDS pointers kept in the LDS address space (32-bit addresses: the operand
+// and fragment addresses of the 192-tile loop stay in single registers).
+typedef __attribute__((address_space(3))) char lds_char;
+template <typename T>
+__device__ __forceinline__ T lds_ld(const lds_char* p) {
+    return *reinterpret_cast<const __attribute__((address_space(3))) T*>(p);
+}
+template <typename T>
+__device__ __forceinline__ void lds_st(lds_char* p, const T& v) {
+    *reinterpret_cast<__attribute__((address_space(3))) T*>(p) = v;
+}
+
+// Inputs of one stage: NA amplitude loads per thread, and coordinate w of
+// the stage's 16 sources by lanes 0-7 of wave w < 3 (waves 0-2 issue NA + 1
+// loads per stage, wave 3 NA).
+template <int NA>
+__device__ __forceinline__ void t192_issue(lds_char* slot, const float* amp_st, const double* crd_st,
+                                           const int* ampoff, int crdoff, int wave, int lane) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+        __builtin_amdgcn_global_load_lds(amp_st + ampoff[i], slot + (i * 4 + wave) * 1024, 16, 0, 0);
+    if (wave < 3 && lane < 8)
+        __builtin_amdgcn_global_load_lds(crd_st + crdoff, slot + T192::AMP + wave * 128, 16, 0, 0);
+}
+
+// Operand row r = lane + 64 I (rows 0-191: the A block, 192-383: the B
+// block) of source group g from an input slot, as fused8_build builds a row;
+// the antenna position comes from LDS (pos: x[384], y[384], z[384]).
+template <int MODE, int I>
+__device__ __forceinline__ void t192_build_row(const lds_char* in, lds_char* op, const lds_char* pos, int lane,
+                                               int g, const double (&x)[4], const double (&y)[4],
+                                               const double (&z)[4], double il) {
+    using L = T192;
+    const int r = lane + 64 * I;
+    const f32x4 am = lds_ld<f32x4>(in + r * 64 + ((g ^ amp_swz(r)) << 4));
+    const double ax = lds_ld<double>(pos + 8 * r);
+    const double ay = lds_ld<double>(pos + 8 * (L::ROWS + r));
+    const double az = lds_ld<double>(pos + 8 * (2 * L::ROWS + r));
+    u32x4 hi, lo;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double d = fma(ax, x[j], fma(ay, y[j], az * z[j]));
+        float sn, cs;
+        sincos_turns_hw(d * il, &sn, &cs);
+        uint32_t hw, lw;
+        split_phasor<MODE>(am[j], cs, sn, hw, lw);
+        hi[j] = hw;
+        lo[j] = lw;
+    }
+    lds_char* base = (I < 3) ? op : op + L::OP;
+    const int rr = (I < 3) ? r : r - L::TILE;
+    lds_st(base + lds_off<true>(rr, 2 * g), hi);
+    lds_st(base + lds_off<true>(rr, 2 * g + 1), lo);
+}
+
+// Chunk C (0-5) of one stage: the MFMAs of B column block j = C % 3 in step
+// s = C / 3 (18 MFMAs, Re += hi.hi + hi.lo + lo.hi and Im with the rotated B
+// fragments, per block in the order of frag21_mma), interleaved with the
+// build of one operand row of the next stage (RPT 6: row C; RPT 3: row C/2
+// at even C), so the matrix pipe and the VALU of the wave's SIMD work side
+// by side.
+template <int MODE, int RPT, int KIND, int C>
+__device__ __forceinline__ void t192_chunk(const lds_char* As, const lds_char* Bs, int ra0, int rb0, int lh,
+                                           u32x4 (&ahi)[3], u32x4 (&alo)[3], f32x16 (&re)[3][3],
+                                           f32x16 (&im)[3][3], const lds_char* bin, lds_char* bop,
+                                           const lds_char* pos, int lane, int g, const double (&x)[4],
+                                           const double (&y)[4], const double (&z)[4], double il) {
+    constexpr int S = C / 3, J = C % 3;
+    const int gg = 2 * S + lh;
+    if constexpr (KIND == 2) {
+        if constexpr (J == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                ahi[i] = lds_ld<u32x4>(As + lds_off<true>(ra0 + 32 * i, 2 * gg));
+                alo[i] = lds_ld<u32x4>(As + lds_off<true>(ra0 + 32 * i, 2 * gg + 1));
+            }
+        }
+        const u32x4 bhi = lds_ld<u32x4>(Bs + lds_off<true>(rb0 + 32 * J, 2 * gg));
+        const u32x4 blo = lds_ld<u32x4>(Bs + lds_off<true>(rb0 + 32 * J, 2 * gg + 1));
+        const u32x4 bhr = rot_neg16(bhi), blr = rot_neg16(blo);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            re[i][J] = mfma16<MODE>(ahi[i], bhi, re[i][J]);
+            im[i][J] = mfma16<MODE>(ahi[i], bhr, im[i][J]);
+            re[i][J] = mfma16<MODE>(ahi[i], blo, re[i][J]);
+            im[i][J] = mfma16<MODE>(ahi[i], blr, im[i][J]);
+            re[i][J] = mfma16<MODE>(alo[i], bhi, re[i][J]);
+            im[i][J] = mfma16<MODE>(alo[i], bhr, im[i][J]);
+        }
+    }
+    constexpr int EVERY = 6 / RPT;
+    if constexpr (C % EVERY == 0) t192_build_row<MODE, C / EVERY>(bin, bop, pos, lane, g, x, y, z, il);
+    if constexpr (KIND == 2) {
+        // one MFMA, then VALU work of the build (masks: 0x008 MFMA, 0x002
+        // VALU, 0x400 transcendental, 0x100 LDS read)
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+            if (k % 2 == 0) __builtin_amdgcn_sched_group_barrier(0x400, 1, 0);
+        }
+    }
+}
+
+// One split of one tile.  RPT 6: off-diagonal tile (A and B blocks built);
+// RPT 3: diagonal tile (A block only, B = A).  KIND 0: the wave has no block
+// of the triangle (it builds and keeps the barriers); 2: all nine blocks.
+template <int MODE, int RPT, int KIND>
+__device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
+                                           const lds_char* pos, const float* amp_t, const double* crd_t,
+                                           size_t amp_stage, int s_begin, int n_st, const int* ampoff, int crdoff,
+                                           int lane, int wave, double il, f32x16 (&re)[3][3],
+                                           f32x16 (&im)[3][3]) {
+    using L = T192;
+    constexpr int NA = (RPT == 6) ? L::NAMP : L::NAMP_DIAG;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int lrow = lane & 31, lh = lane >> 5;
+    const int ra0 = wr * 96 + lrow, rb0 = wc * 96 + lrow;
+    constexpr int boff = (RPT == 6) ? L::OP : 0;
+    const int g = wave;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) { re[i][j][e] = 0; im[i][j][e] = 0; }
+#define HV_T192ISSUE(slot, st)                                                                           \
+    t192_issue<NA>(slot, amp_t + (size_t)(s_begin + (st)) * amp_stage, crd_t + (size_t)(s_begin + (st)) * 16, \
+                   ampoff, crdoff, wave, lane)
+#define HV_T192COORDS(in)                                                         \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                               \
+        x[j] = lds_ld<double>((in) + L::AMP + g * 32 + 8 * j);                    \
+        y[j] = lds_ld<double>((in) + L::AMP + 128 + g * 32 + 8 * j);              \
+        z[j] = lds_ld<double>((in) + L::AMP + 256 + g * 32 + 8 * j);              \
+    }
+
+    HV_T192ISSUE(in0, 0);
+    HV_T192ISSUE(in1, min(1, n_st - 1));   // a one-stage split re-reads stage 0 (never built)
+    if (wave < 3)                          // stage 0 has landed (stage 1 may be in flight)
+        wait_vmcnt<NA + 1>();
+    else
+        wait_vmcnt<NA>();
+    __syncthreads();                       // also publishes the antenna positions in LDS
+    double x[4], y[4], z[4];
+    HV_T192COORDS(in0);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        if (i == 0) t192_build_row<MODE, 0>(in0, op0, pos, lane, g, x, y, z, il);
+        if (i == 1) t192_build_row<MODE, 1>(in0, op0, pos, lane, g, x, y, z, il);
+        if (i == 2) t192_build_row<MODE, 2>(in0, op0, pos, lane, g, x, y, z, il);
+        if (i == 3) t192_build_row<MODE, 3>(in0, op0, pos, lane, g, x, y, z, il);
+        if (i == 4) t192_build_row<MODE, 4>(in0, op0, pos, lane, g, x, y, z, il);
+        if (i == 5) t192_build_row<MODE, 5>(in0, op0, pos, lane, g, x, y, z, il);
+    }
+    for (int st = 0; st < n_st; ++st) {
+        const bool odd = st & 1;
+        const lds_char* cur = odd ? op1 : op0;
+        // stage st+1's inputs have landed and every wave's build of stage st
+        // is in LDS; slot st & 1 (read by that build) and the operand buffer
+        // of stage st-1 are free
+        wait_vm_lgkm<0, 0>();
+        __builtin_amdgcn_s_barrier();
+        if (st + 2 < n_st) HV_T192ISSUE(odd ? in1 : in0, st + 2);
+        // build stage st+1 (at the last stage: stage n_st-2 or 0 again, from a
+        // slot nobody has refilled, into the buffer no MFMA reads) beside the
+        // MFMAs of stage st
+        const lds_char* bin = odd ? in0 : in1;
+        lds_char* bop = odd ? op0 : op1;
+        HV_T192COORDS(bin);
+        u32x4 ahi[3], alo[3];
+#define HV_T192CHUNK(C)                                                                                     \
+    t192_chunk<MODE, RPT, KIND, C>(cur, cur + boff, ra0, rb0, lh, ahi, alo, re, im, bin, bop, pos, lane, g, x, y, \
+                                   z, il)
+        HV_T192CHUNK(0);
+        HV_T192CHUNK(1);
+        HV_T192CHUNK(2);
+        HV_T192CHUNK(3);
+        HV_T192CHUNK(4);
+        HV_T192CHUNK(5);
+#undef HV_T192CHUNK
+    }
+#undef HV_T192COORDS
+#undef HV_T192ISSUE
+    wait_vm_lgkm<0, 0>();
+    __builtin_amdgcn_s_barrier();
+}
+
+// Work item L -> (unit, split, tile), or false for a slot with no work.
+// Combos (unit group, split, tile) go round-robin to the 8 XCDs (block id
+// mod 8); within an XCD, consecutive items are the ug units of one combo, so
+// the workgroups resident on an XCD at once share their input rows.
+__device__ inline bool t192_decode(int nunits, int nsplit, int ntiles, int ug, int64_t L, int& unit, int& split,
+                                   int& tile) {
+    const int x = (int)(L & 7);
+    const int64_t q = L >> 3;
+    const int64_t combo = (q / ug) * 8 + x;
+    const int u_in = (int)(q % ug);
+    const int nug = (nunits + ug - 1) / ug;
+    if (combo >= (int64_t)nug * nsplit * ntiles) return false;
+    tile = (int)(combo % ntiles);
+    const int64_t rest = combo / ntiles;
+    split = (int)(rest % nsplit);
+    unit = (int)(rest / nsplit) * ug + u_in;
+    return unit < nunits;
+}
+
+// Units per combo for a grid of g workgroups: at most g / 8 (the workgroups
+// of one XCD), spread evenly over the launch's units.
+__host__ __device__ inline int t192_unit_group(int nunits, int64_t g) {
+    const int per_xcd = (int)(g / 8 > 0 ? g / 8 : 1);
+    const int nug = (nunits + per_xcd - 1) / per_xcd;
+    return (nunits + nug - 1) / nug;
+}
+
+__host__ __device__ inline int64_t t192_items(int nunits, int nsplit, int ntiles, int ug) {
+    const int64_t nug = (nunits + ug - 1) / ug;
+    const int64_t ncombo = nug * nsplit * ntiles;
+    return (ncombo + 7) / 8 * 8 * ug;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void herk_t192_kernel(FusedArgs a) {
+    using L = T192;
+    constexpr int TILE = L::TILE;
+    __shared__ __attribute__((aligned(1024))) char s_in0[L::IN];
+    __shared__ __attribute__((aligned(1024))) char s_in1[L::IN];
+    __shared__ __attribute__((aligned(1024))) char s_op0[2 * L::OP];
+    __shared__ __attribute__((aligned(1024))) char s_op1[2 * L::OP];
+    __shared__ double s_pos[3 * L::ROWS];  // antenna x[384], y[384], z[384] of the tile's rows
+    lds_char* in0 = (lds_char*)s_in0;
+    lds_char* in1 = (lds_char*)s_in1;
+    lds_char* op0 = (lds_char*)s_op0;
+    lds_char* op1 = (lds_char*)s_op1;
+    const lds_char* pos = (const lds_char*)s_pos;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / 16);
+    const size_t amp_stage = (size_t)a.nant * 16;
+    const size_t amp_time = (size_t)a.kpad * a.nant;
+    constexpr int STAGES_SPLIT = KSPLIT / 16;
+    const int ug = t192_unit_group(a.nunits, gridDim.x);
+    const int64_t nitems = t192_items(a.nunits, a.nsplit, a.ntiles, ug);
+
+    for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
+        int unit, split, tile, bi, bj;
+        if (!t192_decode(a.nunits, a.nsplit, a.ntiles, ug, Lw, unit, split, tile)) continue;
+        decode_tile(tile, a.nb, bi, bj);
+        const bool diag = (bi == bj);
+        const int s_begin = split * STAGES_SPLIT;
+        const int s_end = min(s_begin + STAGES_SPLIT, kst);
+        if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
+        const int n_st = s_end - s_begin;
+        const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
+        const double il = a.inv_lambda[fl];   // turns per metre
+        const float* amp_t = a.amp + (size_t)(a.amp_per_unit ? unit : tb) * amp_time;
+        const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
+
+        unsigned smask = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int R = wr * 3 + i, Cb = wc * 3 + j;
+                const bool on = (bi * TILE + R * 32 < a.nant) && (bj * TILE + Cb * 32 < a.nant) && !(diag && R > Cb);
+                smask |= (unsigned)on << (i * 3 + j);
+            }
+        smask = __builtin_amdgcn_readfirstlane(smask);
+
+        // amplitude chunk c = (i*4 + wave)*64 + lane: row c/4, part c%4
+        int ampoff[L::NAMP];
+#pragma unroll
+        for (int i = 0; i < L::NAMP; ++i) {
+            const int c = (i * 4 + wave) * 64 + lane;
+            const int rr = c >> 2;
+            const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
+            ampoff[i] = ant * 16 + ((c & 3) ^ amp_swz(rr)) * 4;
+        }
+        const int crdoff = min(wave, 2) * (int)a.kpad + 2 * lane;   // used by lanes 0-7 of waves 0-2
+
+        // antenna positions of the tile's rows (the previous item's builds
+        // are behind its last barrier; the split's first barrier publishes these)
+        for (int rr = threadIdx.x; rr < L::ROWS; rr += 256) {
+            const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
+            s_pos[rr] = a.antpos[3 * ant];
+            s_pos[L::ROWS + rr] = a.antpos[3 * ant + 1];
+            s_pos[2 * L::ROWS + rr] = a.antpos[3 * ant + 2];
+        }
+
+        f32x16 re[3][3], im[3][3];
+#define HV_T192SPLIT(RPT, KIND)                                                                             \
+    t192_split<MODE, RPT, KIND>(in0, in1, op0, op1, pos, amp_t, crd_t, amp_stage, s_begin, n_st, ampoff, crdoff, \
+                                lane, wave, il, re, im)
+        // a wave with any block of the triangle computes all nine (blocks
+        // outside it hold finite values nobody stores): no wave of a tile
+        // does more than nine, so this costs no time and keeps the MFMA
+        // sequence free of branches
+        if (diag) {
+            if (smask) HV_T192SPLIT(3, 2);
+            else HV_T192SPLIT(3, 0);
+        } else {
+            if (smask) HV_T192SPLIT(6, 2);
+            else HV_T192SPLIT(6, 0);
+        }
+#undef HV_T192SPLIT
+        if (!smask) continue;
+
+        constexpr size_t TSZ = (size_t)TILE * TILE;
+        float* out = reinterpret_cast<float*>(a.partials) +
+                     (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                if (!(smask & (1u << (i * 3 + j)))) continue;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                    const int row = wr * 96 + i * 32 + r;
+                    const int col = wc * 96 + j * 32 + (lane & 31);
+                    out[(size_t)row * TILE + col] = re[i][j][e];
+                    out[TSZ + (size_t)row * TILE + col] = im[i][j][e];
+                }
+            }
+    }
+}
+
 #undef HV_LDS
 
 // Resident workgroups of a fused kernel on the whole device (a multiple of 8).
@@ -1533,6 +1916,31 @@ static hipError_t launch_fused8_t(const FusedArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int MODE>
+static int64_t t192_resident() {
+    static const int64_t cached = [] {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_t192_kernel<MODE>, 256, 0) != hipSuccess)
+            return (int64_t)0;
+        return (int64_t)ncu * per_cu / 8 * 8;
+    }();
+    return cached;
+}
+
+template <int MODE>
+static hipError_t launch_t192_t(const FusedArgs& a, hipStream_t s) {
+    if ((int64_t)a.nunits * a.nsplit * a.ntiles <= 0) return hipSuccess;
+    if (a.kpad % 16) return hipErrorInvalidValue;
+    const int64_t res = t192_resident<MODE>();
+    const int64_t g0 = res > 0 ? res : 256;
+    const int64_t items = t192_items(a.nunits, a.nsplit, a.ntiles, t192_unit_group(a.nunits, g0));
+    const int64_t grid = g0 < items ? g0 : items;
+    hipLaunchKernelGGL((herk_t192_kernel<MODE>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <int MODE, int TILE>
 static hipError_t launch_fused_t(const FusedArgs& a, hipStream_t s) {
     const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
@@ -1549,6 +1957,8 @@ hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s) {
     if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_fused8_t<MODE_SPLIT_BF16>(a, s);
     if (mode == MODE_SPLIT_FP16 && tile == 64) return launch_fused_t<MODE_SPLIT_FP16, 64>(a, s);
     if (mode == MODE_SPLIT_FP16 && tile == 128) return launch_fused8_t<MODE_SPLIT_FP16>(a, s);
+    if (mode == MODE_SPLIT_BF16 && tile == 192) return launch_t192_t<MODE_SPLIT_BF16>(a, s);
+    if (mode == MODE_SPLIT_FP16 && tile == 192) return launch_t192_t<MODE_SPLIT_FP16>(a, s);
     return hipErrorInvalidValue;
 }
 
@@ -1596,16 +2006,13 @@ void herk_init() {
     (void)fused8_resident<MODE_SPLIT_BF16>();
     (void)fused_resident<MODE_SPLIT_FP16, 64>();
     (void)fused8_resident<MODE_SPLIT_FP16>();
+    (void)t192_resident<MODE_SPLIT_BF16>();
+    (void)t192_resident<MODE_SPLIT_FP16>();
 }
 
 int herk_prefetch(int mode, int tile) {
-    if (g_pf_override && (g_pf_override != 5 || split_mode(mode)) && !(mode == MODE_FP64 && g_pf_override == 2))
-        return g_pf_override;
     (void)tile;
-    // split modes build their operands in place (whole simulations, 2^18
-    // sources: faster than the phasor build + LDS ring at every array size
-    // measured, 1.2x at N = 37 and 350, 1.6x at N = 128, and no operand buffer)
-    return split_mode(mode) ? 5 : 1;
+    return pf_choice(mode);
 }
 
 // Contraction of operands already in HBM (the built-in variant, or with

@@ -184,15 +184,18 @@ int hv_bench_copy(hv_ctx* ctx, size_t bytes, int reps, double* ms_per_copy);
 int hv_bench_mfma(hv_ctx* ctx, int mode, int iters, double* tflops);
 /* The Hermitian contraction alone on synthetic operands of `units` units of
  * nant x nsrc; tile (64/128) and pf select a kernel variant (as for
- * hv_set_herk_variant), 0 = the production choice.  Returns ms per launch. */
+ * hv_set_herk_variant), 0 = the production choice (its HBM-operand form: 128
+ * where the production choice is a 192-tile in-place build).  Returns ms per
+ * launch. */
 int hv_bench_herk(hv_ctx* ctx, int mode, int nant, int64_t nsrc, int units, int tile, int pf, int reps,
                   double* ms_per_launch);
 /* Process-wide choice of contraction variant for later calls (tuning and
- * tests): tile 64/128, pf 1/2 (register-staged prefetch depth), 3 (LDS ring,
- * one workgroup per work item), 4 (LDS ring on a persistent grid) or 5 (split
- * modes: operands built inside the contraction from A sqrt(I) per time, no
- * operand buffer; hv_k_herk and hv_bench_herk, which take operands, use 4);
- * 0 restores the built-in choice.  Not safe while other calls run. */
+ * tests): tile 64/128/192, pf 1/2 (register-staged prefetch depth), 3 (LDS
+ * ring, one workgroup per work item), 4 (LDS ring on a persistent grid) or 5
+ * (split modes: operands built inside the contraction from A sqrt(I) per
+ * time, no operand buffer; hv_k_herk and hv_bench_herk, which take operands,
+ * use 4).  192-tiles exist only with pf 5 (or 0).  0 restores the built-in
+ * choice.  Not safe while other calls run. */
 int hv_set_herk_variant(int tile, int pf);
 /* The contraction variant (tile edge, pf as above) that calls of this mode and
  * antenna count use now. */

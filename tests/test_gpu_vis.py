@@ -213,13 +213,14 @@ def test_batch_schedules_agree_bitwise(gemm, monkeypatch):
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("tile", [64, 128, 192])
 def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile):
     # pf 5 builds the operand stages inside the contraction from A sqrt(I);
     # pf 4 contracts the stages phasor_kernel wrote: same operand bits, same
-    # MFMA sequence, so the same visibilities bit for bit.  N = 300 leaves a
-    # partial last tile; K spans several 8192-source splits; 3 times x 4
-    # frequencies
+    # MFMA sequence per 32x32 block, so the same visibilities bit for bit
+    # (192-tiles exist only in place: they are checked against the 128-tile
+    # ring, whose blocks run the same chains).  N = 300 leaves a partial
+    # last tile; K spans several 8192-source splits; 3 times x 4 frequencies
     from hera_gpu_amd import _lib
     lib = _lib.load()
     w = configs.make_hera350(nsrc=3 * 8192 + 1000, ntimes=3, nfreq=4)
@@ -227,7 +228,7 @@ def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile):
     outs = []
     try:
         for pf in (4, 5):
-            _lib.check(lib.hv_set_herk_variant(tile, pf))
+            _lib.check(lib.hv_set_herk_variant(128 if (tile == 192 and pf == 4) else tile, pf))
             outs.append(vis_gpu(ap, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, bm, gemm=gemm))
     finally:
         _lib.check(lib.hv_set_herk_variant(0, 0))
