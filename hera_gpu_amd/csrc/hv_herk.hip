@@ -38,11 +38,13 @@
 namespace hv {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDS_STRIDE = 144;
@@ -73,10 +75,10 @@ static int pf_choice(int mode) {
 }
 
 // Tile edge: 64 (FP32, FP64, small arrays), 128 (split modes above 64
-// antennas), or 192 (split modes, in-place builds only; selected by an
-// override).  Overrides apply where the variant exists for the mode.
+// antennas), 192 (split modes above 256 antennas; in-place builds only).
+// Overrides apply where the variant exists for the mode.
 int herk_tile(int mode, int nant) {
-    int t = split_mode(mode) ? (nant > 64 ? 128 : 64) : 64;
+    int t = split_mode(mode) ? (nant > 256 ? 192 : nant > 64 ? 128 : 64) : 64;
     if (g_tile_override && (g_tile_override == 64 || split_mode(mode))) t = g_tile_override;
     if (t == 192 && pf_choice(mode) != 5) t = 128;
     return t;
@@ -1513,23 +1515,29 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// 192-tile build-in-place contraction: 4 waves, one per SIMD (each with up to
-// 512 VGPRs), a 96x96 wave tile (3x3 MFMA blocks of 32x32, 288 accumulator
-// registers).  Against the 128-tile kernel a tile covers 2.25x the output for
-// 1.5x the operand rows it builds, so at N = 350 (two 192-blocks, three
-// tiles) each antenna's operands are built twice per unit instead of three
-// times, and a B fragment serves three MFMA blocks instead of two.  Each
-// 32x32 output block runs the MFMA chain of the 128-tile kernels (per stage:
-// source groups 0/1, then 2/3; hi.hi, hi.lo, lo.hi; one chain per split), so
-// the partial sums and the visibilities are those kernels' bit for bit.
+// 192-tile build-in-place contraction: 8 waves, two per SIMD.  The SIMD's
+// pair of waves owns a set of 32x32 MFMA blocks; one wave of the pair
+// accumulates Re, the other Im, of the same blocks (144 accumulator
+// registers each at most), reading the same operand fragments.  Against the
+// 128-tile kernel a tile covers 2.25x the output for 1.5x the operand rows
+// it builds: at N = 350 (two 192-blocks, three tiles) each antenna's
+// operands are built twice per unit instead of three times.  Blocks:
+//   off-diagonal tiles: SIMD q takes the 96x96 quadrant (q / 2, q % 2),
+//     3x3 blocks;
+//   diagonal tiles: the 21 blocks (r, c), r <= c, of the 6x6 block grid, 6
+//     or 5 per SIMD (q 0: row 0; q 1: row 1; q 2: row 2 and (5, 5); q 3:
+//     rows 3 and 4), so a diagonal tile costs 2/3 of an off-diagonal one.
+// Each block runs the MFMA chain of the 128-tile kernels (per stage: source
+// groups 0/1, then 2/3; hi.hi, hi.lo, lo.hi; one chain per split), so the
+// partial sums and the visibilities are those kernels' bit for bit.
 //
-// LDS: two input slots (amplitudes of the 384 rows and the stage's x, y, z)
-// filled by global_load_lds one stage ahead, and two operand buffers (A
-// rows, then B rows).  Per stage k, after one barrier: issue the inputs of
-// stage k+2, build operand stage k+1 (wave w builds source group w, lane l
-// rows l, l+64, ...) and run the MFMAs of stage k.
+// LDS: two input slots (amplitudes of the 384 rows, the stage's x, y, z)
+// filled by global_load_lds one stage ahead and two operand buffers (A rows,
+// then B rows); each lane keeps the antenna positions of its rows.  Per stage k, after one
+// barrier: issue the inputs of stage k+2, build operand rows of stage k+1
+// (rows per lane: T192Rows), then run the MFMAs of stage k.
 //
-// Work order: the 32 workgroups resident on one XCD take the same (split,
+// Work order: the workgroups resident on one XCD take the same (split,
 // tile) of up to 32 units (frequencies of one time) at once, so the
 // amplitude and coordinate rows they read come from that XCD's L2.
 struct T192 {
@@ -1539,15 +1547,10 @@ struct T192 {
     static constexpr int CRD = 512;                      // x, y, z of 16 sources (128 B each)
     static constexpr int IN = AMP + CRD;
     static constexpr int OP = TILE * STAGE_BYTES;        // 24 KB per operand block
-    static constexpr int NAMP = ROWS * 4 / 256;          // amplitude loads per thread per stage
-    static constexpr int NAMP_DIAG = TILE * 4 / 256;     // diagonal tiles: A rows only
+    static constexpr int NAMP = ROWS * 4 / 512;          // amplitude loads per thread per stage
 };
 
-// Inputs of one stage: NA amplitude loads per thread, and coordinate w of
-// the stage's 16 sources by lanes 0-7 of wave w < 3 (waves 0-2 issue NA + 1
-// loads per stage, wave 3 NA).
-// LDS pointers kept in the LDS address space (32-bit addresses: the operand
-// and fragment addresses of the 192-tile loop stay in single registers).
+// LDS pointers kept in the LDS address space (32-bit addresses).
 typedef __attribute__((address_space(3))) char lds_char;
 template <typename T>
 __device__ __forceinline__ T lds_ld(const lds_char* p) {
@@ -1558,149 +1561,206 @@ __device__ __forceinline__ void lds_st(lds_char* p, const T& v) {
     *reinterpret_cast<__attribute__((address_space(3))) T*>(p) = v;
 }
 
-// Inputs of one stage: NA amplitude loads per thread, and coordinate w of
-// the stage's 16 sources by lanes 0-7 of wave w < 3 (waves 0-2 issue NA + 1
-// loads per stage, wave 3 NA).
-template <int NA>
+// Inputs of one stage: amplitude chunks c = (i*8 + wave)*64 + lane of the
+// NROW rows (row c/4), and coordinate w of the stage's 16 sources by lanes
+// 0-7 of wave w < 3.
+template <int NROW>
 __device__ __forceinline__ void t192_issue(lds_char* slot, const float* amp_st, const double* crd_st,
                                            const int* ampoff, int crdoff, int wave, int lane) {
 #pragma unroll
-    for (int i = 0; i < NA; ++i)
-        __builtin_amdgcn_global_load_lds(amp_st + ampoff[i], slot + (i * 4 + wave) * 1024, 16, 0, 0);
+    for (int i = 0; i < (NROW * 4 + 511) / 512; ++i)
+        if ((i * 8 + wave) * 64 < NROW * 4)
+            __builtin_amdgcn_global_load_lds(amp_st + ampoff[i], slot + (i * 8 + wave) * 1024, 16, 0, 0);
     if (wave < 3 && lane < 8)
         __builtin_amdgcn_global_load_lds(crd_st + crdoff, slot + T192::AMP + wave * 128, 16, 0, 0);
 }
 
-// Operand row r = lane + 64 I (rows 0-191: the A block, 192-383: the B
-// block) of source group g from an input slot, as fused8_build builds a row;
-// the antenna position comes from LDS (pos: x[384], y[384], z[384]).
-template <int MODE, int I>
-__device__ __forceinline__ void t192_build_row(const lds_char* in, lds_char* op, const lds_char* pos, int lane,
-                                               int g, const double (&x)[4], const double (&y)[4],
-                                               const double (&z)[4], double il) {
+// Rows a lane builds: lane l takes source group l / 16 (4 sources) of rows
+// row0 + (l % 16) + 16 k, k < NR.  Off-diagonal tiles: wave w builds rows
+// 48 w .. 48 w + 47 (NR 3; rows 0-191 are the A block, 192-383 the B block);
+// diagonal tiles (192 rows): waves 0-3 rows 32 w .. 32 w + 31 (NR 2), waves
+// 4-7 rows 128 + 16 (w - 4) .. + 15 (NR 1).
+template <bool DIAG, bool IM>
+struct T192Rows {
+    static constexpr int NR = DIAG ? (IM ? 1 : 2) : 3;
+    __device__ static int row0(int wave) { return DIAG ? (IM ? 128 + 16 * (wave - 4) : 32 * wave) : 48 * wave; }
+};
+
+// Operand rows of one lane (T192Rows) for its source group of the stage,
+// with the arithmetic of fused8_build; ax/ay/az: the rows' antenna positions.
+template <int MODE, int NR>
+__device__ __forceinline__ void t192_build(const lds_char* in, lds_char* op, int row0, int lane,
+                                           const double (&ax)[3], const double (&ay)[3], const double (&az)[3],
+                                           double il) {
     using L = T192;
-    const int r = lane + 64 * I;
-    const f32x4 am = lds_ld<f32x4>(in + r * 64 + ((g ^ amp_swz(r)) << 4));
-    const double ax = lds_ld<double>(pos + 8 * r);
-    const double ay = lds_ld<double>(pos + 8 * (L::ROWS + r));
-    const double az = lds_ld<double>(pos + 8 * (2 * L::ROWS + r));
-    u32x4 hi, lo;
+    const int g = lane >> 4;
+    double x[4], y[4], z[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const double d = fma(ax, x[j], fma(ay, y[j], az * z[j]));
-        float sn, cs;
-        sincos_turns_hw(d * il, &sn, &cs);
-        uint32_t hw, lw;
-        split_phasor<MODE>(am[j], cs, sn, hw, lw);
-        hi[j] = hw;
-        lo[j] = lw;
+        x[j] = lds_ld<double>(in + L::AMP + 32 * g + 8 * j);
+        y[j] = lds_ld<double>(in + L::AMP + 128 + 32 * g + 8 * j);
+        z[j] = lds_ld<double>(in + L::AMP + 256 + 32 * g + 8 * j);
     }
-    lds_char* base = (I < 3) ? op : op + L::OP;
-    const int rr = (I < 3) ? r : r - L::TILE;
-    lds_st(base + lds_off<true>(rr, 2 * g), hi);
-    lds_st(base + lds_off<true>(rr, 2 * g + 1), lo);
-}
-
-// Chunk C (0-5) of one stage: the MFMAs of B column block j = C % 3 in step
-// s = C / 3 (18 MFMAs, Re += hi.hi + hi.lo + lo.hi and Im with the rotated B
-// fragments, per block in the order of frag21_mma), interleaved with the
-// build of one operand row of the next stage (RPT 6: row C; RPT 3: row C/2
-// at even C), so the matrix pipe and the VALU of the wave's SIMD work side
-// by side.
-template <int MODE, int RPT, int KIND, int C>
-__device__ __forceinline__ void t192_chunk(const lds_char* As, const lds_char* Bs, int ra0, int rb0, int lh,
-                                           u32x4 (&ahi)[3], u32x4 (&alo)[3], f32x16 (&re)[3][3],
-                                           f32x16 (&im)[3][3], const lds_char* bin, lds_char* bop,
-                                           const lds_char* pos, int lane, int g, const double (&x)[4],
-                                           const double (&y)[4], const double (&z)[4], double il) {
-    constexpr int S = C / 3, J = C % 3;
-    const int gg = 2 * S + lh;
-    if constexpr (KIND == 2) {
-        if constexpr (J == 0) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                ahi[i] = lds_ld<u32x4>(As + lds_off<true>(ra0 + 32 * i, 2 * gg));
-                alo[i] = lds_ld<u32x4>(As + lds_off<true>(ra0 + 32 * i, 2 * gg + 1));
-            }
-        }
-        const u32x4 bhi = lds_ld<u32x4>(Bs + lds_off<true>(rb0 + 32 * J, 2 * gg));
-        const u32x4 blo = lds_ld<u32x4>(Bs + lds_off<true>(rb0 + 32 * J, 2 * gg + 1));
-        const u32x4 bhr = rot_neg16(bhi), blr = rot_neg16(blo);
+    for (int k = 0; k < NR; ++k) {
+        const int r = row0 + (lane & 15) + 16 * k;
+        const f32x4 am = lds_ld<f32x4>(in + r * 64 + ((g ^ amp_swz(r)) << 4));
+        u32x4 hi, lo;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            re[i][J] = mfma16<MODE>(ahi[i], bhi, re[i][J]);
-            im[i][J] = mfma16<MODE>(ahi[i], bhr, im[i][J]);
-            re[i][J] = mfma16<MODE>(ahi[i], blo, re[i][J]);
-            im[i][J] = mfma16<MODE>(ahi[i], blr, im[i][J]);
-            re[i][J] = mfma16<MODE>(alo[i], bhi, re[i][J]);
-            im[i][J] = mfma16<MODE>(alo[i], bhr, im[i][J]);
+        for (int j = 0; j < 4; ++j) {
+            const double d = fma(ax[k], x[j], fma(ay[k], y[j], az[k] * z[j]));
+            float sn, cs;
+            sincos_turns_hw(d * il, &sn, &cs);
+            uint32_t hw, lw;
+            split_phasor<MODE>(am[j], cs, sn, hw, lw);
+            hi[j] = hw;
+            lo[j] = lw;
         }
-    }
-    constexpr int EVERY = 6 / RPT;
-    if constexpr (C % EVERY == 0) t192_build_row<MODE, C / EVERY>(bin, bop, pos, lane, g, x, y, z, il);
-    if constexpr (KIND == 2) {
-        // one MFMA, then VALU work of the build (masks: 0x008 MFMA, 0x002
-        // VALU, 0x400 transcendental, 0x100 LDS read)
-        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-#pragma unroll
-        for (int k = 0; k < 18; ++k) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
-            if (k % 2 == 0) __builtin_amdgcn_sched_group_barrier(0x400, 1, 0);
-        }
+        const bool bblk = r >= L::TILE;
+        lds_char* base = bblk ? op + L::OP : op;
+        const int rr = bblk ? r - L::TILE : r;
+        lds_st(base + lds_off<true>(rr, 2 * g), hi);
+        lds_st(base + lds_off<true>(rr, 2 * g + 1), lo);
     }
 }
 
-// One split of one tile.  RPT 6: off-diagonal tile (A and B blocks built);
-// RPT 3: diagonal tile (A block only, B = A).  KIND 0: the wave has no block
-// of the triangle (it builds and keeps the barriers); 2: all nine blocks.
-template <int MODE, int RPT, int KIND>
+// acc += a . b (Re) or a . rot(b) (Im) as hi.hi + hi.lo + lo.hi.
+template <int MODE>
+__device__ __forceinline__ void t192_mac(const u32x4& ahi, const u32x4& alo, const u32x4& bhi, const u32x4& blo,
+                                         f32x16& acc) {
+    acc = mfma16<MODE>(ahi, bhi, acc);
+    acc = mfma16<MODE>(ahi, blo, acc);
+    acc = mfma16<MODE>(alo, bhi, acc);
+}
+
+template <bool IM>
+__device__ __forceinline__ void t192_frag(const lds_char* S, int row, int gg, u32x4& hi, u32x4& lo) {
+    hi = lds_ld<u32x4>(S + lds_off<true>(row, 2 * gg));
+    lo = lds_ld<u32x4>(S + lds_off<true>(row, 2 * gg + 1));
+    if constexpr (IM) {
+        hi = rot_neg16(hi);
+        lo = rot_neg16(lo);
+    }
+}
+
+// Off-diagonal tiles: the 3x3 blocks of quadrant (wr, wc), one component,
+// both steps of a stage.
+template <int MODE, bool IM>
+__device__ __forceinline__ void t192_quad_mma(const lds_char* As, const lds_char* Bs, int ra0, int rb0, int lh,
+                                              f32x16 (&acc)[9]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int gg = 2 * s + lh;
+        // fragments in order of first use (LDS reads return in order)
+        u32x4 bhi[3], blo[3];
+        u32x4 ahi, alo;
+        t192_frag<false>(As, ra0, gg, ahi, alo);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            t192_frag<IM>(Bs, rb0 + 32 * j, gg, bhi[j], blo[j]);
+            t192_mac<MODE>(ahi, alo, bhi[j], blo[j], acc[j]);
+        }
+#pragma unroll
+        for (int i = 1; i < 3; ++i) {
+            t192_frag<false>(As, ra0 + 32 * i, gg, ahi, alo);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) t192_mac<MODE>(ahi, alo, bhi[j], blo[j], acc[3 * i + j]);
+        }
+    }
+}
+
+// Diagonal tiles: block row R, columns C0..C1 into acc[K0 ...], one step.
+template <int MODE, bool IM, int R, int C0, int C1, int K0>
+__device__ __forceinline__ void t192_diag_row(const lds_char* S, int lrow, int gg, f32x16 (&acc)[9]) {
+    u32x4 ahi, alo;
+    t192_frag<false>(S, 32 * R + lrow, gg, ahi, alo);
+#pragma unroll
+    for (int c = C0; c <= C1; ++c) {
+        u32x4 bhi, blo;
+        t192_frag<IM>(S, 32 * c + lrow, gg, bhi, blo);
+        t192_mac<MODE>(ahi, alo, bhi, blo, acc[K0 + c - C0]);
+    }
+}
+
+template <int MODE, bool IM, int Q>
+__device__ __forceinline__ void t192_diag_mma(const lds_char* S, int lrow, int lh, f32x16 (&acc)[9]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int gg = 2 * s + lh;
+        if constexpr (Q == 0) t192_diag_row<MODE, IM, 0, 0, 5, 0>(S, lrow, gg, acc);
+        if constexpr (Q == 1) t192_diag_row<MODE, IM, 1, 1, 5, 0>(S, lrow, gg, acc);
+        if constexpr (Q == 2) {
+            t192_diag_row<MODE, IM, 2, 2, 5, 0>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 5, 5, 5, 4>(S, lrow, gg, acc);
+        }
+        if constexpr (Q == 3) {
+            t192_diag_row<MODE, IM, 3, 3, 5, 0>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 4, 4, 5, 3>(S, lrow, gg, acc);
+        }
+    }
+}
+
+// Block k of a wave's accumulator set -> (block row, block column) in the
+// tile (diagonal lists as in t192_diag_mma), or false past the set.
+__device__ inline bool t192_block(bool diag, int q, int k, int& br, int& bc) {
+    if (!diag) {
+        br = (q >> 1) * 3 + k / 3;
+        bc = (q & 1) * 3 + k % 3;
+        return true;
+    }
+    switch (q) {
+        case 0: br = 0; bc = k; return k < 6;
+        case 1: br = 1; bc = 1 + k; return k < 5;
+        case 2: br = k < 4 ? 2 : 5; bc = k < 4 ? 2 + k : 5; return k < 5;
+        default: br = k < 3 ? 3 : 4; bc = k < 3 ? 3 + k : 4 + (k - 3); return k < 5;
+    }
+}
+
+// One split of one tile for one wave.  DQ < 0: off-diagonal tile (quadrant
+// q); DQ = 0..3: diagonal tile, block list DQ.
+template <int MODE, bool IM, int DQ>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
-                                           const lds_char* pos, const float* amp_t, const double* crd_t,
-                                           size_t amp_stage, int s_begin, int n_st, const int* ampoff, int crdoff,
-                                           int lane, int wave, double il, f32x16 (&re)[3][3],
-                                           f32x16 (&im)[3][3]) {
+                                           const double* antpos, int nant, int bi, int bj, const float* amp_t,
+                                           const double* crd_t, size_t amp_stage, int s_begin, int n_st,
+                                           const int* ampoff, int crdoff, int lane, int wave, double il,
+                                           f32x16 (&acc)[9]) {
     using L = T192;
-    constexpr int NA = (RPT == 6) ? L::NAMP : L::NAMP_DIAG;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int lrow = lane & 31, lh = lane >> 5;
-    const int ra0 = wr * 96 + lrow, rb0 = wc * 96 + lrow;
-    constexpr int boff = (RPT == 6) ? L::OP : 0;
-    const int g = wave;
+    constexpr bool DIAG = DQ >= 0;
+    constexpr int NROW = DIAG ? L::TILE : L::ROWS;
+    using RW = T192Rows<DIAG, IM>;
+    constexpr int NR = RW::NR;
+    const int row0 = RW::row0(wave);
+    // antenna positions of the rows this lane builds
+    double ax[3], ay[3], az[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int e = 0; e < 16; ++e) { re[i][j][e] = 0; im[i][j][e] = 0; }
-#define HV_T192ISSUE(slot, st)                                                                           \
-    t192_issue<NA>(slot, amp_t + (size_t)(s_begin + (st)) * amp_stage, crd_t + (size_t)(s_begin + (st)) * 16, \
-                   ampoff, crdoff, wave, lane)
-#define HV_T192COORDS(in)                                                         \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                               \
-        x[j] = lds_ld<double>((in) + L::AMP + g * 32 + 8 * j);                    \
-        y[j] = lds_ld<double>((in) + L::AMP + 128 + g * 32 + 8 * j);              \
-        z[j] = lds_ld<double>((in) + L::AMP + 256 + g * 32 + 8 * j);              \
+    for (int k = 0; k < 3; ++k) {
+        const int r = row0 + (lane & 15) + 16 * (k < NR ? k : 0);
+        const int ant = min((r < L::TILE ? bi * L::TILE + r : bj * L::TILE + r - L::TILE), nant - 1);
+        ax[k] = antpos[3 * ant];
+        ay[k] = antpos[3 * ant + 1];
+        az[k] = antpos[3 * ant + 2];
     }
+    const int q = wave & 3;
+    const int lrow = lane & 31, lh = lane >> 5;
+    const int ra0 = (q >> 1) * 96 + lrow, rb0 = (q & 1) * 96 + lrow;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[k][e] = 0;
+#define HV_T192ISSUE(slot, st)                                                                               \
+    t192_issue<NROW>(slot, amp_t + (size_t)(s_begin + (st)) * amp_stage, crd_t + (size_t)(s_begin + (st)) * 16, \
+                     ampoff, crdoff, wave, lane)
+#define HV_T192MMA(buf)                                                                   \
+    do {                                                                                   \
+        if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? DQ : 0)>(buf, lrow, lh, acc);  \
+        else t192_quad_mma<MODE, IM>(buf, buf + L::OP, ra0, rb0, lh, acc);                 \
+    } while (0)
 
     HV_T192ISSUE(in0, 0);
+    wait_vmcnt<0>();
+    __syncthreads();                       // stage 0's inputs are in LDS
     HV_T192ISSUE(in1, min(1, n_st - 1));   // a one-stage split re-reads stage 0 (never built)
-    if (wave < 3)                          // stage 0 has landed (stage 1 may be in flight)
-        wait_vmcnt<NA + 1>();
-    else
-        wait_vmcnt<NA>();
-    __syncthreads();                       // also publishes the antenna positions in LDS
-    double x[4], y[4], z[4];
-    HV_T192COORDS(in0);
-#pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-        if (i == 0) t192_build_row<MODE, 0>(in0, op0, pos, lane, g, x, y, z, il);
-        if (i == 1) t192_build_row<MODE, 1>(in0, op0, pos, lane, g, x, y, z, il);
-        if (i == 2) t192_build_row<MODE, 2>(in0, op0, pos, lane, g, x, y, z, il);
-        if (i == 3) t192_build_row<MODE, 3>(in0, op0, pos, lane, g, x, y, z, il);
-        if (i == 4) t192_build_row<MODE, 4>(in0, op0, pos, lane, g, x, y, z, il);
-        if (i == 5) t192_build_row<MODE, 5>(in0, op0, pos, lane, g, x, y, z, il);
-    }
+    t192_build<MODE, NR>(in0, op0, row0, lane, ax, ay, az, il);
     for (int st = 0; st < n_st; ++st) {
         const bool odd = st & 1;
         const lds_char* cur = odd ? op1 : op0;
@@ -1710,25 +1770,18 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         wait_vm_lgkm<0, 0>();
         __builtin_amdgcn_s_barrier();
         if (st + 2 < n_st) HV_T192ISSUE(odd ? in1 : in0, st + 2);
-        // build stage st+1 (at the last stage: stage n_st-2 or 0 again, from a
-        // slot nobody has refilled, into the buffer no MFMA reads) beside the
-        // MFMAs of stage st
+        // build stage st+1 (at the last stage: an earlier stage again, from a
+        // slot nobody refills, into the buffer no MFMA reads)
         const lds_char* bin = odd ? in0 : in1;
         lds_char* bop = odd ? op0 : op1;
-        HV_T192COORDS(bin);
-        u32x4 ahi[3], alo[3];
-#define HV_T192CHUNK(C)                                                                                     \
-    t192_chunk<MODE, RPT, KIND, C>(cur, cur + boff, ra0, rb0, lh, ahi, alo, re, im, bin, bop, pos, lane, g, x, y, \
-                                   z, il)
-        HV_T192CHUNK(0);
-        HV_T192CHUNK(1);
-        HV_T192CHUNK(2);
-        HV_T192CHUNK(3);
-        HV_T192CHUNK(4);
-        HV_T192CHUNK(5);
-#undef HV_T192CHUNK
+        // both waves of a SIMD build, then both run their MFMAs (measured on
+        // MI355X: 7% faster than staggering the build and the MFMAs of the
+        // pair, whose LDS traffic then competes)
+        t192_build<MODE, NR>(bin, bop, row0, lane, ax, ay, az, il);
+        __builtin_amdgcn_sched_barrier(0);
+        HV_T192MMA(cur);
     }
-#undef HV_T192COORDS
+#undef HV_T192MMA
 #undef HV_T192ISSUE
     wait_vm_lgkm<0, 0>();
     __builtin_amdgcn_s_barrier();
@@ -1768,23 +1821,22 @@ __host__ __device__ inline int64_t t192_items(int nunits, int nsplit, int ntiles
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void herk_t192_kernel(FusedArgs a) {
+__global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     using L = T192;
     constexpr int TILE = L::TILE;
     __shared__ __attribute__((aligned(1024))) char s_in0[L::IN];
     __shared__ __attribute__((aligned(1024))) char s_in1[L::IN];
     __shared__ __attribute__((aligned(1024))) char s_op0[2 * L::OP];
     __shared__ __attribute__((aligned(1024))) char s_op1[2 * L::OP];
-    __shared__ double s_pos[3 * L::ROWS];  // antenna x[384], y[384], z[384] of the tile's rows
     lds_char* in0 = (lds_char*)s_in0;
     lds_char* in1 = (lds_char*)s_in1;
     lds_char* op0 = (lds_char*)s_op0;
     lds_char* op1 = (lds_char*)s_op1;
-    const lds_char* pos = (const lds_char*)s_pos;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int q = wave & 3;                // SIMD slot: quadrant or diagonal block list
+    const bool im = wave >= 4;             // component this wave accumulates
     const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / 16);
     const size_t amp_stage = (size_t)a.nant * 16;
     const size_t amp_time = (size_t)a.kpad * a.nant;
@@ -1806,72 +1858,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         const float* amp_t = a.amp + (size_t)(a.amp_per_unit ? unit : tb) * amp_time;
         const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
 
-        unsigned smask = 0;
+        // amplitude chunk c = (i*8 + wave)*64 + lane: row c/4, stored part
+        // c%4 holds the row's part (c%4) ^ amp_swz(row)
+        int ampoff[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int R = wr * 3 + i, Cb = wc * 3 + j;
-                const bool on = (bi * TILE + R * 32 < a.nant) && (bj * TILE + Cb * 32 < a.nant) && !(diag && R > Cb);
-                smask |= (unsigned)on << (i * 3 + j);
-            }
-        smask = __builtin_amdgcn_readfirstlane(smask);
-
-        // amplitude chunk c = (i*4 + wave)*64 + lane: row c/4, part c%4
-        int ampoff[L::NAMP];
-#pragma unroll
-        for (int i = 0; i < L::NAMP; ++i) {
-            const int c = (i * 4 + wave) * 64 + lane;
+        for (int i = 0; i < 3; ++i) {
+            const int c = (i * 8 + wave) * 64 + lane;
             const int rr = c >> 2;
             const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
             ampoff[i] = ant * 16 + ((c & 3) ^ amp_swz(rr)) * 4;
         }
         const int crdoff = min(wave, 2) * (int)a.kpad + 2 * lane;   // used by lanes 0-7 of waves 0-2
 
-        // antenna positions of the tile's rows (the previous item's builds
-        // are behind its last barrier; the split's first barrier publishes these)
-        for (int rr = threadIdx.x; rr < L::ROWS; rr += 256) {
-            const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
-            s_pos[rr] = a.antpos[3 * ant];
-            s_pos[L::ROWS + rr] = a.antpos[3 * ant + 1];
-            s_pos[2 * L::ROWS + rr] = a.antpos[3 * ant + 2];
-        }
-
-        f32x16 re[3][3], im[3][3];
-#define HV_T192SPLIT(RPT, KIND)                                                                             \
-    t192_split<MODE, RPT, KIND>(in0, in1, op0, op1, pos, amp_t, crd_t, amp_stage, s_begin, n_st, ampoff, crdoff, \
-                                lane, wave, il, re, im)
-        // a wave with any block of the triangle computes all nine (blocks
-        // outside it hold finite values nobody stores): no wave of a tile
-        // does more than nine, so this costs no time and keeps the MFMA
-        // sequence free of branches
-        if (diag) {
-            if (smask) HV_T192SPLIT(3, 2);
-            else HV_T192SPLIT(3, 0);
-        } else {
-            if (smask) HV_T192SPLIT(6, 2);
-            else HV_T192SPLIT(6, 0);
-        }
+        f32x16 acc[9];
+#define HV_T192SPLIT(IM, DQ)                                                                                 \
+    t192_split<MODE, IM, DQ>(in0, in1, op0, op1, a.antpos, a.nant, bi, bj, amp_t, crd_t, amp_stage, s_begin, n_st, \
+                             ampoff, crdoff, lane, wave, il, acc)
+#define HV_T192BOTH(DQ)              \
+    do {                             \
+        if (im) HV_T192SPLIT(true, DQ); \
+        else HV_T192SPLIT(false, DQ);   \
+    } while (0)
+        if (!diag) HV_T192BOTH(-1);
+        else if (q == 0) HV_T192BOTH(0);
+        else if (q == 1) HV_T192BOTH(1);
+        else if (q == 2) HV_T192BOTH(2);
+        else HV_T192BOTH(3);
+#undef HV_T192BOTH
 #undef HV_T192SPLIT
-        if (!smask) continue;
 
         constexpr size_t TSZ = (size_t)TILE * TILE;
         float* out = reinterpret_cast<float*>(a.partials) +
-                     (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ;
+                     (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ + (im ? TSZ : 0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 9; ++k) {
+            int br, bc;
+            if (!t192_block(diag, q, k, br, bc)) continue;
+            if (bi * TILE + br * 32 >= a.nant || bj * TILE + bc * 32 >= a.nant) continue;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                if (!(smask & (1u << (i * 3 + j)))) continue;
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const int r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-                    const int row = wr * 96 + i * 32 + r;
-                    const int col = wc * 96 + j * 32 + (lane & 31);
-                    out[(size_t)row * TILE + col] = re[i][j][e];
-                    out[TSZ + (size_t)row * TILE + col] = im[i][j][e];
-                }
+            for (int e = 0; e < 16; ++e) {
+                const int row = br * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                const int col = bc * 32 + (lane & 31);
+                out[(size_t)row * TILE + col] = acc[k][e];
             }
+        }
     }
 }
 
@@ -1922,7 +1952,7 @@ static int64_t t192_resident() {
         int dev = 0, ncu = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_t192_kernel<MODE>, 256, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_t192_kernel<MODE>, 512, 0) != hipSuccess)
             return (int64_t)0;
         return (int64_t)ncu * per_cu / 8 * 8;
     }();
@@ -1937,7 +1967,7 @@ static hipError_t launch_t192_t(const FusedArgs& a, hipStream_t s) {
     const int64_t g0 = res > 0 ? res : 256;
     const int64_t items = t192_items(a.nunits, a.nsplit, a.ntiles, t192_unit_group(a.nunits, g0));
     const int64_t grid = g0 < items ? g0 : items;
-    hipLaunchKernelGGL((herk_t192_kernel<MODE>), dim3((unsigned)grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((herk_t192_kernel<MODE>), dim3((unsigned)grid), dim3(512), 0, s, a);
     return hipGetLastError();
 }
 
