@@ -25,20 +25,31 @@ METRIC = "vis GEMM TFLOP/s & p99 job latency, split-BF16 vs FP32 at ≤1e-5 rel 
 PEAK_TFLOPS = {"split_bf16": 2500.0, "split_fp16": 2500.0, "fp32": 157.3, "fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
 # HBM traffic of one contraction launch at the bench's launch shape (workload,
-# arithmetic, units per launch), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# passes over `bench.py --kernels-only` (KiB per dispatch), scaled by what the
-# same counters read for a 1 GiB device copy: FETCH_SIZE x2.0, WRITE_SIZE x1.0.
+# arithmetic, units per launch, tile edge), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes over `bench.py --kernels-only` (KiB per dispatch), scaled
+# by what the same counters read for a 1 GiB device copy: FETCH_SIZE x2.0,
+# WRITE_SIZE x1.0.
 MEASURED_TRAFFIC = {
-    # operands built in place (herk_fused8_kernel), one launch per time
-    ("hera350", "split_bf16", 64): {"fetch_kib": 138366362, "write_kib": 4325376, "fetch_scale": 2.0,
-                                    "write_scale": 1.0},
+    # 192-tile in-place build (herk_t192_kernel), two times (128 units) per launch
+    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 6992497, "write_kib": 8668160, "fetch_scale": 2.0,
+                                          "write_scale": 1.0},
+    # 128-tile in-place build (--herk-variant 128,5), one time per launch
+    ("hera350", "split_bf16", 64, 128): {"fetch_kib": 138366362, "write_kib": 4325376, "fetch_scale": 2.0,
+                                         "write_scale": 1.0},
     # phasor build + LDS ring (--herk-variant 128,4), 8 units per launch
-    ("hera350", "split_bf16", 8): {"fetch_kib": 12448316, "write_kib": 540672, "fetch_scale": 2.0,
-                                   "write_scale": 1.0},
+    ("hera350", "split_bf16", 8, 128): {"fetch_kib": 12448316, "write_kib": 540672, "fetch_scale": 2.0,
+                                        "write_scale": 1.0},
 }
 PF_NAMES = {1: "register-staged, 1 stage ahead", 2: "register-staged, 2 stages ahead",
             3: "LDS ring, one workgroup per work item", 4: "LDS ring, persistent grid",
             5: "operands built in place from A sqrt(I), 8 waves, persistent grid"}
+
+
+def variant_name(tile, pf):
+    if tile == 192:
+        return ("operands built in place from A sqrt(I), 8 waves (Re and Im wave pair per SIMD), "
+                "persistent XCD-grouped grid")
+    return PF_NAMES[pf]
 
 
 def log(*a):
@@ -191,12 +202,14 @@ def main():
     if args.kernels_only:
         # a short, fixed sequence of dispatches for rocprofv3 counter runs:
         # 3 copies of 1 GiB (the yardstick for FETCH_SIZE / WRITE_SIZE), then
-        # one time of the workload (F units) through the production kernels
+        # the workload's first two times (2 F units: one contraction launch at
+        # the production batching of HERA-350) through the production kernels
         cp = stages.bench_copy(1 << 30, 3)
-        _, st = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[:1], gemm=args.gemm, timing=True,
+        nt = min(2, T)
+        _, st = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[:nt], gemm=args.gemm, timing=True,
                                 want_output=False, dtype=np.float32)
-        u_launch = F / st["herk_n"]
-        print(json.dumps({"copy": cp, "gemm": args.gemm, "units": F, "herk_launches": st["herk_n"],
+        u_launch = nt * F / st["herk_n"]
+        print(json.dumps({"copy": cp, "gemm": args.gemm, "units": nt * F, "herk_launches": st["herk_n"],
                           "units_per_launch": u_launch, "herk_avg_ms": st["herk_ms"] / st["herk_n"],
                           "phasor_avg_ms": st["phasor_ms"] / st["phasor_n"],
                           "reduce_avg_ms": st["reduce_ms"] / st["reduce_n"],
@@ -287,7 +300,7 @@ def main():
 
     peak = PEAK_TFLOPS[args.gemm]
     upl = units * args.steps / st["herk_n"]
-    tr = MEASURED_TRAFFIC.get((args.workload, args.gemm, int(round(upl))))
+    tr = MEASURED_TRAFFIC.get((args.workload, args.gemm, int(round(upl)), tile))
     traffic = ((tr["fetch_kib"] * tr["fetch_scale"] + tr["write_kib"] * tr["write_scale"]) * 1024
                if tr and abs(upl - round(upl)) < 1e-9 else None)
     out = {
@@ -301,7 +314,7 @@ def main():
                    "flop_count": "useful Hermitian 4 N (N+1) K per unit"},
         "roofline": {"bound": "mfma", "achieved": herk_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": herk_tflops / peak, "traffic": traffic,
-                     "kernel": f"Hermitian contraction, {args.gemm}, {tile}x{tile} tiles, {PF_NAMES[pf]}",
+                     "kernel": f"Hermitian contraction, {args.gemm}, {tile}x{tile} tiles, {variant_name(tile, pf)}",
                      "avg_launch_ms": herk_avg_ms, "launches": st["herk_n"], "units_per_launch": upl,
                      "flops_per_launch": flops_per_launch,
                      "operand_bytes_per_launch": (elem * N * K * upl) if pf != 5 else 0,
