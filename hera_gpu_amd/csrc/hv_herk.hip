@@ -1697,27 +1697,43 @@ __device__ __forceinline__ void t192_diag_mma(const lds_char* S, int lrow, int l
             t192_diag_row<MODE, IM, 3, 3, 5, 0>(S, lrow, gg, acc);
             t192_diag_row<MODE, IM, 4, 4, 5, 3>(S, lrow, gg, acc);
         }
+        // the 15 blocks of a diagonal tile with five block rows inside the
+        // array (the last tile at N = 321..352), 4, 4, 4, 3 per SIMD
+        if constexpr (Q == 4) t192_diag_row<MODE, IM, 0, 0, 3, 0>(S, lrow, gg, acc);
+        if constexpr (Q == 5) t192_diag_row<MODE, IM, 1, 1, 4, 0>(S, lrow, gg, acc);
+        if constexpr (Q == 6) {
+            t192_diag_row<MODE, IM, 2, 2, 4, 0>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 0, 4, 4, 3>(S, lrow, gg, acc);
+        }
+        if constexpr (Q == 7) {
+            t192_diag_row<MODE, IM, 3, 3, 4, 0>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 4, 4, 4, 2>(S, lrow, gg, acc);
+        }
     }
 }
 
 // Block k of a wave's accumulator set -> (block row, block column) in the
-// tile (diagonal lists as in t192_diag_mma), or false past the set.
-__device__ inline bool t192_block(bool diag, int q, int k, int& br, int& bc) {
-    if (!diag) {
-        br = (q >> 1) * 3 + k / 3;
-        bc = (q & 1) * 3 + k % 3;
-        return true;
-    }
-    switch (q) {
+// tile (list DQ of t192_diag_mma; DQ < 0: quadrant q), or false past the set.
+__device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
+    switch (dq) {
         case 0: br = 0; bc = k; return k < 6;
         case 1: br = 1; bc = 1 + k; return k < 5;
         case 2: br = k < 4 ? 2 : 5; bc = k < 4 ? 2 + k : 5; return k < 5;
-        default: br = k < 3 ? 3 : 4; bc = k < 3 ? 3 + k : 4 + (k - 3); return k < 5;
+        case 3: br = k < 3 ? 3 : 4; bc = k < 3 ? 3 + k : 4 + (k - 3); return k < 5;
+        case 4: br = 0; bc = k; return k < 4;
+        case 5: br = 1; bc = 1 + k; return k < 4;
+        case 6: br = k < 3 ? 2 : 0; bc = k < 3 ? 2 + k : 4; return k < 4;
+        case 7: br = k < 2 ? 3 : 4; bc = k < 2 ? 3 + k : 4; return k < 3;
+        default:
+            br = (q >> 1) * 3 + k / 3;
+            bc = (q & 1) * 3 + k % 3;
+            return true;
     }
 }
 
 // One split of one tile for one wave.  DQ < 0: off-diagonal tile (quadrant
-// q); DQ = 0..3: diagonal tile, block list DQ.
+// q); DQ = 0..3: diagonal tile, block list DQ (21 blocks); DQ = 4..7:
+// diagonal tile with at most five block rows inside the array (15 blocks).
 template <int MODE, bool IM, int DQ>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
                                            const double* antpos, int nant, int bi, int bj, const float* amp_t,
@@ -1879,11 +1895,19 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         if (im) HV_T192SPLIT(true, DQ); \
         else HV_T192SPLIT(false, DQ);   \
     } while (0)
-        if (!diag) HV_T192BOTH(-1);
-        else if (q == 0) HV_T192BOTH(0);
-        else if (q == 1) HV_T192BOTH(1);
-        else if (q == 2) HV_T192BOTH(2);
-        else HV_T192BOTH(3);
+        // diagonal tiles: the 15-block lists when at most five 32-row blocks
+        // of the tile hold antennas
+        const bool five = diag && a.nant - bi * TILE <= 160;
+        const int dq = !diag ? -1 : (five ? 4 : 0) + q;
+        if (dq < 0) HV_T192BOTH(-1);
+        else if (dq == 0) HV_T192BOTH(0);
+        else if (dq == 1) HV_T192BOTH(1);
+        else if (dq == 2) HV_T192BOTH(2);
+        else if (dq == 3) HV_T192BOTH(3);
+        else if (dq == 4) HV_T192BOTH(4);
+        else if (dq == 5) HV_T192BOTH(5);
+        else if (dq == 6) HV_T192BOTH(6);
+        else HV_T192BOTH(7);
 #undef HV_T192BOTH
 #undef HV_T192SPLIT
 
@@ -1893,7 +1917,7 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             int br, bc;
-            if (!t192_block(diag, q, k, br, bc)) continue;
+            if (!t192_block(dq, q, k, br, bc)) continue;
             if (bi * TILE + br * 32 >= a.nant || bj * TILE + bc * 32 >= a.nant) continue;
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
