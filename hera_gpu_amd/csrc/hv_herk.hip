@@ -1712,6 +1712,43 @@ __device__ __forceinline__ void t192_diag_mma(const lds_char* S, int lrow, int l
     }
 }
 
+// Off-diagonal tiles whose B block has at most five 32-column blocks inside
+// the array (N = 321..352 at the last block): block rows R0..R1 x columns
+// C0..C1 per SIMD (8, 8, 8, 6 blocks instead of 9, 9, 9, 9 with 6 wasted),
+// B fragments held, A streamed.
+template <int MODE, bool IM, int R0, int R1, int C0, int C1>
+__device__ __forceinline__ void t192_rect_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
+                                              f32x16 (&acc)[9]) {
+    constexpr int NC = C1 - C0 + 1;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int gg = 2 * s + lh;
+        u32x4 bhi[NC], blo[NC];
+        u32x4 ahi, alo;
+        t192_frag<false>(As, 32 * R0 + lrow, gg, ahi, alo);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            t192_frag<IM>(Bs, 32 * (C0 + c) + lrow, gg, bhi[c], blo[c]);
+            t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[c]);
+        }
+#pragma unroll
+        for (int r = R0 + 1; r <= R1; ++r) {
+            t192_frag<false>(As, 32 * r + lrow, gg, ahi, alo);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[(r - R0) * NC + c]);
+        }
+    }
+}
+
+template <int MODE, bool IM, int Q>
+__device__ __forceinline__ void t192_rect5_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
+                                               f32x16 (&acc)[9]) {
+    if constexpr (Q == 0) t192_rect_mma<MODE, IM, 0, 1, 0, 3>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 1) t192_rect_mma<MODE, IM, 2, 3, 0, 3>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 2) t192_rect_mma<MODE, IM, 4, 5, 0, 3>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 3) t192_rect_mma<MODE, IM, 0, 5, 4, 4>(As, Bs, lrow, lh, acc);
+}
+
 // Block k of a wave's accumulator set -> (block row, block column) in the
 // tile (list DQ of t192_diag_mma; DQ < 0: quadrant q), or false past the set.
 __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
@@ -1724,6 +1761,8 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
         case 5: br = 1; bc = 1 + k; return k < 4;
         case 6: br = k < 3 ? 2 : 0; bc = k < 3 ? 2 + k : 4; return k < 4;
         case 7: br = k < 2 ? 3 : 4; bc = k < 2 ? 3 + k : 4; return k < 3;
+        case 8: case 9: case 10: br = 2 * (dq - 8) + k / 4; bc = k % 4; return k < 8;
+        case 11: br = k; bc = 4; return k < 6;
         default:
             br = (q >> 1) * 3 + k / 3;
             bc = (q & 1) * 3 + k % 3;
@@ -1733,7 +1772,9 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
 
 // One split of one tile for one wave.  DQ < 0: off-diagonal tile (quadrant
 // q); DQ = 0..3: diagonal tile, block list DQ (21 blocks); DQ = 4..7:
-// diagonal tile with at most five block rows inside the array (15 blocks).
+// diagonal tile with at most five block rows inside the array (15 blocks);
+// DQ = 8..11: off-diagonal tile with at most five B column blocks inside
+// the array (30 blocks).
 template <int MODE, bool IM, int DQ>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
                                            const double* antpos, int nant, int bi, int bj, const float* amp_t,
@@ -1741,7 +1782,8 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
                                            const int* ampoff, int crdoff, int lane, int wave, double il,
                                            f32x16 (&acc)[9]) {
     using L = T192;
-    constexpr bool DIAG = DQ >= 0;
+    constexpr bool DIAG = DQ >= 0 && DQ < 8;
+    constexpr bool RECT5 = DQ >= 8;
     constexpr int NROW = DIAG ? L::TILE : L::ROWS;
     using RW = T192Rows<DIAG, IM>;
     constexpr int NR = RW::NR;
@@ -1769,6 +1811,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
 #define HV_T192MMA(buf)                                                                   \
     do {                                                                                   \
         if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? DQ : 0)>(buf, lrow, lh, acc);  \
+        else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? DQ - 8 : 0)>(buf, buf + L::OP, lrow, lh, acc); \
         else t192_quad_mma<MODE, IM>(buf, buf + L::OP, ra0, rb0, lh, acc);                 \
     } while (0)
 
@@ -1897,8 +1940,10 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     } while (0)
         // diagonal tiles: the 15-block lists when at most five 32-row blocks
         // of the tile hold antennas
-        const bool five = diag && a.nant - bi * TILE <= 160;
-        const int dq = !diag ? -1 : (five ? 4 : 0) + q;
+        // off-diagonal tiles: the 30-block lists when at most five 32-column
+        // blocks of the B block hold antennas
+        const bool five = a.nant - bj * TILE <= 160;
+        const int dq = diag ? (five ? 4 : 0) + q : (five ? 8 + q : -1);
         if (dq < 0) HV_T192BOTH(-1);
         else if (dq == 0) HV_T192BOTH(0);
         else if (dq == 1) HV_T192BOTH(1);
@@ -1907,7 +1952,11 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         else if (dq == 4) HV_T192BOTH(4);
         else if (dq == 5) HV_T192BOTH(5);
         else if (dq == 6) HV_T192BOTH(6);
-        else HV_T192BOTH(7);
+        else if (dq == 7) HV_T192BOTH(7);
+        else if (dq == 8) HV_T192BOTH(8);
+        else if (dq == 9) HV_T192BOTH(9);
+        else if (dq == 10) HV_T192BOTH(10);
+        else HV_T192BOTH(11);
 #undef HV_T192BOTH
 #undef HV_T192SPLIT
 
