@@ -4,7 +4,7 @@
 Importing the package does not touch the GPU; the HIP library is loaded on
 the first call.
 """
-from .vis import baseline_pairs, vis_gpu, vis_gpu_baselines  # noqa: F401
+from .vis import baseline_pairs, vis_gpu, vis_gpu_baselines, vis_gpu_polarized  # noqa: F401
 
-__all__ = ["vis_gpu", "vis_gpu_baselines", "baseline_pairs"]
+__all__ = ["vis_gpu", "vis_gpu_baselines", "vis_gpu_polarized", "baseline_pairs"]
 __version__ = "0.1.0"

@@ -107,8 +107,10 @@ struct Plan {
     size_t footprint() const { return crd_top_bytes + w_bytes + part_bytes + out_bytes; }
 };
 
+// operands: build the operands in HBM (phasor build + LDS-ring contraction)
+// even where the built-in choice builds them inside the contraction.
 Plan plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t operand_budget, int layout = 0,
-              bool beam_per_freq = false);
+              bool beam_per_freq = false, bool operands = false);
 
 struct JobDev {
     int mode;

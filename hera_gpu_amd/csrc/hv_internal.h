@@ -107,14 +107,35 @@ struct HerkArgs {
     const int64_t* dyn_kpad;
 };
 
-// Output layouts of the visibilities (HV_LAYOUT_* in heravis.h).
-enum Layout { LAYOUT_FULL = 0, LAYOUT_FTB = 1, LAYOUT_TBF = 2 };
+// Output layouts of the visibilities (HV_LAYOUT_* in heravis.h).  LAYOUT_POL:
+// a polarised job's (2N) x (2N) correlation matrix (row p N + a: feed p of
+// antenna a) written as [F][2][2][T][N][N] (feed p, feed q, time, a, b).
+enum Layout { LAYOUT_FULL = 0, LAYOUT_FTB = 1, LAYOUT_TBF = 2, LAYOUT_POL = 3 };
 // Complex elements of an output: full N x N per unit, or the N (N+1) / 2
 // baselines a <= b (index a N - a (a-1) / 2 + (b - a)).
 inline size_t out_elems(int layout, int nfreq, int ntimes, int nant) {
     const size_t per = (layout == LAYOUT_FULL) ? (size_t)nant * nant : (size_t)nant * (nant + 1) / 2;
     return (size_t)nfreq * ntimes * per;
 }
+
+// Polarised phasor build: rows (feed p, antenna a) = p N + a of 2N, columns
+// (source s, sky axis k) = 2 s + k of 2 Kpad; operand
+//   v[p N + a][2 s + k] = J_a^{pk}(s) sqrt(I_s) exp(+i 2 pi nu d_a(s) / c)
+// with the complex Jones beam J_a^{pk} interpolated bilinearly (real and
+// imaginary parts with the weights of the scalar beam).
+struct PhasorPolArgs {
+    const double* crd_top;   // [3][kpad] (real sources) of the launch's first time; tb at + tb*3*kpad
+    const double* isqrt;     // [kpad]
+    const void* jones;       // [N][2 feeds][2 axes][P][P] complex (re, im), float or double (in_f64)
+    const double* antpos;    // [N][3]
+    const double* inv_lambda;// [nfreq] nu/c of the launch's frequencies
+    void* W;                 // operands of unit 0 (2N rows, 2 kpad columns, stage-major)
+    size_t unit_stride;
+    int64_t kpad;            // real sources (padded)
+    int nant;                // real antennas
+    int beam_px, nfreq, ntime, in_f64;
+    float amp_scale;
+};
 
 struct ReduceArgs {
     const void* partials;
@@ -134,6 +155,7 @@ struct ReduceArgs {
     int layout;              // Layout
     int nfreq_total;         // F of the output array
 };
+hipError_t launch_phasor_pol(int mode, const PhasorPolArgs& a, hipStream_t s);
 
 struct BeamArgs {
     const void* beam;

@@ -224,14 +224,16 @@ static size_t operand_budget(hv_ctx* c) {
 }
 
 Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t wbudget, int layout,
-                  bool beam_per_freq) {
+                  bool beam_per_freq, bool operands) {
     Plan p;
     p.mode = mode;
     p.nant = nant;
     p.ntimes = ntimes;
     p.nfreq = nfreq;
     p.kpad = kpad;
+    p.fused = herk_fused(mode, nant) && !operands;
     p.tile = herk_tile(mode, nant);
+    if (!p.fused && p.tile == 192) p.tile = 128;   // 192-tiles exist only as in-place builds
     p.nb = (nant + p.tile - 1) / p.tile;
     p.ntiles = p.nb * (p.nb + 1) / 2;
     p.nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
@@ -239,7 +241,6 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.unit_bytes = (size_t)nant * p.row_bytes;
     const size_t acc_bytes = mode_is_f64(mode) ? 8 : 4;
     p.part_unit = (size_t)p.nsplit * p.ntiles * 2 * p.tile * p.tile * acc_bytes;
-    p.fused = herk_fused(mode, nant);
     // batch: all frequencies of a time if they fit, then as many times as fit.
     // Operands per unit bound a batch, or, when the contraction builds them,
     // the partial tiles per unit and A sqrt(I) per time (per unit with a beam
@@ -587,6 +588,108 @@ extern "C" int hv_vis_ex(hv_ctx* c, int mode, int dtype, const void* antpos, int
                       : 1.0f;
     StageTimer tm;
     HV_CHECK(run_job(c, j, tm));
+    HV_HIP(hipMemcpyAsync(vis_out, j.vis, out_bytes, hipMemcpyDeviceToHost, c->stream));
+    HV_HIP(hipStreamSynchronize(c->stream));
+    return HV_OK;
+}
+
+// --------------------------------------------------------- polarised beams
+// A polarised job is the Hermitian contraction of 2N rows (feed p of antenna
+// a: row p N + a) over 2 Kpad columns (sky axis k of source s: column
+// 2 s + k); operands from the polarised phasor build, LDS-ring contraction,
+// reduce into [F][2][2][T][N][N].
+static int run_pol_job(hv_ctx* c, const JobDev& j, const void* jones, StageTimer& tm) {
+    const int nv = 2 * j.nant;
+    const int64_t kv = 2 * j.kpad;
+    Plan p = plan_job(j.mode, nv, kv, j.ntimes, j.nfreq, operand_budget(c), LAYOUT_POL, false, true);
+    HV_CHECK(prepare_workspace(c, c->ws, p, c->stream));
+    hipStream_t s = c->stream;
+    double* crd_top = static_cast<double*>(c->ws.buf[B_CRD_TOP].p);   // [T][3][kpad] of the real sources
+    CoordsArgs ca{j.eq2tops, j.crd_eq, crd_top, j.kpad, j.ntimes, nullptr};
+    HV_CHECK(tm.begin(0, s));
+    HV_HIP(launch_coords(ca, s));
+    HV_CHECK(tm.end(0, s));
+    JobDev jv = j;                  // the contraction and reduce see the 2N x 2Kpad problem
+    jv.nant = nv;
+    jv.kpad = kv;
+    jv.layout = LAYOUT_POL;
+    for (int t0 = 0; t0 < j.ntimes; t0 += p.tb) {
+        const int nt = std::min(p.tb, j.ntimes - t0);
+        for (int f0 = 0; f0 < j.nfreq; f0 += p.fb) {
+            const int nf = std::min(p.fb, j.nfreq - f0);
+            PhasorPolArgs pa{};
+            pa.crd_top = crd_top + (size_t)t0 * 3 * j.kpad;
+            pa.isqrt = j.isqrt;
+            pa.jones = jones;
+            pa.antpos = j.antpos;
+            pa.inv_lambda = j.inv_lambda + f0;
+            pa.W = c->ws.buf[B_W].p;
+            pa.unit_stride = p.unit_bytes;
+            pa.kpad = j.kpad;
+            pa.nant = j.nant;
+            pa.beam_px = j.beam_px;
+            pa.nfreq = nf;
+            pa.ntime = nt;
+            pa.in_f64 = j.beam_f64;
+            pa.amp_scale = j.amp_scale;
+            HV_CHECK(tm.begin(1, s));
+            HV_HIP(launch_phasor_pol(j.mode, pa, s));
+            HV_CHECK(tm.end(1, s));
+            HV_CHECK(enqueue_contract(jv, p, c->ws.buf[B_W].p, c->ws.buf[B_PART].p, t0, nt, f0, nf, s, tm));
+        }
+    }
+    return HV_OK;
+}
+
+extern "C" int hv_vis_pol(hv_ctx* c, int mode, int dtype, const void* antpos, int nant, const double* freqs,
+                          int nfreq, const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt,
+                          int64_t nsrc, const void* jones, int beam_px, void* vis_out) {
+    if (!c) return fail(HV_ERR_ARG, "ctx is NULL");
+    HV_CHECK(check_job_args(mode, dtype, nant, nfreq, ntimes, nsrc, beam_px));
+    if (beam_px > (dtype == HV_F64 ? 71 : 101))
+        return fail(HV_ERR_ARG, "polarised beams: the four planes of a feed must fit in LDS (beam_px <= 101, "
+                                "71 for float64)");
+    if (!antpos || !freqs || !eq2tops || !jones || !vis_out || (nsrc > 0 && (!crd_eq || !isqrt)))
+        return fail(HV_ERR_ARG, "NULL array argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    HV_CHECK(set_device(c));
+    const int64_t kpad = round_up(std::max<int64_t>(nsrc, 1), KALIGN);
+    std::vector<double> ap, eq, il, crd, isq;
+    to_f64(antpos, dtype, (size_t)nant * 3, ap);
+    to_f64(eq2tops, dtype, (size_t)ntimes * 9, eq);
+    inv_lambdas(freqs, nfreq, il);
+    pad_sky(crd_eq, isqrt, dtype, nsrc, kpad, crd, isq);
+    const size_t nreal = (size_t)nant * 4 * beam_px * beam_px * 2;   // re, im of [N][2][2][P][P]
+    const size_t jones_bytes = nreal * (dtype == HV_F64 ? 8 : 4);
+    HV_CHECK(upload(c, B_ANTPOS, ap.data(), ap.size() * 8));
+    HV_CHECK(upload(c, B_EQ2TOPS, eq.data(), eq.size() * 8));
+    HV_CHECK(upload(c, B_INVLAM, il.data(), il.size() * 8));
+    HV_CHECK(upload(c, B_CRD_EQ, crd.data(), crd.size() * 8));
+    HV_CHECK(upload(c, B_ISQRT, isq.data(), isq.size() * 8));
+    HV_CHECK(upload(c, B_BEAM, jones, jones_bytes));
+    const size_t cplx = mode_is_f64(mode) ? 16 : 8;
+    const size_t out_bytes = out_elems(LAYOUT_FULL, nfreq, ntimes, 2 * nant) * cplx;
+    HV_CHECK(ensure(c, B_OUT, out_bytes));
+
+    JobDev j{};
+    j.layout = LAYOUT_POL;
+    j.mode = mode;
+    j.nant = nant; j.ntimes = ntimes; j.nfreq = nfreq; j.beam_px = beam_px; j.beam_f64 = dtype == HV_F64;
+    j.kpad = kpad;
+    j.antpos = (const double*)c->ws.buf[B_ANTPOS].p;
+    j.inv_lambda = (const double*)c->ws.buf[B_INVLAM].p;
+    j.eq2tops = (const double*)c->ws.buf[B_EQ2TOPS].p;
+    j.crd_eq = (const double*)c->ws.buf[B_CRD_EQ].p;
+    j.isqrt = (const double*)c->ws.buf[B_ISQRT].p;
+    j.beam = c->ws.buf[B_BEAM].p;
+    j.vis = c->ws.buf[B_OUT].p;
+    // |J| <= sqrt(2) max |Re, Im| bounds the operand magnitude for the FP16 scale
+    j.amp_scale = (mode == HV_MODE_SPLIT_FP16)
+                      ? fp16_scale(1.4142135623730951 * max_abs(jones, dtype, nreal) *
+                                   max_abs(isq.data(), HV_F64, isq.size()))
+                      : 1.0f;
+    StageTimer tm;
+    HV_CHECK(run_pol_job(c, j, j.beam, tm));
     HV_HIP(hipMemcpyAsync(vis_out, j.vis, out_bytes, hipMemcpyDeviceToHost, c->stream));
     HV_HIP(hipStreamSynchronize(c->stream));
     return HV_OK;

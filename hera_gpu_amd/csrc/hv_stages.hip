@@ -215,6 +215,164 @@ hipError_t launch_phasor(int mode, const PhasorArgs& a, hipStream_t s) {
     return hipErrorInvalidValue;
 }
 
+// ------------------------------------------------- polarised phasor build
+// Grid (virtual row p N + a, chunk of 4096 real sources, time).  The four
+// real planes (Re, Im of J^{p0} and J^{p1}) of antenna a, feed p go to LDS;
+// each thread takes 4 consecutive sources (8 consecutive columns 2 s + k):
+// one bilinear weight set per source for the four planes, the FP64 path
+// difference once, then per frequency the phase in the arithmetic of the
+// mode (whole turns with hardware sine/cosine for the split modes, quarter
+// turns otherwise) and the complex product J sqrt(I) exp(i phase).
+constexpr int PP_SRC_PER_WG = 4096;
+
+template <typename BT>
+__device__ inline void beam_bilinear4(const BT* __restrict__ b, int P, double x, double y, float (&out)[4]) {
+    const double bx = (double)(P - 1) * (0.5 * x + 0.5);
+    const double by = (double)(P - 1) * (0.5 * y + 0.5);
+    const double px = floor(bx), py = floor(by);
+    const BT fx = (BT)(bx - px), fy = (BT)(by - py);
+    const double pm = (double)(P - 1);
+    const int ix0 = (int)clampr(px, 0.0, pm), ix1 = (int)clampr(px + 1.0, 0.0, pm);
+    const int iy0 = (int)clampr(py, 0.0, pm), iy1 = (int)clampr(py + 1.0, 0.0, pm);
+    const size_t pl = (size_t)P * P;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const BT* q = b + c * pl;
+        const BT v00 = q[iy0 * P + ix0], v01 = q[iy0 * P + ix1];
+        const BT v10 = q[iy1 * P + ix0], v11 = q[iy1 * P + ix1];
+        const BT top = fma(fx, v01 - v00, v00);
+        const BT bot = fma(fx, v11 - v10, v10);
+        out[c] = (float)fma(fy, bot - top, top);
+    }
+}
+template <typename BT>
+__device__ inline void beam_bilinear4(const BT* __restrict__ b, int P, double x, double y, double (&out)[4]) {
+    const double bx = (double)(P - 1) * (0.5 * x + 0.5);
+    const double by = (double)(P - 1) * (0.5 * y + 0.5);
+    const double px = floor(bx), py = floor(by);
+    const BT fx = (BT)(bx - px), fy = (BT)(by - py);
+    const double pm = (double)(P - 1);
+    const int ix0 = (int)clampr(px, 0.0, pm), ix1 = (int)clampr(px + 1.0, 0.0, pm);
+    const int iy0 = (int)clampr(py, 0.0, pm), iy1 = (int)clampr(py + 1.0, 0.0, pm);
+    const size_t pl = (size_t)P * P;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const BT* q = b + c * pl;
+        const BT v00 = q[iy0 * P + ix0], v01 = q[iy0 * P + ix1];
+        const BT v10 = q[iy1 * P + ix0], v11 = q[iy1 * P + ix1];
+        const BT top = fma(fx, v01 - v00, v00);
+        const BT bot = fma(fx, v11 - v10, v10);
+        out[c] = (double)fma(fy, bot - top, top);
+    }
+}
+
+template <int MODE, typename R>
+__device__ inline void phase_sincos(double d, double il, R* s, R* c) {
+    if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
+        sincos_turns_hw(d * il, s, c);
+    } else {
+        sincos_quarter_turns(d * (4.0 * il), s, c);
+    }
+}
+
+template <int MODE, typename BT>
+__global__ __launch_bounds__(PH_THREADS) void phasor_pol_kernel(PhasorPolArgs a) {
+    using R = typename std::conditional<MODE == MODE_FP64, double, float>::type;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    BT* sb = reinterpret_cast<BT*>(smem);                 // [4][P][P]: Re J0, Im J0, Re J1, Im J1
+    const int vrow = blockIdx.x;                          // p N + a
+    const int p = vrow / a.nant, ant = vrow - p * a.nant;
+    const int tb = blockIdx.z;
+    const int P = a.beam_px;
+    const size_t pl = (size_t)P * P;
+    const BT* gj = reinterpret_cast<const BT*>(a.jones) + ((size_t)ant * 4 + (size_t)p * 2) * pl * 2;
+    for (size_t i = threadIdx.x; i < 2 * pl; i += PH_THREADS) {
+        const size_t k = i / pl, e = i - k * pl;          // sky axis k, element e
+        sb[(2 * k) * pl + e] = gj[(k * pl + e) * 2];
+        sb[(2 * k + 1) * pl + e] = gj[(k * pl + e) * 2 + 1];
+    }
+    __syncthreads();
+
+    const double ax = a.antpos[3 * ant], ay = a.antpos[3 * ant + 1], az = a.antpos[3 * ant + 2];
+    const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
+    const double* cy = cx + a.kpad;
+    const double* cz = cx + 2 * a.kpad;
+    const int nrow = 2 * a.nant;
+    const int64_t kv = 2 * a.kpad;                        // columns
+    constexpr int SST = stage_src(MODE == MODE_FP64);
+    constexpr size_t elem = (MODE == MODE_FP64) ? 16 : 8;
+    char* wunit = reinterpret_cast<char*>(a.W) + (size_t)tb * a.nfreq * a.unit_stride;
+    const int64_t s_lo = (int64_t)blockIdx.y * PP_SRC_PER_WG;
+    const int64_t s_hi = min(s_lo + (int64_t)PP_SRC_PER_WG, a.kpad);
+    for (int64_t s0 = s_lo + (int64_t)threadIdx.x * 4; s0 < s_hi; s0 += PH_THREADS * 4) {
+        R jr[4][2], ji[4][2], amp[4];
+        double d[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t s = s0 + j;
+            const double x = cx[s], y = cy[s], z = cz[s];
+            R bj[4] = {0, 0, 0, 0};
+            if (z > 0.0) beam_bilinear4<BT>(sb, P, x, y, bj);
+            jr[j][0] = bj[0]; ji[j][0] = bj[1]; jr[j][1] = bj[2]; ji[j][1] = bj[3];
+            amp[j] = (R)a.isqrt[s] * (R)a.amp_scale;
+            d[j] = fma(ax, x, fma(ay, y, az * z));
+        }
+        const int64_t col0 = 2 * s0;                      // 8 columns, all in one stage (SST >= 8)
+        for (int u = 0; u < a.nfreq; ++u) {
+            const double il = a.inv_lambda[u];
+            R re[8], im[8];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                R sn, cs;
+                phase_sincos<MODE, R>(d[j], il, &sn, &cs);
+                const R er = amp[j] * cs, ei = amp[j] * sn;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    re[2 * j + k] = jr[j][k] * er - ji[j][k] * ei;
+                    im[2 * j + k] = jr[j][k] * ei + ji[j][k] * er;
+                }
+            }
+            store_phasors8<MODE, R>(wunit + (size_t)u * a.unit_stride + operand_offset(col0 / SST, nrow, vrow) +
+                                        (size_t)(col0 % SST) * elem,
+                                    re, im);
+        }
+    }
+    (void)kv;
+}
+
+template <int MODE, typename BT>
+static hipError_t launch_phasor_pol_t(const PhasorPolArgs& a, hipStream_t s) {
+    const size_t lds = (size_t)4 * a.beam_px * a.beam_px * sizeof(BT);
+    const int64_t nchunk = (a.kpad + PP_SRC_PER_WG - 1) / PP_SRC_PER_WG;
+    if (nchunk > 65535 || a.ntime > 65535 || a.kpad % 4) return hipErrorInvalidConfiguration;
+    if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
+    dim3 grid((unsigned)(2 * a.nant), (unsigned)nchunk, (unsigned)a.ntime);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)phasor_pol_kernel<MODE, BT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((phasor_pol_kernel<MODE, BT>), grid, dim3(PH_THREADS), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_phasor_pol(int mode, const PhasorPolArgs& a, hipStream_t s) {
+    const bool f64 = a.in_f64 != 0;
+    switch (mode) {
+        case MODE_FP32:
+            return f64 ? launch_phasor_pol_t<MODE_FP32, double>(a, s) : launch_phasor_pol_t<MODE_FP32, float>(a, s);
+        case MODE_SPLIT_BF16:
+            return f64 ? launch_phasor_pol_t<MODE_SPLIT_BF16, double>(a, s)
+                       : launch_phasor_pol_t<MODE_SPLIT_BF16, float>(a, s);
+        case MODE_SPLIT_FP16:
+            return f64 ? launch_phasor_pol_t<MODE_SPLIT_FP16, double>(a, s)
+                       : launch_phasor_pol_t<MODE_SPLIT_FP16, float>(a, s);
+        case MODE_FP64:
+            return f64 ? launch_phasor_pol_t<MODE_FP64, double>(a, s) : launch_phasor_pol_t<MODE_FP64, float>(a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 // ------------------------------------------- beam amplitude per time
 // A sqrt(I) (times the FP16 operand scale) per (antenna, source, time) in
 // FP32, for the contraction that builds its operands in place: the same
@@ -355,7 +513,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
     const int a = blockIdx.y * 8 + (threadIdx.x >> 5);
     const int u = blockIdx.z;
     if (a >= r.nant || b >= r.nant) return;
-    if (r.layout != LAYOUT_FULL && a > b) return;        // baseline layouts hold a <= b only
+    if ((r.layout == LAYOUT_FTB || r.layout == LAYOUT_TBF) && a > b) return;   // baseline layouts: a <= b only
     const int T = r.tile;
     const int ta = a / T, tb = b / T;
     const bool mirror = (ta > tb) || (ta == tb && a > b);
@@ -377,6 +535,10 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
     int64_t idx;
     if (r.layout == LAYOUT_FULL) {
         idx = (f * r.ntimes_total + t) * (int64_t)r.nant * r.nant + (int64_t)a * r.nant + b;
+    } else if (r.layout == LAYOUT_POL) {
+        const int n = r.nant / 2;                         // antennas per feed
+        const int p = a / n, q = b / n;
+        idx = (((f * 2 + p) * 2 + q) * r.ntimes_total + t) * (int64_t)n * n + (int64_t)(a - p * n) * n + (b - q * n);
     } else {
         const int64_t nbl = (int64_t)r.nant * (r.nant + 1) / 2;
         const int64_t bl = (int64_t)a * r.nant - (int64_t)a * (a - 1) / 2 + (b - a);

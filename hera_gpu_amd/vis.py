@@ -101,6 +101,54 @@ def vis_gpu_baselines(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube, precision=
     return out, ant1, ant2
 
 
+def vis_gpu_polarized(antpos, freq, eq2tops, crd_eq, I_sky, jones, precision=1, *, real_dtype=None,
+                      complex_dtype=None, gemm=None, device=0):
+    """Visibilities of every feed pair for complex (Jones) beams.
+
+    ``jones``: complex (nant, 2, 2, P, P) = [antenna, feed p, sky axis k, y, x]
+    (a real array is taken as its complex value).  For an unpolarised sky,
+    V[p, q, t, a, b] = sum_s sum_k J_a^{pk}(s) conj(J_b^{qk}(s)) I(s)
+    exp(+2 pi i nu (antpos_a - antpos_b) . crd_top(s) / c), J interpolated
+    bilinearly per real and imaginary part (zero below the horizon).  Returns
+    (2, 2, ntimes, nant, nant), or (nfreq, 2, 2, ntimes, nant, nant) for an
+    array of frequencies.  With J_a^{pk} = A_a delta_pk the p = q blocks are
+    ``vis_gpu``'s result (to the arithmetic's rounding) and p != q are zero.
+    The reference's beam is a real scalar (vis.py:162-175, "XXX complex?" at
+    vis.py:175); this extends it.
+    """
+    real_dtype, complex_dtype, mode, dtype = _resolve(precision, real_dtype, complex_dtype, gemm)
+    antpos = np.asarray(antpos)
+    eq2tops = np.asarray(eq2tops)
+    crd_eq = np.asarray(crd_eq)
+    I_sky = np.asarray(I_sky)
+    jones = np.asarray(jones)
+    nant = antpos.shape[0]
+    assert antpos.shape == (nant, 3)
+    assert crd_eq.ndim == 2 and crd_eq.shape[0] == 3
+    npix = crd_eq.shape[1]
+    assert I_sky.shape == (npix,)
+    assert eq2tops.ndim == 3 and eq2tops.shape[1:] == (3, 3)
+    ntimes = eq2tops.shape[0]
+    scalar_freq = np.ndim(freq) == 0
+    freqs = np.ascontiguousarray(np.atleast_1d(np.asarray(freq, dtype=np.float64)))
+    assert freqs.ndim == 1 and freqs.size >= 1
+    assert jones.ndim == 5
+    beam_px = jones.shape[-1]
+    assert jones.shape == (nant, 2, 2, beam_px, beam_px)
+    antpos = np.ascontiguousarray(antpos, dtype=real_dtype)
+    eq2tops = np.ascontiguousarray(eq2tops, dtype=real_dtype)
+    crd_eq = np.ascontiguousarray(crd_eq, dtype=real_dtype)
+    isqrt = np.ascontiguousarray(np.sqrt(I_sky).astype(real_dtype))
+    jones = np.ascontiguousarray(jones, dtype=np.complex128 if real_dtype == np.float64 else np.complex64)
+    out = np.empty((freqs.size, 2, 2, ntimes, nant, nant), dtype=complex_dtype)
+    ctx = _lib.context(device)
+    rc = _lib.load().hv_vis_pol(ctx.handle, mode, dtype, _lib.ptr(antpos), nant, _lib.ptr(freqs), freqs.size,
+                                _lib.ptr(eq2tops), ntimes, _lib.ptr(crd_eq), _lib.ptr(isqrt), npix,
+                                _lib.ptr(jones), beam_px, _lib.ptr(out))
+    _lib.check(rc, "hv_vis_pol")
+    return out[0] if scalar_freq else out
+
+
 def _vis(antpos, freq, eq2tops, crd_eq, I_sky, bm_cube, precision, real_dtype, complex_dtype, gemm, device,
          layout):
     real_dtype, complex_dtype, mode, dtype = _resolve(precision, real_dtype, complex_dtype, gemm)

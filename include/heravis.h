@@ -175,6 +175,24 @@ int hv_simulate(hv_ctx* ctx, int mode, const hv_sky* sky, int64_t nsrc_used,
                 void* vis_out, int reps, hv_stage_times* times);
 
 /* ---------------------------------------------------------------------------
+ * Polarised (Jones) beams.  No counterpart in the reference, whose beam is a
+ * real scalar per antenna (vis.py:162-175, "XXX complex?" at vis.py:175).
+ * jones: complex [nant][2 feeds p][2 sky axes k][beam_px][beam_px], (re, im)
+ * interleaved, float32 (dtype HV_F32) or float64; the other arrays as for
+ * hv_vis.  For an unpolarised sky of intensity I_s:
+ *   V_ab^{pq} = sum_s sum_k J_a^{pk}(s) conj(J_b^{qk}(s)) I_s exp(i 2 pi nu (tau_a - tau_b))
+ * with J interpolated bilinearly per real and imaginary part (the scalar
+ * beam's rule; zero below the horizon).  vis_out: complex [nfreq][2][2]
+ * [ntimes][nant][nant] (feed p, feed q, time, a, b).  J_a^{pk} = A_a
+ * delta_pk gives the scalar result on p = q and zeros on p != q.  Runs the
+ * phasor build + LDS-ring contraction on 2 nant rows and 2 nsrc columns;
+ * beam_px <= 101 (71 for float64).
+ * ------------------------------------------------------------------------- */
+int hv_vis_pol(hv_ctx* ctx, int mode, int dtype, const void* antpos, int nant, const double* freqs, int nfreq,
+               const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt, int64_t nsrc,
+               const void* jones, int beam_px, void* vis_out);
+
+/* ---------------------------------------------------------------------------
  * Calibration (no reference counterpart): a plain device copy of `bytes`
  * (ms per copy; HBM bandwidth = 2*bytes/time, and the yardstick for
  * rocprofv3 byte counters) and register-resident MFMA loops of each

@@ -146,6 +146,38 @@ def vis_oracle_units(antpos, freqs, eq2tops, crd_eq, I_sky, bm_cube, units):
     return out
 
 
+def vis_oracle_pol(antpos, freqs, eq2tops, crd_eq, I_sky, jones) -> np.ndarray:
+    """FP64 polarised visibilities: complex128 (F, 2, 2, T, N, N).
+
+    ``jones``: complex (N, 2, 2, P, P) = [antenna, feed p, sky axis k, y, x].
+    V[f, p, q, t] = sum_k (J^{pk} sqrt(I) e^{i phase}) @ (J^{qk} sqrt(I) e^{i phase})^H
+    with J^{pk} at each source from the bilinear rule of ``beam_amplitude``
+    applied to the real and imaginary parts (the reference's beam is the
+    real, one-component case, vis.py:46-89; "XXX complex?" at vis.py:175).
+    """
+    antpos = np.asarray(antpos, dtype=np.float64)
+    eq2tops = np.asarray(eq2tops, dtype=np.float64)
+    crd_eq = np.asarray(crd_eq, dtype=np.float64)
+    I_sky = np.asarray(I_sky, dtype=np.float64)
+    jones = np.asarray(jones, dtype=np.complex128)
+    freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    N, F, T = antpos.shape[0], freqs.size, eq2tops.shape[0]
+    assert jones.shape[:3] == (N, 2, 2)
+    out = np.empty((F, 2, 2, T, N, N), dtype=np.complex128)
+    for t in range(T):
+        crd_top = eq2tops[t] @ crd_eq
+        # J[p][k]: (N, K) complex at every source, times sqrt(I)
+        J = [[beam_amplitude(jones[:, p, k].real, crd_top, I_sky) +
+              1j * beam_amplitude(jones[:, p, k].imag, crd_top, I_sky) for k in range(2)] for p in range(2)]
+        dist = antpos @ crd_top
+        for f, nu in enumerate(freqs):
+            ph = np.exp(1j * (2.0 * np.pi * nu / C_LIGHT) * dist)
+            for p in range(2):
+                for q in range(2):
+                    out[f, p, q, t] = sum((J[p][k] * ph) @ (J[q][k] * ph).conj().T for k in range(2))
+    return out
+
+
 def rel_err(v, v_ref) -> float:
     """Frobenius relative error ``||v - v_ref|| / ||v_ref||`` (0 if both zero)."""
     v = np.asarray(v, dtype=np.complex128)
