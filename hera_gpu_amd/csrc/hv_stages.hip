@@ -550,7 +550,84 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
     o[1] = (OUT)(im * r.scale);
 }
 
+// Full-matrix layouts (LAYOUT_FULL, LAYOUT_POL): one workgroup per 32x32
+// output block (i, j), i <= j, of one unit.  Each partial is read once; the
+// block is written as summed and, through LDS, transposed and conjugated
+// into block (j, i) - the values reduce_kernel writes, bit for bit, with
+// half its reads.
+__device__ inline int64_t full_index(const ReduceArgs& r, int64_t f, int64_t t, int a, int b) {
+    if (r.layout == LAYOUT_POL) {
+        const int n = r.nant / 2;                         // antennas per feed
+        const int p = a / n, q = b / n;
+        return (((f * 2 + p) * 2 + q) * r.ntimes_total + t) * (int64_t)n * n + (int64_t)(a - p * n) * n + (b - q * n);
+    }
+    return (f * r.ntimes_total + t) * (int64_t)r.nant * r.nant + (int64_t)a * r.nant + b;
+}
+
+template <typename ACC, typename OUT>
+__global__ __launch_bounds__(256) void reduce_sym_kernel(ReduceArgs r) {
+    __shared__ OUT sre[32][33], sim[32][33];
+    const int n32 = (r.nant + 31) / 32;
+    int i = 0, rem = blockIdx.x;
+    while (rem >= n32 - i) { rem -= n32 - i; ++i; }
+    const int j = i + rem;                                // 32-blocks i <= j
+    const int u = blockIdx.y;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int T = r.tile;
+    const int tid = tile_id((i * 32) / T, (j * 32) / T, r.nb);
+    const size_t tsz = (size_t)T * T;
+    const ACC* base = reinterpret_cast<const ACC*>(r.partials) + ((size_t)u * r.nsplit * r.ntiles + tid) * 2 * tsz;
+    const size_t split_stride = (size_t)r.ntiles * 2 * tsz;
+    const int nsplit = (int)((eff_kpad((int64_t)r.nsplit * KSPLIT, r.dyn_kpad) + KSPLIT - 1) / KSPLIT);
+    const int64_t t = r.t0 + u / r.nfreq_launch, f = r.f0 + u % r.nfreq_launch;
+    OUT* vis = reinterpret_cast<OUT*>(r.vis);
+    const int b = j * 32 + tx;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int ly = ty + 8 * k;
+        const int a = i * 32 + ly;
+        double re = 0.0, im = 0.0;
+        const bool in = a < r.nant && b < r.nant;
+        if (in) {
+            const ACC* p = base + (size_t)(a % T) * T + (b % T);
+            for (int sp = 0; sp < nsplit; ++sp) {
+                re += (double)p[(size_t)sp * split_stride];
+                im += (double)p[(size_t)sp * split_stride + tsz];
+            }
+        }
+        const OUT vr = (OUT)(re * r.scale), vi = (a == b) ? (OUT)0 : (OUT)(im * r.scale);
+        if (in && (i < j || ly <= tx)) {
+            OUT* o = vis + 2 * full_index(r, f, t, a, b);
+            o[0] = vr;
+            o[1] = vi;
+        }
+        sre[ly][tx] = vr;
+        sim[ly][tx] = vi;
+    }
+    __syncthreads();
+    // block (j, i): V[A][B] = conj(V[B][A]) for A > B
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int ry = ty + 8 * k;
+        const int A = j * 32 + ry, B = i * 32 + tx;
+        if (A < r.nant && B < r.nant && (i < j || tx < ry)) {
+            OUT* o = vis + 2 * full_index(r, f, t, A, B);
+            o[0] = sre[tx][ry];
+            o[1] = -sim[tx][ry];
+        }
+    }
+}
+
 hipError_t launch_reduce(int mode, const ReduceArgs& r, hipStream_t s) {
+    if (r.layout == LAYOUT_FULL || r.layout == LAYOUT_POL) {
+        const int n32 = (r.nant + 31) / 32;
+        dim3 grid((unsigned)(n32 * (n32 + 1) / 2), (unsigned)r.nunits);
+        if (mode == MODE_FP64)
+            hipLaunchKernelGGL((reduce_sym_kernel<double, double>), grid, dim3(256), 0, s, r);
+        else
+            hipLaunchKernelGGL((reduce_sym_kernel<float, float>), grid, dim3(256), 0, s, r);
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)((r.nant + 31) / 32), (unsigned)((r.nant + 7) / 8), (unsigned)r.nunits);
     if (mode == MODE_FP64)
         hipLaunchKernelGGL((reduce_kernel<double, double>), grid, dim3(256), 0, s, r);
