@@ -191,6 +191,7 @@ struct AmpArgs {
     size_t beam_fstride;     // 0: one beam for all frequencies (amp per time);
                              // else elements between frequency planes (amp per unit)
     int nfreq;               // frequencies per time of the launch (per-unit amp)
+    int ants_per_wg;         // set by launch_amp (beams that fit in LDS)
 };
 
 // Contraction of split-mode operands generated in place (see hv_herk.hip).

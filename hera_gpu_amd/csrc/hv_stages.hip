@@ -11,6 +11,8 @@
 // difference d = antpos . crd_top in FP64, converts it to quarter turns with
 // 4 nu / c, keeps only the fractional part (exact in FP64 for |turns| < 2^49)
 // and the quadrant, and evaluates sin/cos of that small angle.
+#include <algorithm>
+
 #include "hv_internal.h"
 #include "hv_phasor_math.h"
 
@@ -377,52 +379,91 @@ hipError_t launch_phasor_pol(int mode, const PhasorPolArgs& a, hipStream_t s) {
 // A sqrt(I) (times the FP16 operand scale) per (antenna, source, time) in
 // FP32, for the contraction that builds its operands in place: the same
 // arithmetic, in the same order, as the first half of phasor_kernel, so the
-// operands built from it are bit for bit those phasor_kernel writes.  Grid
-// (antenna, source chunk, time) with the antenna fastest, as phasor_kernel.
+// operands built from it are bit for bit those phasor_kernel writes.  A
+// workgroup serves AMP_ANT antennas (their beams in LDS): per source, the
+// coordinates, the horizon test and the texel position and weights are
+// formed once and applied to each antenna's beam.  Grid (antenna group,
+// source chunk, time or unit).
+constexpr int AMP_ANT = 4;                        // antennas per workgroup, at most
+
 template <typename BT>
 __global__ __launch_bounds__(PH_THREADS) void amp_kernel(AmpArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    BT* sb = reinterpret_cast<BT*>(smem);
+    BT* sb = reinterpret_cast<BT*>(smem);                 // [AMP_ANT][P][P]
     const int64_t kact = eff_kpad(a.kpad, a.dyn_kpad);
     const int64_t wg0 = (int64_t)blockIdx.y * PH_SRC_PER_WG;
     if (wg0 >= kact) return;                       // uniform per workgroup
-    const int ant = blockIdx.x;
+    const int ant0 = blockIdx.x * a.ants_per_wg;
+    const int nq = min(a.ants_per_wg, a.nant - ant0);
     // z: the time of the launch, or (frequency-dependent beams) its unit
     const int z = blockIdx.z;
     const int tb = a.beam_fstride ? z / a.nfreq : z;
     const int fu = a.beam_fstride ? z - tb * a.nfreq : 0;
     const int P = a.beam_px;
-    const BT* gb = reinterpret_cast<const BT*>(a.beam) + fu * a.beam_fstride + (size_t)ant * P * P;
-    for (int i = threadIdx.x; i < P * P; i += PH_THREADS) sb[i] = gb[i];
+    const int pp = P * P;
+    const BT* gb = reinterpret_cast<const BT*>(a.beam) + fu * a.beam_fstride + (size_t)ant0 * pp;
+    for (int i = threadIdx.x; i < nq * pp; i += PH_THREADS) sb[i] = gb[i];
     __syncthreads();
     const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
     const double* cy = cx + a.kpad;
     const double* cz = cx + 2 * a.kpad;
     const int64_t wg1 = min(wg0 + (int64_t)PH_SRC_PER_WG, kact);
     float* out = a.amp + (size_t)z * a.kpad * a.nant;
+    const double pm = (double)(P - 1);
     for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
          s0 += PH_THREADS * PH_SRC_PER_THREAD) {
-        float amp[8];
+        // texel corners and weights of each source (beam_bilinear's arithmetic)
+        int o00[8], o01[8], o10[8], o11[8];
+        BT fx[8], fy[8];
+        float is[8];
+        bool up[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int64_t s = s0 + j;
-            const double x = cx[s], y = cy[s], z = cz[s];
-            float A = 0.f;
-            if (z > 0.0) A = (float)beam_bilinear<BT>(sb, P, x, y);
-            amp[j] = A * (float)a.isqrt[s] * a.amp_scale;
+            const double x = cx[s], y = cy[s], zz = cz[s];
+            up[j] = zz > 0.0;
+            const double bx = (double)(P - 1) * (0.5 * x + 0.5);
+            const double by = (double)(P - 1) * (0.5 * y + 0.5);
+            const double px = floor(bx), py = floor(by);
+            fx[j] = (BT)(bx - px);
+            fy[j] = (BT)(by - py);
+            const int ix0 = (int)clampr(px, 0.0, pm), ix1 = (int)clampr(px + 1.0, 0.0, pm);
+            const int iy0 = (int)clampr(py, 0.0, pm), iy1 = (int)clampr(py + 1.0, 0.0, pm);
+            o00[j] = iy0 * P + ix0; o01[j] = iy0 * P + ix1;
+            o10[j] = iy1 * P + ix0; o11[j] = iy1 * P + ix1;
+            is[j] = (float)a.isqrt[s];
         }
-        float4* d = reinterpret_cast<float4*>(out + ((size_t)(s0 / 16) * a.nant + ant) * 16 + (s0 % 16));
-        d[0] = make_float4(amp[0], amp[1], amp[2], amp[3]);
-        d[1] = make_float4(amp[4], amp[5], amp[6], amp[7]);
+        for (int q = 0; q < nq; ++q) {
+            const BT* b = sb + q * pp;
+            float amp[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                float A = 0.f;
+                if (up[j]) {
+                    const BT v00 = b[o00[j]], v01 = b[o01[j]], v10 = b[o10[j]], v11 = b[o11[j]];
+                    const BT top = fma(fx[j], v01 - v00, v00);
+                    const BT bot = fma(fx[j], v11 - v10, v10);
+                    A = (float)fma(fy[j], bot - top, top);
+                }
+                amp[j] = A * is[j] * a.amp_scale;
+            }
+            float4* d = reinterpret_cast<float4*>(out + ((size_t)(s0 / 16) * a.nant + ant0 + q) * 16 + (s0 % 16));
+            d[0] = make_float4(amp[0], amp[1], amp[2], amp[3]);
+            d[1] = make_float4(amp[4], amp[5], amp[6], amp[7]);
+        }
     }
 }
 
-hipError_t launch_amp(const AmpArgs& a, hipStream_t s) {
+hipError_t launch_amp(const AmpArgs& args, hipStream_t s) {
+    AmpArgs a = args;
+    const size_t plane = (size_t)a.beam_px * a.beam_px * (a.in_f64 ? 8 : 4);
+    a.ants_per_wg = (int)std::max<size_t>(1, std::min<size_t>(AMP_ANT, (size_t)(160 * 1024) / plane));
     const int64_t nchunk = (a.kpad + PH_SRC_PER_WG - 1) / PH_SRC_PER_WG;
     const int64_t nz = a.beam_fstride ? (int64_t)a.ntime * a.nfreq : a.ntime;
     if (nchunk > 65535 || nz > 65535 || a.kpad % 16) return hipErrorInvalidConfiguration;
-    dim3 grid((unsigned)a.nant, (unsigned)nchunk, (unsigned)nz);
-    const size_t lds = (size_t)a.beam_px * a.beam_px * (a.in_f64 ? 8 : 4);
+    dim3 grid((unsigned)((a.nant + a.ants_per_wg - 1) / a.ants_per_wg), (unsigned)nchunk, (unsigned)nz);
+    const size_t lds = (size_t)a.ants_per_wg * plane;
+    if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
     const void* k = a.in_f64 ? (const void*)amp_kernel<double> : (const void*)amp_kernel<float>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
