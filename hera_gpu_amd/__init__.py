@@ -4,6 +4,7 @@
 Importing the package does not touch the GPU; the HIP library is loaded on
 the first call.
 """
+from . import redcal  # noqa: F401  (Omnical: redcal.omnical)
 from .vis import baseline_pairs, vis_gpu, vis_gpu_baselines, vis_gpu_polarized  # noqa: F401
 
 __all__ = ["vis_gpu", "vis_gpu_baselines", "vis_gpu_polarized", "baseline_pairs"]

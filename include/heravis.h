@@ -193,6 +193,25 @@ int hv_vis_pol(hv_ctx* ctx, int mode, int dtype, const void* antpos, int nant, c
                const void* jones, int beam_px, void* vis_out);
 
 /* ---------------------------------------------------------------------------
+ * Omnical redundant calibration, replacing hera_gpu.redcal.omnical
+ * (redcal.py:358-743; kernels redcal.py:70-355).  ndata independent pixels,
+ * each with nbls baselines (ggu_indices[b] = (i, j, k): data ~ g_i conj(g_j)
+ * u_k), nants gains and nubls unique baselines.  precision 1: float32 /
+ * complex64 arrays, 2: float64 / complex128.  gains [ndata][nants] and
+ * ubls [ndata][nubls] complex are read as the starting point and overwritten
+ * with the solution; data [ndata][nbls] complex, wgts [ndata][nbls] real;
+ * chisq, conv [ndata] real and iters [ndata] out.  conv_crit, maxiter,
+ * check_every (> 1), check_after and gain have the meaning of the
+ * reference's arguments.  Deterministic (fixed summation order, no atomics);
+ * a pixel that converges or whose chi^2 rises keeps its previous solution.
+ * Needs 5 (nants + nubls) + 256 values of LDS per workgroup (<= 160 KB).
+ * ------------------------------------------------------------------------- */
+int hv_omnical(hv_ctx* ctx, int precision, const uint32_t* ggu_indices, int nbls, void* gains, int nants,
+               void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata, double conv_crit,
+               int maxiter, int check_every, int check_after, double gain, void* chisq, uint32_t* iters,
+               void* conv);
+
+/* ---------------------------------------------------------------------------
  * Calibration (no reference counterpart): a plain device copy of `bytes`
  * (ms per copy; HBM bandwidth = 2*bytes/time, and the yardstick for
  * rocprofv3 byte counters) and register-resident MFMA loops of each
