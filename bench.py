@@ -104,6 +104,45 @@ def cpu_baseline(w, nsrc, gpu_unit):
     return res
 
 
+def omnical_leg(n=11, ndata=2000, maxiter=50, cpu_pixels=8):
+    """Omnical (redcal.omnical) on a synthetic redundant array: an n x n grid
+    (n = 11: 121 antennas, 7260 baselines, 220 unique baselines; the
+    reference's profiling sizes are 2000 pixels of 54-180 antennas,
+    hera_gpu/redcal.py:426-455), noisy data, starts 5% off the truth.  GPU
+    wall time of the whole call (host arrays in, uploads and downloads
+    included) per precision; the CPU restatement on a few pixels beside it,
+    also the parity check of those pixels."""
+    from hera_gpu_amd.redcal import omnical
+    from oracle.omnical_oracle import omnical as omnical_cpu, redundant_array
+
+    ggu, g, u, data, wgts = redundant_array(n=n, seed=5, nd=ndata, noise=0.01)
+    rng = np.random.default_rng(6)
+    g0 = g * (1 + 0.05 * (rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape)))
+    u0 = u * (1 + 0.05 * (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)))
+    out = {"problem": f"{ndata} pixels x {ggu.shape[0]} baselines, {g.shape[1]} antennas, {u.shape[1]} unique "
+                      f"baselines; maxiter {maxiter}, check_every 4, gain 0.3",
+           "time": "wall time of the call, host arrays in and out (uploads included)"}
+    for precision in (1, 2):
+        kw = dict(maxiter=maxiter, precision=precision)
+        omnical(ggu, g0[:64], u0[:64], data[:64], wgts[:64], **kw)          # warm-up
+        t0 = time.perf_counter()
+        r = omnical(ggu, g0, u0, data, wgts, **kw)
+        dt = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        c = omnical_cpu(ggu, g0[:cpu_pixels], u0[:cpu_pixels], data[:cpu_pixels], wgts[:cpu_pixels], **kw)
+        dc = time.perf_counter() - t1
+        same_iters = bool(np.array_equal(r["iters"][:cpu_pixels], c["iters"]))
+        dg = float(np.max(np.abs(r["gains"][:cpu_pixels] - c["gains"])) / np.max(np.abs(c["gains"])))
+        key = "fp32" if precision == 1 else "fp64"
+        out[key] = {"seconds": dt, "pixels_per_s": ndata / dt, "mean_iters": float(np.mean(r["iters"])),
+                    "cpu_pixels_per_s": cpu_pixels / dc, "cpu_seconds": dc, "cpu_pixels": cpu_pixels,
+                    "cpu_kind": "port (oracle/omnical_oracle.py, numpy, one thread per array op)",
+                    "gpu_vs_cpu_same_iters": same_iters, "gpu_vs_cpu_max_gain_diff": dg}
+        log(f"[bench] omnical {key}: {ndata / dt:.0f} pixels/s ({dt * 1e3:.1f} ms, mean iters "
+            f"{np.mean(r['iters']):.1f}); cpu restatement {cpu_pixels / dc:.2f} pixels/s; same iters {same_iters}")
+    return out
+
+
 def serving_leg(njobs, gemms, nslots, rate=None):
     """The mixed-job serving stream: jobs with N in {37,128,350}, Nsrc log-uniform
     in [1e3, 2^20], ntimes in {1,8,32}, Nfreq in {1,16,64}.  For each arithmetic the
@@ -166,6 +205,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 sample comparison")
     ap.add_argument("--cpu-nsrc", type=int, default=2 ** 16)
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer vis_gpu timing")
+    ap.add_argument("--no-omnical", action="store_true", help="skip the Omnical calibration leg")
     ap.add_argument("--serve-jobs", type=int, default=120, help="serving stream length (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrivals per second (default: 70%% of the measured serving capacity)")
@@ -293,6 +333,8 @@ def main():
         cpu = cpu_baseline(w, nsrc, gpu_unit)
         log(f"[bench] cpu: {cpu['seconds']:.1f} s, {cpu['value']:.4f} TFLOP/s")
 
+    omni = None if args.no_omnical else omnical_leg()
+
     serving = None
     if args.serve_jobs > 0:
         gemms = [args.gemm] + (["fp32"] if args.gemm != "fp32" and not args.no_fp32 else [])
@@ -330,6 +372,7 @@ def main():
         "dropin_pcie_inclusive": dropin,
         "cpu_baseline": cpu,
         "serving": serving,
+        "omnical": omni,
         "p99_latency_ms": serving[args.gemm]["p99_ms"] if serving else None,
         "p50_latency_ms": serving[args.gemm]["p50_ms"] if serving else None,
         "wall_s_timed": wall,
