@@ -143,6 +143,44 @@ def omnical_leg(n=11, ndata=2000, maxiter=50, cpu_pixels=8):
     return out
 
 
+MEDFILT_CASES = (((7000, 7000), 3), ((4000, 4000), 5), ((997, 997), 9))   # test_median_filter.py:8-63
+
+
+def medfilt_leg(reps=20, cpu_rows=256):
+    """The 2-D median filter (MedianFilter) on the reference's own test images
+    (random float32, hera_gpu/tests/test_median_filter.py): the kernel with
+    the image resident in HBM (ms per launch, algorithmic bytes 4 in + 4 out
+    per pixel), the whole call with host arrays, and the CPU restatement on
+    the first `cpu_rows` rows beside it, also their parity check."""
+    from hera_gpu_amd.median_filter import MedianFilter, bench_medfilt2d
+    from oracle.medfilt_oracle import medfilt2d_one
+
+    rng = np.random.default_rng(12)
+    out = {"time": "kernel: ms per launch, image resident in HBM; call: MedianFilter with host arrays"}
+    for shape, ks in MEDFILT_CASES:
+        x = rng.random(shape).astype(np.float32)
+        ms = bench_medfilt2d(x, ks, reps=reps)
+        MedianFilter(input=x[:64], kernel_size=ks)
+        t0 = time.perf_counter()
+        got = MedianFilter(input=x, kernel_size=ks)[0]
+        call = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        ref = medfilt2d_one(x[:cpu_rows], ks)
+        dc = time.perf_counter() - t1
+        h = ks // 2    # rows whose window stays inside the CPU sample
+        same = bool(np.array_equal(got[:cpu_rows - h].view(np.int32), ref[:cpu_rows - h].view(np.int32)))
+        px = x.size
+        gbs = 8 * px / ms / 1e6
+        key = f"{shape[0]}x{shape[1]}_w{ks}"
+        out[key] = {"kernel_ms": ms, "gpixels_per_s": px / ms / 1e6, "GB_per_s": gbs, "hbm_frac": gbs / 8000.0,
+                    "call_ms": call * 1e3, "cpu_pixels_per_s": cpu_rows * shape[1] / dc,
+                    "cpu_sample": f"first {cpu_rows} rows", "cpu_kind": "port (oracle/medfilt_oracle.py, numpy)",
+                    "gpu_equals_cpu_on_sample": same}
+        log(f"[bench] medfilt {key}: kernel {ms:.3f} ms ({px / ms / 1e6:.1f} Gpixel/s, {gbs:.0f} GB/s), call "
+            f"{call * 1e3:.1f} ms; cpu restatement {cpu_rows * shape[1] / dc / 1e6:.2f} Mpixel/s; equal {same}")
+    return out
+
+
 def serving_leg(njobs, gemms, nslots, rate=None):
     """The mixed-job serving stream: jobs with N in {37,128,350}, Nsrc log-uniform
     in [1e3, 2^20], ntimes in {1,8,32}, Nfreq in {1,16,64}.  For each arithmetic the
@@ -206,6 +244,7 @@ def main():
     ap.add_argument("--cpu-nsrc", type=int, default=2 ** 16)
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer vis_gpu timing")
     ap.add_argument("--no-omnical", action="store_true", help="skip the Omnical calibration leg")
+    ap.add_argument("--no-medfilt", action="store_true", help="skip the median filter leg")
     ap.add_argument("--serve-jobs", type=int, default=120, help="serving stream length (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrivals per second (default: 70%% of the measured serving capacity)")
@@ -334,6 +373,7 @@ def main():
         log(f"[bench] cpu: {cpu['seconds']:.1f} s, {cpu['value']:.4f} TFLOP/s")
 
     omni = None if args.no_omnical else omnical_leg()
+    medf = None if args.no_medfilt else medfilt_leg()
 
     serving = None
     if args.serve_jobs > 0:
@@ -373,6 +413,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "omnical": omni,
+        "medfilt": medf,
         "p99_latency_ms": serving[args.gemm]["p99_ms"] if serving else None,
         "p50_latency_ms": serving[args.gemm]["p50_ms"] if serving else None,
         "wall_s_timed": wall,

@@ -212,6 +212,23 @@ int hv_omnical(hv_ctx* ctx, int precision, const uint32_t* ggu_indices, int nbls
                void* conv);
 
 /* ---------------------------------------------------------------------------
+ * 2-D median filter, replacing hera_gpu.median_filter.MedianFilter
+ * (median_filter.py:8-365; kernel mf, :176-216).  images and out are host
+ * arrays [nimg][nrows][ncols] float32; out[i][r][c] is the element of rank
+ * (wrows*wcols)/2 of the wrows x wcols window whose top-left corner is at
+ * (r - wrows/2, c - wcols/2), zeros outside the image (the reference's zero
+ * padding, :318).  Ranks follow IEEE totalOrder (-0 < +0, NaNs at the ends by
+ * sign).  Window sides 1..127; odd sides up to 11 use compile-time selection
+ * networks, other shapes a radix selection.  All images in one launch.
+ * ------------------------------------------------------------------------- */
+int hv_medfilt2d(hv_ctx* ctx, const float* images, float* out, int nimg, int nrows, int ncols, int wrows,
+                 int wcols);
+/* The filter alone over `images` uploaded once: ms per launch over `reps`
+ * launches (after one warm-up), inputs and outputs resident in HBM. */
+int hv_bench_medfilt2d(hv_ctx* ctx, const float* images, int nimg, int nrows, int ncols, int wrows, int wcols,
+                       int reps, double* ms_per_launch);
+
+/* ---------------------------------------------------------------------------
  * Calibration (no reference counterpart): a plain device copy of `bytes`
  * (ms per copy; HBM bandwidth = 2*bytes/time, and the yardstick for
  * rocprofv3 byte counters) and register-resident MFMA loops of each
