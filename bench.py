@@ -112,7 +112,7 @@ def omnical_leg(n=11, ndata=2000, maxiter=50, cpu_pixels=8):
     wall time of the whole call (host arrays in, uploads and downloads
     included) per precision; the CPU restatement on a few pixels beside it,
     also the parity check of those pixels."""
-    from hera_gpu_amd.redcal import omnical
+    from hera_gpu_amd.redcal import bench_omnical, omnical
     from oracle.omnical_oracle import omnical as omnical_cpu, redundant_array
 
     ggu, g, u, data, wgts = redundant_array(n=n, seed=5, nd=ndata, noise=0.01)
@@ -121,25 +121,29 @@ def omnical_leg(n=11, ndata=2000, maxiter=50, cpu_pixels=8):
     u0 = u * (1 + 0.05 * (rng.standard_normal(u.shape) + 1j * rng.standard_normal(u.shape)))
     out = {"problem": f"{ndata} pixels x {ggu.shape[0]} baselines, {g.shape[1]} antennas, {u.shape[1]} unique "
                       f"baselines; maxiter {maxiter}, check_every 4, gain 0.3",
-           "time": "wall time of the call, host arrays in and out (uploads included)"}
+           "time": "pixels_per_s: wall time of the call, host arrays in and out (uploads included); "
+                   "kernel_pixels_per_s: the solve alone with the inputs in HBM"}
     for precision in (1, 2):
         kw = dict(maxiter=maxiter, precision=precision)
         omnical(ggu, g0[:64], u0[:64], data[:64], wgts[:64], **kw)          # warm-up
         t0 = time.perf_counter()
         r = omnical(ggu, g0, u0, data, wgts, **kw)
         dt = time.perf_counter() - t0
+        kms = bench_omnical(ggu, g0, u0, data, wgts, reps=5, **kw)["kernel_ms"]
         t1 = time.perf_counter()
         c = omnical_cpu(ggu, g0[:cpu_pixels], u0[:cpu_pixels], data[:cpu_pixels], wgts[:cpu_pixels], **kw)
         dc = time.perf_counter() - t1
         same_iters = bool(np.array_equal(r["iters"][:cpu_pixels], c["iters"]))
         dg = float(np.max(np.abs(r["gains"][:cpu_pixels] - c["gains"])) / np.max(np.abs(c["gains"])))
         key = "fp32" if precision == 1 else "fp64"
-        out[key] = {"seconds": dt, "pixels_per_s": ndata / dt, "mean_iters": float(np.mean(r["iters"])),
+        out[key] = {"seconds": dt, "pixels_per_s": ndata / dt, "kernel_ms": kms,
+                    "kernel_pixels_per_s": ndata / (kms / 1e3), "mean_iters": float(np.mean(r["iters"])),
                     "cpu_pixels_per_s": cpu_pixels / dc, "cpu_seconds": dc, "cpu_pixels": cpu_pixels,
                     "cpu_kind": "port (oracle/omnical_oracle.py, numpy, one thread per array op)",
                     "gpu_vs_cpu_same_iters": same_iters, "gpu_vs_cpu_max_gain_diff": dg}
-        log(f"[bench] omnical {key}: {ndata / dt:.0f} pixels/s ({dt * 1e3:.1f} ms, mean iters "
-            f"{np.mean(r['iters']):.1f}); cpu restatement {cpu_pixels / dc:.2f} pixels/s; same iters {same_iters}")
+        log(f"[bench] omnical {key}: {ndata / dt:.0f} pixels/s ({dt * 1e3:.1f} ms; kernel {kms:.1f} ms, "
+            f"{ndata / (kms / 1e3):.0f} pixels/s), mean iters "
+            f"{np.mean(r['iters']):.1f}; cpu restatement {cpu_pixels / dc:.2f} pixels/s; same iters {same_iters}")
     return out
 
 

@@ -79,6 +79,9 @@ SIGNATURES = {
     "hv_vis_pol": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _vp]),
     "hv_omnical": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, C.c_double, _i, _i, _i, C.c_double,
                         _vp, _vp, _vp]),
+    "hv_set_omnical_place": (_i, [_i]),
+    "hv_bench_omnical": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, C.c_double, _i, _i, _i, C.c_double,
+                              _vp, _vp, _vp, _i, C.POINTER(_d)]),
     "hv_medfilt2d": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i]),
     "hv_bench_medfilt2d": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, C.POINTER(_d)]),
     "hv_k_rotate": (_i, [_vp, _i, _vp, _vp, _i64, _vp]),

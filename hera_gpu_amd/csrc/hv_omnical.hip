@@ -38,7 +38,7 @@ struct OmniArgs {
     const R* wgts;                        // [ndata][nbls]
     R* gains;                             // [ndata][nants] complex, in/out
     R* ubls;                              // [ndata][nubls] complex, in/out
-    R* scratch;                           // [grid][nbls][5]: model (2), dwgts (1), weighted product (2)
+    R* scratch;                           // per workgroup: the per-baseline arrays kept outside LDS
     R* chisq;                             // [ndata]
     uint32_t* iters;                      // [ndata]
     R* conv;                              // [ndata]
@@ -47,6 +47,12 @@ struct OmniArgs {
     int maxiter, check_every, check_after;
     R one_m, gs, crit;
 };
+
+// Where a workgroup keeps its per-baseline arrays: the weighted products
+// (2 values per baseline, gathered per antenna and unique baseline) and the
+// data weights |model|^2 w of the last check (1 value).  Chosen per call from
+// what fits in LDS beside the gains and unique baselines.
+enum OmniPlace { OMNI_GLOBAL = 0, OMNI_PROD_LDS = 1, OMNI_ALL_LDS = 2 };
 
 template <typename R>
 __device__ __forceinline__ R mag2(R x, R y) {
@@ -62,38 +68,91 @@ __device__ __forceinline__ void cmul(R ar, R ai, R br, R bi, R& cr, R& ci) {
 }
 
 // Sum of one value per thread-strided index: partial p = v(p) + v(p+256) + ...
-// in order, then the pairwise tree.  Every thread gets the total.
+// in order, then the pairwise tree p[i] += p[i + s], s = 128, 64, ..., 1 (the
+// last six levels inside wave 0 by lane shuffles, the same additions).
+// Every thread gets the total.
 template <typename R, typename F>
 __device__ __forceinline__ R fixed_sum(R* red, int n, F v) {
 #pragma clang fp contract(off)
     R acc = 0;
+#pragma unroll 4
     for (int b = threadIdx.x; b < n; b += OMNI_THREADS) acc = acc + v(b);
-    red[threadIdx.x] = acc;
+    const int t = threadIdx.x;
+    red[t] = acc;
     __syncthreads();
-    for (int s = OMNI_THREADS / 2; s >= 1; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
-        __syncthreads();
+    if (t < 128) red[t] = red[t] + red[t + 128];
+    __syncthreads();
+    if (t < 64) {
+        R x = red[t] + red[t + 64];
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) x = x + __shfl_down(x, s, 64);
+        if (t == 0) red[0] = x;
     }
+    __syncthreads();
     const R total = red[0];
     __syncthreads();
     return total;
 }
 
-// model g_i conj(g_j) u_k of every baseline into the scratch slice
+// g_i conj(g_j) u_k of baseline b (redcal.py:70-93), G and U in LDS
 template <typename R>
-__device__ __forceinline__ void gen_model(const OmniArgs<R>& a, const R* g, const R* u, R* sc) {
+__device__ __forceinline__ void model_of(const OmniArgs<R>& a, int b, const R* G, const R* U, R& mr, R& mi) {
 #pragma clang fp contract(off)
-    for (int b = threadIdx.x; b < a.nbls; b += OMNI_THREADS) {
-        const uint32_t i = a.ggu[3 * b], j = a.ggu[3 * b + 1], k = a.ggu[3 * b + 2];
-        R tr, ti, mr, mi;
-        cmul(g[2 * i], g[2 * i + 1], g[2 * j], -g[2 * j + 1], tr, ti);
-        cmul(tr, ti, u[2 * k], u[2 * k + 1], mr, mi);
-        sc[5 * b] = mr;
-        sc[5 * b + 1] = mi;
+    const uint32_t i = a.ggu[3 * b], j = a.ggu[3 * b + 1], k = a.ggu[3 * b + 2];
+    R tr, ti;
+    cmul(G[2 * i], G[2 * i + 1], G[2 * j], -G[2 * j + 1], tr, ti);
+    cmul(tr, ti, U[2 * k], U[2 * k + 1], mr, mi);
+}
+
+// s + v[idx[q0]] + v[idx[q0 + 1]] + ... + v[idx[q1 - 1]] in that order; the
+// indices and values of 8 terms are loaded before they are added.
+template <typename R>
+__device__ __forceinline__ R gather_sum(const int* idx, int q0, int q1, const R* v, R s) {
+#pragma clang fp contract(off)
+    int q = q0;
+    for (; q + 8 <= q1; q += 8) {
+        int b[8];
+        R x[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) b[t] = idx[q + t];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) x[t] = v[b[t]];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s = s + x[t];
+    }
+    for (; q < q1; ++q) s = s + v[idx[q]];
+    return s;
+}
+
+// The same over complex pairs v[2b], v[2b + 1]; NEG: the imaginary parts enter negated.
+template <typename R, bool NEG>
+__device__ __forceinline__ void gather_sum2(const int* idx, int q0, int q1, const R* v, R& sr, R& si) {
+#pragma clang fp contract(off)
+    int q = q0;
+    for (; q + 8 <= q1; q += 8) {
+        int b[8];
+        R xr[8], xi[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) b[t] = idx[q + t];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            xr[t] = v[2 * b[t]];
+            xi[t] = v[2 * b[t] + 1];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            sr = sr + xr[t];
+            si = NEG ? si + -xi[t] : si + xi[t];
+        }
+    }
+    for (; q < q1; ++q) {
+        const int b = idx[q];
+        sr = sr + v[2 * b];
+        si = NEG ? si + -v[2 * b + 1] : si + v[2 * b + 1];
     }
 }
 
-template <typename R>
+template <typename R, int PLACE>
 __global__ __launch_bounds__(OMNI_THREADS) void omnical_kernel(OmniArgs<R> a) {
 #pragma clang fp contract(off)
     extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -104,7 +163,12 @@ __global__ __launch_bounds__(OMNI_THREADS) void omnical_kernel(OmniArgs<R> a) {
     R* gw = un + 2 * a.nubls;            // [nants]
     R* uw = gw + a.nants;                // [nubls]
     R* red = uw + a.nubls;               // [256]
-    R* sc = a.scratch + (size_t)blockIdx.x * a.nbls * 5;
+    R* lds_b = red + OMNI_THREADS;       // per-baseline arrays held in LDS
+    // weighted products [nbls][2] and weights [nbls]
+    constexpr int NGLOBAL = PLACE == OMNI_GLOBAL ? 3 : PLACE == OMNI_PROD_LDS ? 1 : 0;
+    R* gsc = a.scratch + (size_t)blockIdx.x * a.nbls * NGLOBAL;
+    R* prod = PLACE == OMNI_GLOBAL ? gsc + a.nbls : lds_b;
+    R* dw = PLACE == OMNI_ALL_LDS ? lds_b + 2 * a.nbls : gsc;
     const int tid = threadIdx.x;
     const int nsol = a.nants + a.nubls;
 
@@ -114,94 +178,97 @@ __global__ __launch_bounds__(OMNI_THREADS) void omnical_kernel(OmniArgs<R> a) {
         for (int n = tid; n < 2 * a.nants; n += OMNI_THREADS) g[n] = a.gains[(size_t)px * a.nants * 2 + n];
         for (int n = tid; n < 2 * a.nubls; n += OMNI_THREADS) u[n] = a.ubls[(size_t)px * a.nubls * 2 + n];
         __syncthreads();
-        gen_model(a, g, u, sc);
-        __syncthreads();
-        auto resid = [&](int b) {
-            const R er = data[2 * b] - sc[5 * b], ei = data[2 * b + 1] - sc[5 * b + 1];
-            return mag2(er, ei) * wgts[b];
+        // chi^2 of the model of (G, U) (redcal.py:96-114)
+        auto chisq_of = [&](const R* G, const R* U) {
+            return fixed_sum(red, a.nbls, [&](int b) {
+                R mr, mi;
+                model_of(a, b, G, U, mr, mi);
+                const R er = data[2 * b] - mr, ei = data[2 * b + 1] - mi;
+                return mag2(er, ei) * wgts[b];
+            });
         };
-        R chisq = fixed_sum(red, a.nbls, resid);
+        R chisq = chisq_of(g, u);
         uint32_t iters = 0;
         R conv = 0;
         for (int it = 1; it <= a.maxiter; ++it) {
             if (it % a.check_every == 1) {
-                for (int b = tid; b < a.nbls; b += OMNI_THREADS) sc[5 * b + 2] = mag2(sc[5 * b], sc[5 * b + 1]) * wgts[b];
+                // data weights |model|^2 w and their sums per antenna and
+                // unique baseline (redcal.py:117-163)
+                for (int b = tid; b < a.nbls; b += OMNI_THREADS) {
+                    R mr, mi;
+                    model_of(a, b, g, u, mr, mi);
+                    dw[b] = mag2(mr, mi) * wgts[b];
+                }
                 __syncthreads();
                 for (int n = tid; n < nsol; n += OMNI_THREADS) {
                     R s = 0;
                     if (n < a.nants) {
-                        for (int q = a.iptr[n]; q < a.iptr[n + 1]; ++q) s = s + sc[5 * a.ib[q] + 2];
-                        for (int q = a.jptr[n]; q < a.jptr[n + 1]; ++q) s = s + sc[5 * a.jb[q] + 2];
-                        gw[n] = s;
+                        s = gather_sum(a.ib, a.iptr[n], a.iptr[n + 1], dw, s);
+                        gw[n] = gather_sum(a.jb, a.jptr[n], a.jptr[n + 1], dw, s);
                     } else {
                         const int k = n - a.nants;
-                        for (int q = a.kptr[k]; q < a.kptr[k + 1]; ++q) s = s + sc[5 * a.kb[q] + 2];
-                        uw[k] = s;
+                        uw[k] = gather_sum(a.kb, a.kptr[k], a.kptr[k + 1], dw, s);
                     }
                 }
             }
-            // weighted products data conj(model) dwgts / |model|^2
-            for (int b = tid; b < a.nbls; b += OMNI_THREADS) {
-                const R mr = sc[5 * b], mi = sc[5 * b + 1];
-                const R wt = sc[5 * b + 2] / mag2(mr, mi);
-                R pr, pi;
-                cmul(data[2 * b], data[2 * b + 1], mr, -mi, pr, pi);
-                sc[5 * b + 3] = pr * wt;
-                sc[5 * b + 4] = pi * wt;
+            // weighted products data conj(model) dwgts / |model|^2 (redcal.py:166-199)
+            for (int b0 = tid; b0 < a.nbls; b0 += 4 * OMNI_THREADS) {
+                R pr[4], pi[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int b = b0 + t * OMNI_THREADS;
+                    if (b >= a.nbls) break;
+                    R mr, mi;
+                    model_of(a, b, g, u, mr, mi);
+                    const R wt = dw[b] / mag2(mr, mi);
+                    cmul(data[2 * b], data[2 * b + 1], mr, -mi, pr[t], pi[t]);
+                    pr[t] = pr[t] * wt;
+                    pi[t] = pi[t] * wt;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int b = b0 + t * OMNI_THREADS;
+                    if (b >= a.nbls) break;
+                    prod[2 * b] = pr[t];
+                    prod[2 * b + 1] = pi[t];
+                }
             }
             __syncthreads();
-            // candidate solution: x * ((1 - g) + g buf / wgt)
+            // candidate solution: x * ((1 - g) + g buf / wgt) (redcal.py:202-314)
             for (int n = tid; n < nsol; n += OMNI_THREADS) {
                 R br = 0, bi = 0;
                 if (n < a.nants) {
-                    for (int q = a.iptr[n]; q < a.iptr[n + 1]; ++q) {
-                        br = br + sc[5 * a.ib[q] + 3];
-                        bi = bi + sc[5 * a.ib[q] + 4];
-                    }
-                    for (int q = a.jptr[n]; q < a.jptr[n + 1]; ++q) {
-                        br = br + sc[5 * a.jb[q] + 3];
-                        bi = bi + -sc[5 * a.jb[q] + 4];
-                    }
+                    gather_sum2<R, false>(a.ib, a.iptr[n], a.iptr[n + 1], prod, br, bi);
+                    gather_sum2<R, true>(a.jb, a.jptr[n], a.jptr[n + 1], prod, br, bi);
                     const R wr = a.one_m + a.gs * br / gw[n], wi = a.gs * bi / gw[n];
                     cmul(g[2 * n], g[2 * n + 1], wr, wi, gn[2 * n], gn[2 * n + 1]);
                 } else {
                     const int k = n - a.nants;
-                    for (int q = a.kptr[k]; q < a.kptr[k + 1]; ++q) {
-                        br = br + sc[5 * a.kb[q] + 3];
-                        bi = bi + sc[5 * a.kb[q] + 4];
-                    }
+                    gather_sum2<R, false>(a.kb, a.kptr[k], a.kptr[k + 1], prod, br, bi);
                     const R wr = a.one_m + a.gs * br / uw[k], wi = a.gs * bi / uw[k];
                     cmul(u[2 * k], u[2 * k + 1], wr, wi, un[2 * k], un[2 * k + 1]);
                 }
             }
             __syncthreads();
             const bool fast = it < a.maxiter && (it < a.check_after || it % a.check_every != 0);
-            if (fast) {
-                for (int n = tid; n < 2 * a.nants; n += OMNI_THREADS) g[n] = gn[n];
-                for (int n = tid; n < 2 * a.nubls; n += OMNI_THREADS) u[n] = un[n];
-                __syncthreads();
-                gen_model(a, g, u, sc);
-                __syncthreads();
-                continue;
+            if (!fast) {
+                const R new_chisq = chisq_of(gn, un);
+                auto dsol = [&](int n) {
+                    return n < a.nants ? mag2(gn[2 * n] - g[2 * n], gn[2 * n + 1] - g[2 * n + 1])
+                                       : mag2(un[2 * (n - a.nants)] - u[2 * (n - a.nants)],
+                                              un[2 * (n - a.nants) + 1] - u[2 * (n - a.nants) + 1]);
+                };
+                auto nsq = [&](int n) {
+                    return n < a.nants ? mag2(gn[2 * n], gn[2 * n + 1])
+                                       : mag2(un[2 * (n - a.nants)], un[2 * (n - a.nants) + 1]);
+                };
+                const R csum = fixed_sum(red, nsol, dsol);
+                const R cwgt = fixed_sum(red, nsol, nsq);
+                conv = sqrt(csum / cwgt);
+                if (conv < a.crit || new_chisq > chisq) break;   // uniform: keep the previous solution
+                chisq = new_chisq;
+                iters = (uint32_t)it;
             }
-            gen_model(a, gn, un, sc);
-            __syncthreads();
-            const R new_chisq = fixed_sum(red, a.nbls, resid);
-            auto dsol = [&](int n) {
-                return n < a.nants ? mag2(gn[2 * n] - g[2 * n], gn[2 * n + 1] - g[2 * n + 1])
-                                   : mag2(un[2 * (n - a.nants)] - u[2 * (n - a.nants)],
-                                          un[2 * (n - a.nants) + 1] - u[2 * (n - a.nants) + 1]);
-            };
-            auto nsq = [&](int n) {
-                return n < a.nants ? mag2(gn[2 * n], gn[2 * n + 1])
-                                   : mag2(un[2 * (n - a.nants)], un[2 * (n - a.nants) + 1]);
-            };
-            const R csum = fixed_sum(red, nsol, dsol);
-            const R cwgt = fixed_sum(red, nsol, nsq);
-            conv = sqrt(csum / cwgt);
-            if (conv < a.crit || new_chisq > chisq) break;   // uniform: keep the previous solution
-            chisq = new_chisq;
-            iters = (uint32_t)it;
             for (int n = tid; n < 2 * a.nants; n += OMNI_THREADS) g[n] = gn[n];
             for (int n = tid; n < 2 * a.nubls; n += OMNI_THREADS) u[n] = un[n];
             __syncthreads();
@@ -227,33 +294,71 @@ void csr(const uint32_t* ggu, int nbls, int col, int nkeys, std::vector<int>& pt
     for (int b = 0; b < nbls; ++b) idx[fill[ggu[3 * b + col]]++] = b;
 }
 
+int g_omni_place = -1;   // process-wide override (hv_set_omnical_place), -1: automatic
+
+template <typename R, int PLACE>
+int omni_occupancy(size_t lds, int* per_cu) {
+    if (lds > 64 * 1024)
+        HV_HIP(hipFuncSetAttribute((const void*)omnical_kernel<R, PLACE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, omnical_kernel<R, PLACE>, OMNI_THREADS, lds));
+    return HV_OK;
+}
+
+// One solve of every pixel.  reps > 0 (timing): the solve runs 1 + reps times
+// from the same starting point, *ms = mean kernel time of the last reps.
 template <typename R>
 int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, void* ubls, int nubls,
               const void* data, const void* wgts, int64_t ndata, double conv_crit, int maxiter, int check_every,
-              int check_after, double gain, void* chisq, uint32_t* iters, void* conv) {
-    const size_t lds = ((size_t)4 * nants + 4 * nubls + nants + nubls + OMNI_THREADS) * sizeof(R);
-    if (lds > 160 * 1024)
+              int check_after, double gain, void* chisq, uint32_t* iters, void* conv, int reps = 0,
+              double* ms = nullptr) {
+    constexpr size_t LDS_MAX = 160 * 1024;
+    const size_t lds_base = ((size_t)4 * nants + 4 * nubls + nants + nubls + OMNI_THREADS) * sizeof(R);
+    if (lds_base > LDS_MAX)
         return fail(HV_ERR_ARG, "omnical: gains, unique baselines and their candidates must fit in LDS "
                                 "(5 (nants + nubls) + 256 values <= 160 KB)");
+    int dev = 0, ncu = 0;
+    HV_HIP(hipGetDevice(&dev));
+    HV_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // Per-baseline arrays in LDS as far as they fit, unless that leaves fewer
+    // than two resident workgroups per CU for a batch that fills the device:
+    // the solve is latency-bound, and on MI355X more resident pixels win over
+    // LDS placement (121 antennas, 2000 pixels: float 7.9 ms with the products
+    // in LDS at 2 per CU against 10.9 ms all in LDS at 1; double 15.5 ms in
+    // device memory at 3 per CU against 19.2 ms at 1).
+    // values per baseline held in LDS: 3 (all), 2 (the products) or none
+    auto lds_of = [&](int pl) {
+        return lds_base + (size_t)(pl == OMNI_ALL_LDS ? 3 : pl == OMNI_PROD_LDS ? 2 : 0) * nbls * sizeof(R);
+    };
+    auto occupancy = [&](int pl, int* per_cu) -> int {
+        const size_t l = lds_of(pl);
+        if (pl == OMNI_ALL_LDS) return omni_occupancy<R, OMNI_ALL_LDS>(l, per_cu);
+        if (pl == OMNI_PROD_LDS) return omni_occupancy<R, OMNI_PROD_LDS>(l, per_cu);
+        return omni_occupancy<R, OMNI_GLOBAL>(l, per_cu);
+    };
+    int place = lds_of(OMNI_ALL_LDS) <= LDS_MAX ? OMNI_ALL_LDS : lds_of(OMNI_PROD_LDS) <= LDS_MAX ? OMNI_PROD_LDS
+                                                                                               : OMNI_GLOBAL;
+    if (g_omni_place >= 0) place = std::min(place, g_omni_place);
+    int per_cu = 0;
+    HV_CHECK(occupancy(place, &per_cu));
+    if (g_omni_place < 0)
+        while (place > OMNI_GLOBAL && per_cu < 2 && ndata > ncu) HV_CHECK(occupancy(--place, &per_cu));
+    const int nglobal = place == OMNI_GLOBAL ? 3 : place == OMNI_PROD_LDS ? 1 : 0;
+    const size_t lds = lds_of(place);
     std::vector<int> iptr, ib, jptr, jb, kptr, kb;
     csr(ggu, nbls, 0, nants, iptr, ib);
     csr(ggu, nbls, 1, nants, jptr, jb);
     csr(ggu, nbls, 2, nubls, kptr, kb);
-    int dev = 0, ncu = 0, per_cu = 0;
-    HV_HIP(hipGetDevice(&dev));
-    HV_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    if (lds > 64 * 1024)
-        HV_HIP(hipFuncSetAttribute((const void*)omnical_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, omnical_kernel<R>, OMNI_THREADS, lds));
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(ndata, (int64_t)ncu * std::max(per_cu, 1)));
 
     const size_t b_ggu = (size_t)nbls * 3 * 4, b_csr = ((size_t)2 * (nants + 1) + nubls + 1 + 3 * (size_t)nbls) * 4;
     const size_t b_data = (size_t)ndata * nbls * 2 * sizeof(R), b_wgts = (size_t)ndata * nbls * sizeof(R);
     const size_t b_g = (size_t)ndata * nants * 2 * sizeof(R), b_u = (size_t)ndata * nubls * 2 * sizeof(R);
-    const size_t b_sc = (size_t)grid * nbls * 5 * sizeof(R), b_out = (size_t)ndata * (2 * sizeof(R) + 4);
+    const size_t b_sc = std::max<size_t>(1, (size_t)grid * nbls * nglobal) * sizeof(R);
     char* base = nullptr;
-    const size_t total = b_ggu + b_csr + b_data + b_wgts + b_g + b_u + b_sc + b_out + 16 * 256;   // + alignment of 10 slices
+    const size_t b_start = reps > 0 ? b_g + b_u + 512 : 0;   // starting points kept for repeated solves
+    const size_t total = b_ggu + b_csr + b_data + b_wgts + b_g + b_u + b_sc + (size_t)ndata * (2 * sizeof(R) + 4) +
+                         16 * 256 + b_start;   // + alignment of 10 slices
     HV_CHECK(dev_alloc(c, (void**)&base, total));
     auto carve = [&](size_t bytes) {
         char* p = base;
@@ -271,6 +376,9 @@ int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, 
     R* d_chisq = (R*)carve(ndata * sizeof(R));
     R* d_conv = (R*)carve(ndata * sizeof(R));
     uint32_t* d_iters = (uint32_t*)carve(ndata * 4);
+    R* d_g0 = reps > 0 ? (R*)carve(b_g) : nullptr;
+    R* d_u0 = reps > 0 ? (R*)carve(b_u) : nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<int> all;
     all.insert(all.end(), iptr.begin(), iptr.end());
     all.insert(all.end(), jptr.begin(), jptr.end());
@@ -279,7 +387,6 @@ int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, 
     all.insert(all.end(), jb.begin(), jb.end());
     all.insert(all.end(), kb.begin(), kb.end());
     hipStream_t s = c->stream;
-    int rc = HV_OK;
     auto run = [&]() -> int {
         HV_HIP(hipMemcpyAsync(d_ggu, ggu, b_ggu, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(d_csr, all.data(), all.size() * 4, hipMemcpyHostToDevice, s));
@@ -314,8 +421,36 @@ int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, 
         a.one_m = (R)(1.0f - gs);          // (1 - gstep) in float32, redcal.py:274
         a.gs = (R)gs;
         a.crit = (R)conv_crit;
-        hipLaunchKernelGGL((omnical_kernel<R>), dim3((unsigned)grid), dim3(OMNI_THREADS), lds, s, a);
-        HV_HIP(hipGetLastError());
+        const dim3 gr((unsigned)grid), bl(OMNI_THREADS);
+        auto launch = [&]() -> int {
+            if (place == OMNI_ALL_LDS) hipLaunchKernelGGL((omnical_kernel<R, OMNI_ALL_LDS>), gr, bl, lds, s, a);
+            else if (place == OMNI_PROD_LDS) hipLaunchKernelGGL((omnical_kernel<R, OMNI_PROD_LDS>), gr, bl, lds, s, a);
+            else hipLaunchKernelGGL((omnical_kernel<R, OMNI_GLOBAL>), gr, bl, lds, s, a);
+            HV_HIP(hipGetLastError());
+            return HV_OK;
+        };
+        if (reps > 0) {
+            HV_HIP(hipEventCreate(&e0));
+            HV_HIP(hipEventCreate(&e1));
+            HV_HIP(hipMemcpyAsync(d_g0, d_g, b_g, hipMemcpyDeviceToDevice, s));
+            HV_HIP(hipMemcpyAsync(d_u0, d_u, b_u, hipMemcpyDeviceToDevice, s));
+            HV_CHECK(launch());   // warm-up
+            double total_ms = 0;
+            for (int r = 0; r < reps; ++r) {
+                HV_HIP(hipMemcpyAsync(d_g, d_g0, b_g, hipMemcpyDeviceToDevice, s));
+                HV_HIP(hipMemcpyAsync(d_u, d_u0, b_u, hipMemcpyDeviceToDevice, s));
+                HV_HIP(hipEventRecord(e0, s));
+                HV_CHECK(launch());
+                HV_HIP(hipEventRecord(e1, s));
+                HV_HIP(hipEventSynchronize(e1));
+                float t = 0;
+                HV_HIP(hipEventElapsedTime(&t, e0, e1));
+                total_ms += t;
+            }
+            *ms = total_ms / reps;
+        } else {
+            HV_CHECK(launch());
+        }
         HV_HIP(hipMemcpyAsync(gains, d_g, b_g, hipMemcpyDeviceToHost, s));
         HV_HIP(hipMemcpyAsync(ubls, d_u, b_u, hipMemcpyDeviceToHost, s));
         HV_HIP(hipMemcpyAsync(chisq, d_chisq, ndata * sizeof(R), hipMemcpyDeviceToHost, s));
@@ -324,17 +459,26 @@ int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, 
         HV_HIP(hipStreamSynchronize(s));
         return HV_OK;
     };
-    rc = run();
+    const int rc = run();
+    if (rc != HV_OK) (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     dev_free(c, mem, total);
     return rc;
 }
 
 }  // namespace
 
-extern "C" int hv_omnical(hv_ctx* c, int precision, const uint32_t* ggu_indices, int nbls, void* gains, int nants,
-                          void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata,
-                          double conv_crit, int maxiter, int check_every, int check_after, double gain,
-                          void* chisq, uint32_t* iters, void* conv) {
+extern "C" int hv_set_omnical_place(int place) {
+    if (place < -1 || place > 2) return fail(HV_ERR_ARG, "omnical placement must be -1, 0, 1 or 2");
+    g_omni_place = place;
+    return HV_OK;
+}
+
+static int omnical_entry(hv_ctx* c, int precision, const uint32_t* ggu_indices, int nbls, void* gains, int nants,
+                         void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata, double conv_crit,
+                         int maxiter, int check_every, int check_after, double gain, void* chisq, uint32_t* iters,
+                         void* conv, int reps, double* ms) {
     if (!c || !ggu_indices || !gains || !ubls || !data || !wgts || !chisq || !iters || !conv)
         return fail(HV_ERR_ARG, "NULL argument");
     if (precision != 1 && precision != 2) return fail(HV_ERR_ARG, "precision must be 1 or 2");
@@ -349,7 +493,24 @@ extern "C" int hv_omnical(hv_ctx* c, int precision, const uint32_t* ggu_indices,
     HV_HIP(hipSetDevice(c->device));
     if (precision == 1)
         return omnical_t<float>(c, ggu_indices, nbls, gains, nants, ubls, nubls, data, wgts, ndata, conv_crit,
-                                maxiter, check_every, check_after, gain, chisq, iters, conv);
+                                maxiter, check_every, check_after, gain, chisq, iters, conv, reps, ms);
     return omnical_t<double>(c, ggu_indices, nbls, gains, nants, ubls, nubls, data, wgts, ndata, conv_crit, maxiter,
-                             check_every, check_after, gain, chisq, iters, conv);
+                             check_every, check_after, gain, chisq, iters, conv, reps, ms);
+}
+
+extern "C" int hv_omnical(hv_ctx* c, int precision, const uint32_t* ggu_indices, int nbls, void* gains, int nants,
+                          void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata,
+                          double conv_crit, int maxiter, int check_every, int check_after, double gain,
+                          void* chisq, uint32_t* iters, void* conv) {
+    return omnical_entry(c, precision, ggu_indices, nbls, gains, nants, ubls, nubls, data, wgts, ndata, conv_crit,
+                         maxiter, check_every, check_after, gain, chisq, iters, conv, 0, nullptr);
+}
+
+extern "C" int hv_bench_omnical(hv_ctx* c, int precision, const uint32_t* ggu_indices, int nbls, void* gains,
+                                int nants, void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata,
+                                double conv_crit, int maxiter, int check_every, int check_after, double gain,
+                                void* chisq, uint32_t* iters, void* conv, int reps, double* ms_per_solve) {
+    if (reps < 1 || !ms_per_solve) return fail(HV_ERR_ARG, "bad argument");
+    return omnical_entry(c, precision, ggu_indices, nbls, gains, nants, ubls, nubls, data, wgts, ndata, conv_crit,
+                         maxiter, check_every, check_after, gain, chisq, iters, conv, reps, ms_per_solve);
 }

@@ -21,7 +21,7 @@ NTHREADS = 1024   # accepted for the reference's signature; one 256-thread workg
 
 
 def omnical(ggu_indices, gains, ubls, data, wgts, conv_crit=1e-10, maxiter=50, check_every=4, check_after=1,
-            gain=0.3, nthreads=NTHREADS, precision=1, verbose=False, *, device=0):
+            gain=0.3, nthreads=NTHREADS, precision=1, verbose=False, *, device=0, _reps=0):
     """Solve every pixel; returns {'gains', 'ubls', 'chisq', 'iters', 'conv'}.
 
     Arguments as redcal.py:358-390: ggu_indices (nbls, 3) (i, j, k) per
@@ -57,9 +57,28 @@ def omnical(ggu_indices, gains, ubls, data, wgts, conv_crit=1e-10, maxiter=50, c
     if verbose:
         print("PRECISION:", precision, "NDATA:", ndata, "NANTS:", nants, "NUBLS:", nubls, "NBLS:", nbls)
     ctx = _lib.context(device)
-    rc = _lib.load().hv_omnical(ctx.handle, precision, _lib.ptr(ggu), nbls, _lib.ptr(g), nants, _lib.ptr(u), nubls,
-                                _lib.ptr(data), _lib.ptr(wgts), ndata, float(conv_crit), int(maxiter),
-                                int(check_every), int(check_after), float(gain), _lib.ptr(chisq), _lib.ptr(iters),
-                                _lib.ptr(conv))
-    _lib.check(rc, "hv_omnical")
-    return {"gains": g, "ubls": u, "chisq": chisq, "iters": iters, "conv": conv}
+    args = (ctx.handle, precision, _lib.ptr(ggu), nbls, _lib.ptr(g), nants, _lib.ptr(u), nubls, _lib.ptr(data),
+            _lib.ptr(wgts), ndata, float(conv_crit), int(maxiter), int(check_every), int(check_after), float(gain),
+            _lib.ptr(chisq), _lib.ptr(iters), _lib.ptr(conv))
+    out = {"gains": g, "ubls": u, "chisq": chisq, "iters": iters, "conv": conv}
+    if _reps:
+        ms = C.c_double()
+        _lib.check(_lib.load().hv_bench_omnical(*args, int(_reps), C.byref(ms)), "hv_bench_omnical")
+        out["kernel_ms"] = ms.value
+    else:
+        _lib.check(_lib.load().hv_omnical(*args), "hv_omnical")
+    return out
+
+
+def bench_omnical(ggu_indices, gains, ubls, data, wgts, reps=5, **kw):
+    """omnical with the inputs uploaded once and the solve repeated from the
+    same start: the returned dict also holds 'kernel_ms', the mean kernel
+    time of one solve of every pixel."""
+    return omnical(ggu_indices, gains, ubls, data, wgts, _reps=reps, **kw)
+
+
+def set_placement(place: int = -1) -> None:
+    """Cap where the solver keeps its per-baseline arrays (tests and tuning):
+    -1 automatic, 0 device memory, 1 weighted products in LDS, 2 products and
+    data weights in LDS (lowered to what fits)."""
+    _lib.check(_lib.load().hv_set_omnical_place(int(place)), "hv_set_omnical_place")

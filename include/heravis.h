@@ -204,12 +204,27 @@ int hv_vis_pol(hv_ctx* ctx, int mode, int dtype, const void* antpos, int nant, c
  * check_every (> 1), check_after and gain have the meaning of the
  * reference's arguments.  Deterministic (fixed summation order, no atomics);
  * a pixel that converges or whose chi^2 rises keeps its previous solution.
- * Needs 5 (nants + nubls) + 256 values of LDS per workgroup (<= 160 KB).
+ * Needs 5 (nants + nubls) + 256 values of LDS per workgroup (<= 160 KB); the
+ * per-baseline arrays (weighted products, data weights) go to LDS when they
+ * fit and at least two workgroups per CU stay resident, else to device memory.
  * ------------------------------------------------------------------------- */
 int hv_omnical(hv_ctx* ctx, int precision, const uint32_t* ggu_indices, int nbls, void* gains, int nants,
                void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata, double conv_crit,
                int maxiter, int check_every, int check_after, double gain, void* chisq, uint32_t* iters,
                void* conv);
+/* hv_omnical with the inputs uploaded once and the solve run 1 + reps times
+ * from the same starting point: ms per solve (kernel only, mean of the last
+ * reps); the outputs are those of the last solve.  No reference counterpart
+ * (measurement). */
+int hv_bench_omnical(hv_ctx* ctx, int precision, const uint32_t* ggu_indices, int nbls, void* gains, int nants,
+                     void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata, double conv_crit,
+                     int maxiter, int check_every, int check_after, double gain, void* chisq, uint32_t* iters,
+                     void* conv, int reps, double* ms_per_solve);
+/* Process-wide cap on where hv_omnical keeps its per-baseline arrays (no
+ * reference counterpart; tests and tuning): -1 automatic (as much in LDS as
+ * fits), 0 all in device memory, 1 the weighted products in LDS, 2 products
+ * and data weights in LDS.  A cap above what fits is lowered to what fits. */
+int hv_set_omnical_place(int place);
 
 /* ---------------------------------------------------------------------------
  * 2-D median filter, replacing hera_gpu.median_filter.MedianFilter

@@ -6,7 +6,7 @@ against hera_cal, which is not installed."""
 import numpy as np
 import pytest
 
-from hera_gpu_amd.redcal import omnical
+from hera_gpu_amd.redcal import bench_omnical, omnical, set_placement
 from oracle import omnical_oracle as oo
 from oracle.omnical_oracle import redundant_array
 
@@ -52,6 +52,22 @@ def test_omnical_bitwise_equal_to_restatement_fp64():
         np.testing.assert_array_equal(gpu[key], ref[key], err_msg=key)
 
 
+@pytest.mark.parametrize("place", [0, 1, 2])
+def test_omnical_every_placement_bitwise_fp64(place):
+    # per-baseline arrays in device memory, products in LDS, or all in LDS:
+    # the same operations in the same order whichever is used
+    ggu, g0, u0, data, wgts = perturbed(5, n=5, nd=300)
+    kw = dict(maxiter=24, check_every=4, check_after=1, precision=2)
+    set_placement(place)
+    try:
+        gpu = omnical(ggu, g0, u0, data, wgts, **kw)
+    finally:
+        set_placement(-1)
+    ref = oo.omnical(ggu, g0, u0, data, wgts, **kw)
+    for key in ("gains", "ubls", "chisq", "iters", "conv"):
+        np.testing.assert_array_equal(gpu[key], ref[key], err_msg=f"{key} (placement {place})")
+
+
 def test_omnical_many_pixels_and_options():
     # more pixels than resident workgroups (each workgroup solves several),
     # convergence checks every 3 iterations from iteration 5, a zero-weight
@@ -77,3 +93,13 @@ def test_omnical_argument_checks():
         omnical(ggu, g0, u0, data, wgts, check_every=1)
     with pytest.raises(AssertionError):
         omnical(ggu, g0[:, :3], u0, data, wgts)
+
+
+def test_timed_solve_repeats_the_same_solve():
+    ggu, g0, u0, data, wgts = perturbed(6, n=4, nd=40)
+    kw = dict(maxiter=20, precision=2)
+    one = omnical(ggu, g0, u0, data, wgts, **kw)
+    timed = bench_omnical(ggu, g0, u0, data, wgts, reps=3, **kw)
+    assert 0 < timed["kernel_ms"] < 1e4
+    for key in ("gains", "ubls", "chisq", "iters", "conv"):
+        np.testing.assert_array_equal(timed[key], one[key], err_msg=key)
