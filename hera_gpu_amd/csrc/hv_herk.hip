@@ -87,8 +87,11 @@ int herk_tile(int mode, int nant) {
 // (x, y) pairs of 16-bit values -> (-y, x): rotate each 32-bit word by 16 and
 // flip the sign bit of the lower half.  Applied to the B operand b = (x, y)
 // of a split-mode contraction, a . rot(b) = a_x (-b_y) + a_y b_x, the
-// imaginary part of a conj(b).  Every split-mode kernel rotates B (never A),
-// so all of them feed the matrix cores the same operands.
+// imaginary part of a conj(b).  The 192-tile kernel rotates A instead where
+// that side has fewer fragments (rot_a16: the same two products in swapped
+// halves of each 32-bit pair); the matrix cores return the same bits either
+// way on gfx950, which the variant tests check (every split kernel's result is
+// bitwise equal to the others').
 __device__ inline u32x4 rot_neg16(u32x4 v) {
     u32x4 r;
 #pragma unroll
@@ -1633,13 +1636,25 @@ __device__ __forceinline__ void t192_mac(const u32x4& ahi, const u32x4& alo, con
     acc = mfma16<MODE>(alo, bhi, acc);
 }
 
-template <bool IM>
+// a' with a' . b = a . rot(b) = a_y b_x - a_x b_y: (a_y, -a_x) per 16-bit pair.
+__device__ inline u32x4 rot_a16(u32x4 v) {
+    u32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(v[i], v[i], 16) ^ 0x80000000u;
+    return r;
+}
+
+// Fragment of one 32-row block; ROT 1: rotated as B (rot_neg16), 2: as A (rot_a16).
+template <int ROT>
 __device__ __forceinline__ void t192_frag(const lds_char* S, int row, int gg, u32x4& hi, u32x4& lo) {
     hi = lds_ld<u32x4>(S + lds_off<true>(row, 2 * gg));
     lo = lds_ld<u32x4>(S + lds_off<true>(row, 2 * gg + 1));
-    if constexpr (IM) {
+    if constexpr (ROT == 1) {
         hi = rot_neg16(hi);
         lo = rot_neg16(lo);
+    } else if constexpr (ROT == 2) {
+        hi = rot_a16(hi);
+        lo = rot_a16(lo);
     }
 }
 
@@ -1654,15 +1669,15 @@ __device__ __forceinline__ void t192_quad_mma(const lds_char* As, const lds_char
         // fragments in order of first use (LDS reads return in order)
         u32x4 bhi[3], blo[3];
         u32x4 ahi, alo;
-        t192_frag<false>(As, ra0, gg, ahi, alo);
+        t192_frag<0>(As, ra0, gg, ahi, alo);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            t192_frag<IM>(Bs, rb0 + 32 * j, gg, bhi[j], blo[j]);
+            t192_frag<IM ? 1 : 0>(Bs, rb0 + 32 * j, gg, bhi[j], blo[j]);
             t192_mac<MODE>(ahi, alo, bhi[j], blo[j], acc[j]);
         }
 #pragma unroll
         for (int i = 1; i < 3; ++i) {
-            t192_frag<false>(As, ra0 + 32 * i, gg, ahi, alo);
+            t192_frag<0>(As, ra0 + 32 * i, gg, ahi, alo);
 #pragma unroll
             for (int j = 0; j < 3; ++j) t192_mac<MODE>(ahi, alo, bhi[j], blo[j], acc[3 * i + j]);
         }
@@ -1672,12 +1687,14 @@ __device__ __forceinline__ void t192_quad_mma(const lds_char* As, const lds_char
 // Diagonal tiles: block row R, columns C0..C1 into acc[K0 ...], one step.
 template <int MODE, bool IM, int R, int C0, int C1, int K0>
 __device__ __forceinline__ void t192_diag_row(const lds_char* S, int lrow, int gg, f32x16 (&acc)[9]) {
+    // Im: the one A fragment of the row is rotated instead of its B fragments
+    constexpr int RA = IM ? 2 : 0;
     u32x4 ahi, alo;
-    t192_frag<false>(S, 32 * R + lrow, gg, ahi, alo);
+    t192_frag<RA>(S, 32 * R + lrow, gg, ahi, alo);
 #pragma unroll
     for (int c = C0; c <= C1; ++c) {
         u32x4 bhi, blo;
-        t192_frag<IM>(S, 32 * c + lrow, gg, bhi, blo);
+        t192_frag<0>(S, 32 * c + lrow, gg, bhi, blo);
         t192_mac<MODE>(ahi, alo, bhi, blo, acc[K0 + c - C0]);
     }
 }
@@ -1720,20 +1737,23 @@ template <int MODE, bool IM, int R0, int R1, int C0, int C1>
 __device__ __forceinline__ void t192_rect_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
                                               f32x16 (&acc)[9]) {
     constexpr int NC = C1 - C0 + 1;
+    // Im: rotate the side with fewer fragments per step
+    constexpr bool ROTA = NC > R1 - R0 + 1;
+    constexpr int RA = (IM && ROTA) ? 2 : 0, RB = (IM && !ROTA) ? 1 : 0;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         const int gg = 2 * s + lh;
         u32x4 bhi[NC], blo[NC];
         u32x4 ahi, alo;
-        t192_frag<false>(As, 32 * R0 + lrow, gg, ahi, alo);
+        t192_frag<RA>(As, 32 * R0 + lrow, gg, ahi, alo);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            t192_frag<IM>(Bs, 32 * (C0 + c) + lrow, gg, bhi[c], blo[c]);
+            t192_frag<RB>(Bs, 32 * (C0 + c) + lrow, gg, bhi[c], blo[c]);
             t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[c]);
         }
 #pragma unroll
         for (int r = R0 + 1; r <= R1; ++r) {
-            t192_frag<false>(As, 32 * r + lrow, gg, ahi, alo);
+            t192_frag<RA>(As, 32 * r + lrow, gg, ahi, alo);
 #pragma unroll
             for (int c = 0; c < NC; ++c) t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[(r - R0) * NC + c]);
         }
