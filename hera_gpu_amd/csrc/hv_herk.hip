@@ -991,7 +991,7 @@ __device__ __forceinline__ void fused_issue(char* slot, const float* amp_st, con
 // rest to the B tile).
 template <int MODE, int TILE, int RPT>
 __device__ __forceinline__ void fused_build(const char* in, char* op, int lane, int g, const double (&ax)[RPT],
-                                            const double (&ay)[RPT], const double (&az)[RPT], double il) {
+                                            const double (&ay)[RPT], const double (&az)[RPT]) {
     using L = FusedLds<TILE>;
     const f64x2* cx = reinterpret_cast<const f64x2*>(in + L::AMP + g * 32);
     const f64x2* cy = reinterpret_cast<const f64x2*>(in + L::AMP + 1024 + g * 32);
@@ -1007,9 +1007,8 @@ __device__ __forceinline__ void fused_build(const char* in, char* op, int lane, 
         u32x4 hi, lo;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const double d = fma(ax[i], xs[j], fma(ay[i], ys[j], az[i] * zs[j]));
             float sn, cs;
-            sincos_turns_hw(d * il, &sn, &cs);
+            sincos_turns_hw(phase_turns(ax[i], ay[i], az[i], xs[j], ys[j], zs[j]), &sn, &cs);
             uint32_t h, l;
             split_phasor<MODE>(am[j], cs, sn, h, l);
             hi[j] = h;
@@ -1030,7 +1029,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
                                             const float* amp_t, const double* crd_t, size_t amp_stage, int s_begin,
                                             int n_st, const int* ampoff, int crdoff, int lane, int wave, int wr,
                                             int wc, const double (&ax)[RPT], const double (&ay)[RPT],
-                                            const double (&az)[RPT], double il, bool diag,
+                                            const double (&az)[RPT], bool diag,
                                             f32x16 (&tre)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS],
                                             f32x16 (&tim)[Cfg<MODE, TILE>::NS][Cfg<MODE, TILE>::NS]) {
     using C = Cfg<MODE, TILE>;
@@ -1062,7 +1061,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
     HV_FISSUE(in3, 3);
     wait_vmcnt<(FRING - 1) * PER>();
     __builtin_amdgcn_s_barrier();
-    fused_build<MODE, TILE, RPT>(in0, op0, lane, wave, ax, ay, az, il);
+    fused_build<MODE, TILE, RPT>(in0, op0, lane, wave, ax, ay, az);
 
     for (int st = 0; st < n_st;) {
         const int g_end = min(st + RFOLD, n_st);
@@ -1091,7 +1090,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
                 read_frag<NS>(fa, cur, cur + boff, ra0, rb0, lh);
                 read_frag<NS>(fb, cur, cur + boff, ra0, rb0, 2 + lh);
                 // operand stage st+k+1 (past the end: rebuilt from a re-read, unused)
-                fused_build<MODE, TILE, RPT>(ins[(k + 1) % FRING], nxt, lane, wave, ax, ay, az, il);
+                fused_build<MODE, TILE, RPT>(ins[(k + 1) % FRING], nxt, lane, wave, ax, ay, az);
                 if constexpr (KIND != 0) {
                     frag_mma<MODE, NS, true>(fa, 0u, re, im);
                     frag_mma<MODE, NS, true>(fb, 0u, re, im);
@@ -1190,28 +1189,28 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
 #pragma unroll
             for (int i = 0; i < RD; ++i) {
                 const int ant = min(bi * TILE + lane + 64 * i, a.nant - 1);
-                ax[i] = a.antpos[3 * ant]; ay[i] = a.antpos[3 * ant + 1]; az[i] = a.antpos[3 * ant + 2];
+                ax[i] = a.antpos[3 * ant] * il; ay[i] = a.antpos[3 * ant + 1] * il; az[i] = a.antpos[3 * ant + 2] * il;
             }
             if (smask)
                 fused_split<MODE, TILE, RD, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, true, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, true, tre, tim);
             else
                 fused_split<MODE, TILE, RD, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, true, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, true, tre, tim);
         } else {
             double ax[RO], ay[RO], az[RO];
 #pragma unroll
             for (int i = 0; i < RO; ++i) {
                 const int rr = lane + 64 * i;
                 const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
-                ax[i] = a.antpos[3 * ant]; ay[i] = a.antpos[3 * ant + 1]; az[i] = a.antpos[3 * ant + 2];
+                ax[i] = a.antpos[3 * ant] * il; ay[i] = a.antpos[3 * ant + 1] * il; az[i] = a.antpos[3 * ant + 2] * il;
             }
             if (smask)
                 fused_split<MODE, TILE, RO, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, false, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, false, tre, tim);
             else
                 fused_split<MODE, TILE, RO, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, il, false, tre, tim);
+                                               ampoff, crdoff, lane, wave, wr, wc, ax, ay, az, false, tre, tim);
         }
         if (!smask) continue;
 
@@ -1302,7 +1301,7 @@ __device__ __forceinline__ void fused8_issue(char* slot, const float* amp_st, co
 // input slot (as fused_build).
 template <int MODE, int RPT>
 __device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase, int g, const double (&ax)[RPT],
-                                             const double (&ay)[RPT], const double (&az)[RPT], double il) {
+                                             const double (&ay)[RPT], const double (&az)[RPT]) {
     using L = Fused8Lds;
     constexpr int TILE = 128;
     const double* cx = reinterpret_cast<const double*>(in + L::AMP + g * 32);
@@ -1318,9 +1317,8 @@ __device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase
         const double x = cx[j], y = cy[j], z = cz[j];      // one source, every row of the lane
 #pragma unroll
         for (int i = 0; i < RPT; ++i) {
-            const double d = fma(ax[i], x, fma(ay[i], y, az[i] * z));
             float sn, cs;
-            sincos_turns_hw(d * il, &sn, &cs);
+            sincos_turns_hw(phase_turns(ax[i], ay[i], az[i], x, y, z), &sn, &cs);
             uint32_t hw, lw;
             split_phasor<MODE>(am[i][j], cs, sn, hw, lw);
             hi[i][j] = hw;
@@ -1354,7 +1352,7 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
                                              const float* amp_t, const double* crd_t, size_t amp_stage,
                                              int s_begin, int n_st, const int* ampoff, int crdoff, int lane,
                                              int wave, const double (&ax)[RPT], const double (&ay)[RPT],
-                                             const double (&az)[RPT], double il, bool diag, f32x16 (&re)[2],
+                                             const double (&az)[RPT], bool diag, f32x16 (&re)[2],
                                              f32x16 (&im)[2]) {
     using L = Fused8Lds;
     constexpr int TILE = 128;
@@ -1385,7 +1383,7 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
     HV_F8ISSUE(in3, 3);
     wait_vmcnt<(FRING - 1) * PER>();
     __builtin_amdgcn_s_barrier();
-    fused8_build<MODE, RPT>(in0, op0, rbase, g, ax, ay, az, il);
+    fused8_build<MODE, RPT>(in0, op0, rbase, g, ax, ay, az);
 
     for (int st = 0; st < n_st; st += FRING) {
 #pragma unroll
@@ -1399,13 +1397,13 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
             // MFMAs in opposite order, so one's matrix work runs while the
             // other's VALU work does
             if (h == 0) {
-                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il);
+                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az);
                 __builtin_amdgcn_sched_barrier(0);
                 if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
             } else {
                 if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
                 __builtin_amdgcn_sched_barrier(0);
-                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az, il);
+                fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az);
             }
         }
     }
@@ -1474,27 +1472,27 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
         if (diag) {
             double ax[1], ay[1], az[1];
             const int ant = min(bi * TILE + h * 64 + lane, a.nant - 1);
-            ax[0] = a.antpos[3 * ant]; ay[0] = a.antpos[3 * ant + 1]; az[0] = a.antpos[3 * ant + 2];
+            ax[0] = a.antpos[3 * ant] * il; ay[0] = a.antpos[3 * ant + 1] * il; az[0] = a.antpos[3 * ant + 2] * il;
             if (smask)
                 fused8_split<MODE, 1, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il, true, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, true, re, im);
             else
                 fused8_split<MODE, 1, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il, true, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, true, re, im);
         } else {
             double ax[2], ay[2], az[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int rr = h * 128 + lane + 64 * i;
                 const int ant = min((rr < TILE ? bi * TILE + rr : bj * TILE + rr - TILE), a.nant - 1);
-                ax[i] = a.antpos[3 * ant]; ay[i] = a.antpos[3 * ant + 1]; az[i] = a.antpos[3 * ant + 2];
+                ax[i] = a.antpos[3 * ant] * il; ay[i] = a.antpos[3 * ant + 1] * il; az[i] = a.antpos[3 * ant + 2] * il;
             }
             if (smask)
                 fused8_split<MODE, 2, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il, false, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, false, re, im);
             else
                 fused8_split<MODE, 2, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, il, false, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, false, re, im);
         }
         if (!smask) continue;
 
@@ -1593,8 +1591,7 @@ struct T192Rows {
 // with the arithmetic of fused8_build; ax/ay/az: the rows' antenna positions.
 template <int MODE, int NR>
 __device__ __forceinline__ void t192_build(const lds_char* in, lds_char* op, int row0, int lane,
-                                           const double (&ax)[3], const double (&ay)[3], const double (&az)[3],
-                                           double il) {
+                                           const double (&ax)[3], const double (&ay)[3], const double (&az)[3]) {
     using L = T192;
     const int g = lane >> 4;
     double x[4], y[4], z[4];
@@ -1611,9 +1608,8 @@ __device__ __forceinline__ void t192_build(const lds_char* in, lds_char* op, int
         u32x4 hi, lo;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const double d = fma(ax[k], x[j], fma(ay[k], y[j], az[k] * z[j]));
             float sn, cs;
-            sincos_turns_hw(d * il, &sn, &cs);
+            sincos_turns_hw(phase_turns(ax[k], ay[k], az[k], x[j], y[j], z[j]), &sn, &cs);
             uint32_t hw, lw;
             split_phasor<MODE>(am[j], cs, sn, hw, lw);
             hi[j] = hw;
@@ -1808,15 +1804,15 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
     using RW = T192Rows<DIAG, IM>;
     constexpr int NR = RW::NR;
     const int row0 = RW::row0(wave);
-    // antenna positions of the rows this lane builds
+    // antenna positions of the rows this lane builds, in wavelengths
     double ax[3], ay[3], az[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int r = row0 + (lane & 15) + 16 * (k < NR ? k : 0);
         const int ant = min((r < L::TILE ? bi * L::TILE + r : bj * L::TILE + r - L::TILE), nant - 1);
-        ax[k] = antpos[3 * ant];
-        ay[k] = antpos[3 * ant + 1];
-        az[k] = antpos[3 * ant + 2];
+        ax[k] = antpos[3 * ant] * il;
+        ay[k] = antpos[3 * ant + 1] * il;
+        az[k] = antpos[3 * ant + 2] * il;
     }
     const int q = wave & 3;
     const int lrow = lane & 31, lh = lane >> 5;
@@ -1839,7 +1835,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
     wait_vmcnt<0>();
     __syncthreads();                       // stage 0's inputs are in LDS
     HV_T192ISSUE(in1, min(1, n_st - 1));   // a one-stage split re-reads stage 0 (never built)
-    t192_build<MODE, NR>(in0, op0, row0, lane, ax, ay, az, il);
+    t192_build<MODE, NR>(in0, op0, row0, lane, ax, ay, az);
     for (int st = 0; st < n_st; ++st) {
         const bool odd = st & 1;
         const lds_char* cur = odd ? op1 : op0;
@@ -1856,7 +1852,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         // both waves of a SIMD build, then both run their MFMAs (measured on
         // MI355X: 7% faster than staggering the build and the MFMAs of the
         // pair, whose LDS traffic then competes)
-        t192_build<MODE, NR>(bin, bop, row0, lane, ax, ay, az, il);
+        t192_build<MODE, NR>(bin, bop, row0, lane, ax, ay, az);
         __builtin_amdgcn_sched_barrier(0);
         HV_T192MMA(cur);
     }

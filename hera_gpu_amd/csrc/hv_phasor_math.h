@@ -76,6 +76,14 @@ __device__ inline void sincos_turns_hw(double t, float* s, float* c) {
     *c = __builtin_amdgcn_cosf(r);
 }
 
+// Delay phase in turns of a source at (x, y, z) for an antenna whose position
+// is already in wavelengths (metres times 1/lambda, rounded once): the dot
+// product in FP64.  Every split-mode operand builder forms the phase this
+// way, so their operands agree bit for bit.
+__device__ inline double phase_turns(double axl, double ayl, double azl, double x, double y, double z) {
+    return fma(axl, x, fma(ayl, y, azl * z));
+}
+
 // Round-to-nearest-even float -> bf16 / fp16 bit patterns and back.
 __device__ inline uint16_t to_bf16(float x) {
     __bf16 h = (__bf16)x;

@@ -71,17 +71,20 @@ constexpr int PH_THREADS = 256;
 constexpr int PH_SRC_PER_THREAD = 8;
 constexpr int PH_SRC_PER_WG = 8192;
 
-// Phasors of 8 consecutive sources of one antenna at one frequency (path
-// differences d in metres, amplitudes amp) written to their operand slots.
+constexpr bool split_mode(int mode) { return mode == MODE_SPLIT_BF16 || mode == MODE_SPLIT_FP16; }
+
+// Phasors of 8 consecutive sources of one antenna at one frequency written to
+// their operand slots: ph holds the delay phases in turns (split modes) or
+// the path differences in metres (FP32 / FP64: scaled by il4 here).
 template <int MODE, typename R>
-__device__ __forceinline__ void phasor_store(char* dst, const R* amp, const double* d, double il, double il4) {
-    if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
+__device__ __forceinline__ void phasor_store(char* dst, const R* amp, const double* ph, double il4) {
+    if constexpr (split_mode(MODE)) {
         // two 32-byte groups [4 hi words | 4 lo words]
         uint32_t hw[8], lw[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             float sn, cs;
-            sincos_turns_hw(d[j] * il, &sn, &cs);
+            sincos_turns_hw(ph[j], &sn, &cs);
             split_phasor<MODE>(amp[j], cs, sn, hw[j], lw[j]);
         }
         uint4* o = reinterpret_cast<uint4*>(dst);
@@ -94,7 +97,7 @@ __device__ __forceinline__ void phasor_store(char* dst, const R* amp, const doub
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             R sn, cs;
-            sincos_quarter_turns(d[j] * il4, &sn, &cs);
+            sincos_quarter_turns(ph[j] * il4, &sn, &cs);
             re[j] = amp[j] * cs;
             im[j] = amp[j] * sn;
         }
@@ -141,10 +144,11 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
             }
             const double il = a.inv_lambda[u];
             const double il4 = 4.0 * il;
+            const double axl = ax * il, ayl = ay * il, azl = az * il;
             for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
                  s0 += PH_THREADS * PH_SRC_PER_THREAD) {
                 R amp[8];
-                double d[8];
+                double ph[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const int64_t s = s0 + j;
@@ -152,11 +156,11 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
                     R A = (R)0;
                     if (z > 0.0) A = (R)beam_bilinear<BT>(sb, P, x, y);
                     amp[j] = A * (R)isq[s] * (R)a.amp_scale;
-                    d[j] = fma(ax, x, fma(ay, y, az * z));
+                    ph[j] = split_mode(MODE) ? phase_turns(axl, ayl, azl, x, y, z) : fma(ax, x, fma(ay, y, az * z));
                 }
                 phasor_store<MODE, R>(wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
                                           (size_t)(s0 % SST) * elem,
-                                      amp, d, il, il4);
+                                      amp, ph, il4);
             }
         }
         return;
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
          s0 += PH_THREADS * PH_SRC_PER_THREAD) {
         R amp[8];
-        double d[8];
+        double d[8], sx[8], sy[8], sz[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int64_t s = s0 + j;
@@ -173,14 +177,23 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
             R A = (R)0;
             if (z > 0.0) A = (R)beam_bilinear<BT>(sb, P, x, y);
             amp[j] = A * (R)isq[s] * (R)a.amp_scale;
-            d[j] = fma(ax, x, fma(ay, y, az * z));
+            sx[j] = x;
+            sy[j] = y;
+            sz[j] = z;
+            d[j] = fma(ax, x, fma(ay, y, az * z));   // FP32 / FP64: metres, shared by the frequencies
         }
         for (int u = 0; u < a.nfreq; ++u) {
             const double il = a.inv_lambda[u];          // turns per metre
             const double il4 = 4.0 * il;                // quarter turns per metre (exact scaling)
+            double ph[8];
+            if constexpr (split_mode(MODE)) {
+                const double axl = ax * il, ayl = ay * il, azl = az * il;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) ph[j] = phase_turns(axl, ayl, azl, sx[j], sy[j], sz[j]);
+            }
             phasor_store<MODE, R>(wunit + (size_t)u * a.unit_stride + operand_offset(s0 / SST, a.nant, ant) +
                                       (size_t)(s0 % SST) * elem,
-                                  amp, d, il, il4);
+                                  amp, split_mode(MODE) ? ph : d, il4);
         }
     }
 }
@@ -268,12 +281,15 @@ __device__ inline void beam_bilinear4(const BT* __restrict__ b, int P, double x,
     }
 }
 
+// sin and cos of the delay phase of antenna (ax, ay, az) m and source (x, y,
+// z) at 1/lambda = il, as the scalar-beam builders form it.
 template <int MODE, typename R>
-__device__ inline void phase_sincos(double d, double il, R* s, R* c) {
-    if constexpr (MODE == MODE_SPLIT_BF16 || MODE == MODE_SPLIT_FP16) {
-        sincos_turns_hw(d * il, s, c);
+__device__ inline void phase_sincos(double ax, double ay, double az, double x, double y, double z, double il, R* s,
+                                    R* c) {
+    if constexpr (split_mode(MODE)) {
+        sincos_turns_hw(phase_turns(ax * il, ay * il, az * il, x, y, z), s, c);
     } else {
-        sincos_quarter_turns(d * (4.0 * il), s, c);
+        sincos_quarter_turns(fma(ax, x, fma(ay, y, az * z)) * (4.0 * il), s, c);
     }
 }
 
@@ -308,7 +324,7 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_pol_kernel(PhasorPolArgs a)
     const int64_t s_hi = min(s_lo + (int64_t)PP_SRC_PER_WG, a.kpad);
     for (int64_t s0 = s_lo + (int64_t)threadIdx.x * 4; s0 < s_hi; s0 += PH_THREADS * 4) {
         R jr[4][2], ji[4][2], amp[4];
-        double d[4];
+        double sx[4], sy[4], sz[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int64_t s = s0 + j;
@@ -317,7 +333,9 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_pol_kernel(PhasorPolArgs a)
             if (z > 0.0) beam_bilinear4<BT>(sb, P, x, y, bj);
             jr[j][0] = bj[0]; ji[j][0] = bj[1]; jr[j][1] = bj[2]; ji[j][1] = bj[3];
             amp[j] = (R)a.isqrt[s] * (R)a.amp_scale;
-            d[j] = fma(ax, x, fma(ay, y, az * z));
+            sx[j] = x;
+            sy[j] = y;
+            sz[j] = z;
         }
         const int64_t col0 = 2 * s0;                      // 8 columns, all in one stage (SST >= 8)
         for (int u = 0; u < a.nfreq; ++u) {
@@ -326,7 +344,7 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_pol_kernel(PhasorPolArgs a)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 R sn, cs;
-                phase_sincos<MODE, R>(d[j], il, &sn, &cs);
+                phase_sincos<MODE, R>(ax, ay, az, sx[j], sy[j], sz[j], il, &sn, &cs);
                 const R er = amp[j] * cs, ei = amp[j] * sn;
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
