@@ -31,7 +31,7 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 enum Buf {
     B_ANTPOS, B_INVLAM, B_EQ2TOPS, B_CRD_EQ, B_ISQRT, B_BEAM, B_CRD_TOP, B_W, B_PART,
-    B_OUT, B_AUX, B_DYN, B_COUNT
+    B_OUT, B_AUX, B_DYN, B_EXT, B_COUNT   // B_EXT: Omnical and the median filter
 };
 
 struct DevBuf {
@@ -63,6 +63,8 @@ struct hv_ctx {
     std::mutex mu;       // guards `ws` and calls on `stream`
     std::mutex mem_mu;   // guards `used`
     hv::Workspace ws;
+    void* stage[2] = {};           // pinned staging buffers for copies of caller memory
+    hipEvent_t stage_ev[2] = {};   // last DMA that used each staging buffer
 };
 
 struct hv_sky {
@@ -89,6 +91,14 @@ void dev_free(hv_ctx* c, void* p, size_t bytes);
 // releasing an old buffer).  Returns whether the buffer moved in *moved.
 int ensure(hv_ctx* c, Workspace& ws, Buf b, size_t bytes, hipStream_t stream, bool* moved = nullptr);
 void release(hv_ctx* c, Workspace& ws);
+// Copies between caller (pageable) host memory and the device on stream s.
+// Large ones go through the context's two pinned staging buffers: the host
+// side of each chunk is spread over threads while the DMA of the other runs.
+// copy_h2d returns once the source may be reused; copy_d2h once dst holds the
+// data (it waits for the stream's earlier work).  Calls on one context are
+// serialised by its lock.
+int copy_h2d(hv_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s);
+int copy_d2h(hv_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s);
 
 float fp16_scale(double max_amp);
 

@@ -330,21 +330,13 @@ extern "C" int hv_medfilt2d(hv_ctx* c, const float* images, float* out, int nimg
     HV_HIP(hipSetDevice(c->device));
     const size_t bytes = (size_t)nimg * nrows * ncols * sizeof(float);
     const size_t slice = (bytes + 255) / 256 * 256;
-    char* base = nullptr;
-    HV_CHECK(dev_alloc(c, (void**)&base, 2 * slice));
+    HV_CHECK(ensure(c, c->ws, B_EXT, 2 * slice, c->stream));
+    char* base = static_cast<char*>(c->ws.buf[B_EXT].p);
     float* d_in = (float*)base;
     float* d_out = (float*)(base + slice);
-    auto run = [&]() -> int {
-        HV_HIP(hipMemcpyAsync(d_in, images, bytes, hipMemcpyHostToDevice, c->stream));
-        HV_CHECK(launch_medfilt(d_in, d_out, nimg, nrows, ncols, wrows, wcols, c->stream));
-        HV_HIP(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, c->stream));
-        HV_HIP(hipStreamSynchronize(c->stream));
-        return HV_OK;
-    };
-    const int rc = run();
-    if (rc != HV_OK) (void)hipStreamSynchronize(c->stream);
-    dev_free(c, base, 2 * slice);
-    return rc;
+    HV_CHECK(copy_h2d(c, d_in, images, bytes, c->stream));
+    HV_CHECK(launch_medfilt(d_in, d_out, nimg, nrows, ncols, wrows, wcols, c->stream));
+    return copy_d2h(c, out, d_out, bytes, c->stream);
 }
 
 extern "C" int hv_bench_medfilt2d(hv_ctx* c, const float* images, int nimg, int nrows, int ncols, int wrows,
@@ -355,15 +347,15 @@ extern "C" int hv_bench_medfilt2d(hv_ctx* c, const float* images, int nimg, int 
     HV_HIP(hipSetDevice(c->device));
     const size_t bytes = (size_t)nimg * nrows * ncols * sizeof(float);
     const size_t slice = (bytes + 255) / 256 * 256;
-    char* base = nullptr;
-    HV_CHECK(dev_alloc(c, (void**)&base, 2 * slice));
+    HV_CHECK(ensure(c, c->ws, B_EXT, 2 * slice, c->stream));
+    char* base = static_cast<char*>(c->ws.buf[B_EXT].p);
     float* d_in = (float*)base;
     float* d_out = (float*)(base + slice);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto run = [&]() -> int {
         HV_HIP(hipEventCreate(&e0));
         HV_HIP(hipEventCreate(&e1));
-        HV_HIP(hipMemcpyAsync(d_in, images, bytes, hipMemcpyHostToDevice, c->stream));
+        HV_CHECK(copy_h2d(c, d_in, images, bytes, c->stream));
         HV_CHECK(launch_medfilt(d_in, d_out, nimg, nrows, ncols, wrows, wcols, c->stream));   // warm-up
         HV_HIP(hipEventRecord(e0, c->stream));
         for (int r = 0; r < reps; ++r)
@@ -379,6 +371,5 @@ extern "C" int hv_bench_medfilt2d(hv_ctx* c, const float* images, int nimg, int 
     (void)hipStreamSynchronize(c->stream);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    dev_free(c, base, 2 * slice);
     return rc;
 }

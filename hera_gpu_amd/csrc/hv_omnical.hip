@@ -355,17 +355,16 @@ int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, 
     const size_t b_data = (size_t)ndata * nbls * 2 * sizeof(R), b_wgts = (size_t)ndata * nbls * sizeof(R);
     const size_t b_g = (size_t)ndata * nants * 2 * sizeof(R), b_u = (size_t)ndata * nubls * 2 * sizeof(R);
     const size_t b_sc = std::max<size_t>(1, (size_t)grid * nbls * nglobal) * sizeof(R);
-    char* base = nullptr;
     const size_t b_start = reps > 0 ? b_g + b_u + 512 : 0;   // starting points kept for repeated solves
     const size_t total = b_ggu + b_csr + b_data + b_wgts + b_g + b_u + b_sc + (size_t)ndata * (2 * sizeof(R) + 4) +
                          16 * 256 + b_start;   // + alignment of 10 slices
-    HV_CHECK(dev_alloc(c, (void**)&base, total));
+    HV_CHECK(ensure(c, c->ws, B_EXT, total, c->stream));
+    char* base = static_cast<char*>(c->ws.buf[B_EXT].p);
     auto carve = [&](size_t bytes) {
         char* p = base;
         base += (bytes + 255) / 256 * 256;
         return p;
     };
-    char* const mem = base;
     uint32_t* d_ggu = (uint32_t*)carve(b_ggu);
     int* d_csr = (int*)carve(b_csr);
     R* d_data = (R*)carve(b_data);
@@ -390,8 +389,8 @@ int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, 
     auto run = [&]() -> int {
         HV_HIP(hipMemcpyAsync(d_ggu, ggu, b_ggu, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(d_csr, all.data(), all.size() * 4, hipMemcpyHostToDevice, s));
-        HV_HIP(hipMemcpyAsync(d_data, data, b_data, hipMemcpyHostToDevice, s));
-        HV_HIP(hipMemcpyAsync(d_wgts, wgts, b_wgts, hipMemcpyHostToDevice, s));
+        HV_CHECK(copy_h2d(c, d_data, data, b_data, s));
+        HV_CHECK(copy_h2d(c, d_wgts, wgts, b_wgts, s));
         HV_HIP(hipMemcpyAsync(d_g, gains, b_g, hipMemcpyHostToDevice, s));
         HV_HIP(hipMemcpyAsync(d_u, ubls, b_u, hipMemcpyHostToDevice, s));
         OmniArgs<R> a{};
@@ -463,7 +462,6 @@ int omnical_t(hv_ctx* c, const uint32_t* ggu, int nbls, void* gains, int nants, 
     if (rc != HV_OK) (void)hipStreamSynchronize(s);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    dev_free(c, mem, total);
     return rc;
 }
 

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -69,6 +70,86 @@ void hv::dev_free(hv_ctx* c, void* p, size_t bytes) {
     (void)hipFree(p);
     std::lock_guard<std::mutex> lk(c->mem_mu);
     c->used -= bytes;
+}
+
+// ------------------------------------------------------ staged host copies
+namespace {
+constexpr size_t STAGE_CHUNK = (size_t)64 << 20;
+constexpr size_t STAGE_MIN = (size_t)8 << 20;   // below: one plain hipMemcpyAsync
+
+// memcpy spread over up to 8 threads (pages of a fresh caller buffer fault in
+// on first touch, which also spreads over cores)
+void par_memcpy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t PER_MIN = (size_t)4 << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(8, hw), bytes / PER_MIN);
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = (bytes / nt + 4095) / 4096 * 4096;
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (size_t i = 0; i < nt; ++i) {
+        const size_t off = i * per;
+        if (off >= bytes) break;
+        const size_t n = std::min(per, bytes - off);
+        th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, n); });
+    }
+    for (auto& t : th) t.join();
+}
+
+int ensure_stage(hv_ctx* c) {
+    if (c->stage[0]) return HV_OK;
+    for (int k = 0; k < 2; ++k) {
+        HV_HIP(hipHostMalloc(&c->stage[k], STAGE_CHUNK, hipHostMallocDefault));
+        HV_HIP(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+    }
+    return HV_OK;
+}
+}  // namespace
+
+int hv::copy_h2d(hv_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes < STAGE_MIN) {
+        HV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return HV_OK;
+    }
+    HV_CHECK(ensure_stage(c));
+    int k = 0;
+    for (size_t off = 0; off < bytes; off += STAGE_CHUNK, k ^= 1) {
+        const size_t n = std::min(STAGE_CHUNK, bytes - off);
+        HV_HIP(hipEventSynchronize(c->stage_ev[k]));   // the buffer's last DMA (this call or an earlier one)
+        par_memcpy(c->stage[k], static_cast<const char*>(src) + off, n);
+        HV_HIP(hipMemcpyAsync(static_cast<char*>(dst) + off, c->stage[k], n, hipMemcpyHostToDevice, s));
+        HV_HIP(hipEventRecord(c->stage_ev[k], s));
+    }
+    return HV_OK;
+}
+
+int hv::copy_d2h(hv_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes < STAGE_MIN) {
+        HV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HV_HIP(hipStreamSynchronize(s));
+        return HV_OK;
+    }
+    HV_CHECK(ensure_stage(c));
+    for (int k = 0; k < 2; ++k) HV_HIP(hipEventSynchronize(c->stage_ev[k]));
+    const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+    auto issue = [&](size_t i) -> int {
+        const size_t off = i * STAGE_CHUNK, n = std::min(STAGE_CHUNK, bytes - off);
+        HV_HIP(hipMemcpyAsync(c->stage[i & 1], static_cast<const char*>(src) + off, n, hipMemcpyDeviceToHost, s));
+        HV_HIP(hipEventRecord(c->stage_ev[i & 1], s));
+        return HV_OK;
+    };
+    HV_CHECK(issue(0));
+    if (nch > 1) HV_CHECK(issue(1));
+    for (size_t i = 0; i < nch; ++i) {
+        const size_t off = i * STAGE_CHUNK, n = std::min(STAGE_CHUNK, bytes - off);
+        HV_HIP(hipEventSynchronize(c->stage_ev[i & 1]));
+        par_memcpy(static_cast<char*>(dst) + off, c->stage[i & 1], n);
+        if (i + 2 < nch) HV_CHECK(issue(i + 2));
+    }
+    return HV_OK;
 }
 
 int hv::ensure(hv_ctx* c, Workspace& ws, Buf b, size_t bytes, hipStream_t stream, bool* moved) {
@@ -145,6 +226,11 @@ extern "C" int hv_ctx_destroy(hv_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamSynchronize(c->stream2);
     release(c, c->ws);
+    for (int k = 0; k < 2; ++k) {
+        if (c->stage_ev[k]) (void)hipEventSynchronize(c->stage_ev[k]);
+        if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
+        if (c->stage[k]) (void)hipHostFree(c->stage[k]);
+    }
     for (auto e : c->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
@@ -588,8 +674,7 @@ extern "C" int hv_vis_ex(hv_ctx* c, int mode, int dtype, const void* antpos, int
                       : 1.0f;
     StageTimer tm;
     HV_CHECK(run_job(c, j, tm));
-    HV_HIP(hipMemcpyAsync(vis_out, j.vis, out_bytes, hipMemcpyDeviceToHost, c->stream));
-    HV_HIP(hipStreamSynchronize(c->stream));
+    HV_CHECK(copy_d2h(c, vis_out, j.vis, out_bytes, c->stream));
     return HV_OK;
 }
 
@@ -690,8 +775,7 @@ extern "C" int hv_vis_pol(hv_ctx* c, int mode, int dtype, const void* antpos, in
                       : 1.0f;
     StageTimer tm;
     HV_CHECK(run_pol_job(c, j, j.beam, tm));
-    HV_HIP(hipMemcpyAsync(vis_out, j.vis, out_bytes, hipMemcpyDeviceToHost, c->stream));
-    HV_HIP(hipStreamSynchronize(c->stream));
+    HV_CHECK(copy_d2h(c, vis_out, j.vis, out_bytes, c->stream));
     return HV_OK;
 }
 
@@ -855,8 +939,7 @@ extern "C" int hv_simulate(hv_ctx* c, int mode, const hv_sky* sky, int64_t nsrc_
         times->total_ms = total;
     }
     if (vis_out) {
-        HV_HIP(hipMemcpyAsync(vis_out, j.vis, out_bytes, hipMemcpyDeviceToHost, c->stream));
-        HV_HIP(hipStreamSynchronize(c->stream));
+        HV_CHECK(copy_d2h(c, vis_out, j.vis, out_bytes, c->stream));
     }
     return HV_OK;
 }
