@@ -60,6 +60,20 @@ def herk_flops(N, K, units):
     return 4.0 * N * (N + 1) * K * units
 
 
+def t192_mfma_blocks(N):
+    """32x32 MFMA blocks the 192-tile kernel runs per unit and 16-source stage
+    for each component: per tile (bi <= bj) 21 (diagonal) or 36 (off-diagonal)
+    blocks, 15 and 30 when at most five 32-row blocks of the last block row or
+    column hold antennas."""
+    nb = -(-N // 192)
+    total = 0
+    for bi in range(nb):
+        for bj in range(bi, nb):
+            five = N - bj * 192 <= 160
+            total += (15 if five else 21) if bi == bj else (30 if five else 36)
+    return total
+
+
 def rel_err(a, b):
     a = np.asarray(a, dtype=np.complex128)
     b = np.asarray(b, dtype=np.complex128)
@@ -385,6 +399,13 @@ def main():
         serving = serving_leg(args.serve_jobs, gemms, args.serve_slots, args.serve_rate)
 
     peak = PEAK_TFLOPS[args.gemm]
+    executed = {}
+    if tile == 192 and args.gemm in ("split_bf16", "split_fp16"):
+        # matrix-core flops the kernel executes: 3 products x 2 components x
+        # 32x32x16 per block and 16-source step pair, against the useful ones
+        per_useful = (t192_mfma_blocks(N) * 2 * 3 * 2 * 2 * 32 * 32 * 16) / (4.0 * N * (N + 1) * 16)
+        executed = {"mfma_flops_per_useful": per_useful, "executed_tflops": herk_tflops * per_useful,
+                    "executed_frac": herk_tflops * per_useful / peak}
     upl = units * args.steps / st["herk_n"]
     tr = MEASURED_TRAFFIC.get((args.workload, args.gemm, int(round(upl)), tile))
     traffic = ((tr["fetch_kib"] * tr["fetch_scale"] + tr["write_kib"] * tr["write_scale"]) * 1024
@@ -405,7 +426,8 @@ def main():
                      "flops_per_launch": flops_per_launch,
                      "operand_bytes_per_launch": (elem * N * K * upl) if pf != 5 else 0,
                      "flop_count": "useful 4 N (N+1) K per unit; peak = published dense MFMA rate of the "
-                                   "arithmetic (split-BF16 spends 3 BF16 products per useful product)"},
+                                   "arithmetic (split-BF16 spends 3 BF16 products per useful product)",
+                     **executed},
         "stages_ms_per_step": stage_ms,
         "phasor": {"kernel": stage1, "avg_launch_ms": phasor_avg_ms, "bytes_per_launch": phasor_bytes,
                    "GBps": phasor_bytes / (phasor_avg_ms / 1e3) / 1e9, "hbm_frac": phasor_bytes /
