@@ -161,6 +161,73 @@ def omnical_leg(n=11, ndata=2000, maxiter=50, cpu_pixels=8):
     return out
 
 
+def variants_leg(w, gemm, ntimes=4):
+    """The simulator's extensions on the same workload: a beam per frequency
+    (resident inputs, the first `ntimes` times against the one-beam run of
+    the same units), the baseline-major (T, Nbl, F) output against the full
+    matrices (host buffers in and out), and complex 2x2 Jones beams (host
+    buffers; 2N rows x 2K columns, the first two times)."""
+    from hera_gpu_amd import stages, vis_gpu, vis_gpu_baselines, vis_gpu_polarized
+
+    N, K, F = w.antpos.shape[0], w.crd_eq.shape[1], len(w.freqs)
+    T = min(ntimes, w.eq2tops.shape[0])
+    eq = w.eq2tops[:T]
+    flops = herk_flops(N, K, T * F)
+    out = {"times": T, "freqs": F}
+    sky = stages.Sky(w.crd_eq, w.I_sky)
+    one = stages.Beam(w.bm_cube)
+    scale = (1.0 + 0.2 * np.arange(F) / F).astype(np.float32)[:, None, None, None]
+    per_f = stages.Beam(w.bm_cube[None] * scale)
+    res = {}
+    for name, beam in (("one_beam", one), ("beam_per_freq", per_f)):
+        stages.simulate(sky, beam, w.antpos, w.freqs, eq[:1], gemm=gemm, want_output=False, dtype=np.float32)
+        _, st = stages.simulate(sky, beam, w.antpos, w.freqs, eq, gemm=gemm, timing=True, want_output=False,
+                                dtype=np.float32)
+        res[name] = st["total_ms"]
+    out["beam_per_freq"] = {"ms": res["beam_per_freq"], "tflops": flops / (res["beam_per_freq"] / 1e3) / 1e12,
+                            "one_beam_ms": res["one_beam"],
+                            "one_beam_tflops": flops / (res["one_beam"] / 1e3) / 1e12,
+                            "time": "resident inputs, whole pipeline"}
+    log(f"[bench] beam per frequency: {res['beam_per_freq']:.1f} ms ({out['beam_per_freq']['tflops']:.1f} "
+        f"TFLOP/s) vs one beam {res['one_beam']:.1f} ms, {T * F} units")
+    sky.close()
+    one.close()
+    per_f.close()
+
+    wall = {}
+    for name, fn in (("full", vis_gpu), ("baselines_tbf", vis_gpu_baselines)):
+        fn(w.antpos, w.freqs[:1], eq[:1], w.crd_eq, w.I_sky, w.bm_cube, gemm=gemm)
+        t0 = time.perf_counter()
+        r = fn(w.antpos, w.freqs, eq, w.crd_eq, w.I_sky, w.bm_cube, gemm=gemm)
+        wall[name] = (time.perf_counter() - t0, (r[0] if isinstance(r, tuple) else r).nbytes)
+    out["baseline_layout"] = {"full_s": wall["full"][0], "full_bytes": wall["full"][1],
+                              "tbf_s": wall["baselines_tbf"][0], "tbf_bytes": wall["baselines_tbf"][1],
+                              "tbf_tflops": flops / wall["baselines_tbf"][0] / 1e12,
+                              "time": "host buffers in and out (uploads and result copy included)"}
+    log(f"[bench] baseline-major output: {wall['baselines_tbf'][0]:.2f} s ({wall['baselines_tbf'][1] / 1e9:.2f} GB) "
+        f"vs full matrices {wall['full'][0]:.2f} s ({wall['full'][1] / 1e9:.2f} GB)")
+
+    Tp = min(2, T)
+    rng = np.random.default_rng(13)
+    P = w.bm_cube.shape[-1]
+    jones = np.zeros((N, 2, 2, P, P), dtype=np.complex64)
+    jones[:, 0, 0] = w.bm_cube
+    jones[:, 1, 1] = w.bm_cube
+    leak = (0.05 * np.exp(2j * np.pi * rng.random((N, 1, 1)))).astype(np.complex64)
+    jones[:, 0, 1] = leak * w.bm_cube
+    jones[:, 1, 0] = np.conj(leak) * w.bm_cube
+    vis_gpu_polarized(w.antpos, w.freqs[:1], eq[:1], w.crd_eq, w.I_sky, jones, gemm=gemm)
+    t0 = time.perf_counter()
+    vp = vis_gpu_polarized(w.antpos, w.freqs, eq[:Tp], w.crd_eq, w.I_sky, jones, gemm=gemm)
+    dt = time.perf_counter() - t0
+    pflops = 4.0 * (2 * N) * (2 * N + 1) * (2 * K) * Tp * F
+    out["polarized"] = {"seconds": dt, "tflops": pflops / dt / 1e12, "times": Tp, "output_bytes": vp.nbytes,
+                        "flop_count": "useful 4 (2N)(2N+1)(2K) per unit (feed-antenna rows x source-axis columns)",
+                        "time": "host buffers in and out; phasor build + LDS-ring contraction"}
+    log(f"[bench] polarized (Jones) beams: {dt:.2f} s for {Tp * F} units, {pflops / dt / 1e12:.1f} TFLOP/s")
+    return out
+
+
 MEDFILT_CASES = (((7000, 7000), 3), ((4000, 4000), 5), ((997, 997), 9))   # test_median_filter.py:8-63
 
 
@@ -263,6 +330,8 @@ def main():
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer vis_gpu timing")
     ap.add_argument("--no-omnical", action="store_true", help="skip the Omnical calibration leg")
     ap.add_argument("--no-medfilt", action="store_true", help="skip the median filter leg")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the per-frequency beam, baseline-layout and polarised-beam leg")
     ap.add_argument("--serve-jobs", type=int, default=120, help="serving stream length (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrivals per second (default: 70%% of the measured serving capacity)")
@@ -390,6 +459,7 @@ def main():
         cpu = cpu_baseline(w, nsrc, gpu_unit)
         log(f"[bench] cpu: {cpu['seconds']:.1f} s, {cpu['value']:.4f} TFLOP/s")
 
+    variants = None if args.no_variants else variants_leg(w, args.gemm)
     omni = None if args.no_omnical else omnical_leg()
     medf = None if args.no_medfilt else medfilt_leg()
 
@@ -438,6 +508,7 @@ def main():
         "dropin_pcie_inclusive": dropin,
         "cpu_baseline": cpu,
         "serving": serving,
+        "variants": variants,
         "omnical": omni,
         "medfilt": medf,
         "p99_latency_ms": serving[args.gemm]["p99_ms"] if serving else None,
