@@ -90,20 +90,27 @@ void par_memcpy(void* dst, const void* src, size_t bytes) {
     const size_t per = (bytes / nt + 4095) / 4096 * 4096;
     std::vector<std::thread> th;
     th.reserve(nt);
-    for (size_t i = 0; i < nt; ++i) {
-        const size_t off = i * per;
-        if (off >= bytes) break;
-        const size_t n = std::min(per, bytes - off);
-        th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, n); });
+    size_t done = 0;   // bytes handed to threads
+    try {
+        for (size_t i = 0; i < nt; ++i) {
+            const size_t off = i * per;
+            if (off >= bytes) break;
+            const size_t n = std::min(per, bytes - off);
+            th.emplace_back(
+                [=] { std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, n); });
+            done = off + n;
+        }
+    } catch (...) {
+        // no more threads: the rest on this one
     }
+    if (done < bytes) std::memcpy(static_cast<char*>(dst) + done, static_cast<const char*>(src) + done, bytes - done);
     for (auto& t : th) t.join();
 }
 
 int ensure_stage(hv_ctx* c) {
-    if (c->stage[0]) return HV_OK;
     for (int k = 0; k < 2; ++k) {
-        HV_HIP(hipHostMalloc(&c->stage[k], STAGE_CHUNK, hipHostMallocDefault));
-        HV_HIP(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+        if (!c->stage[k]) HV_HIP(hipHostMalloc(&c->stage[k], STAGE_CHUNK, hipHostMallocDefault));
+        if (!c->stage_ev[k]) HV_HIP(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
     }
     return HV_OK;
 }
