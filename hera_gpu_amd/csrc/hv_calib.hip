@@ -181,7 +181,7 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
         mode > HV_MODE_FP64)
         return fail(HV_ERR_ARG, "bad argument");
     if (tile == 192) return fail(HV_ERR_ARG, "192-tiles build their operands in place: time them through hv_simulate");
-    if (tile == 0) tile = herk_tile(mode, nant) == 192 ? 128 : herk_tile(mode, nant);   // the HBM-operand kernel
+    if (tile == 0) tile = herk_tile_operands(mode, nant);   // the HBM-operand kernel
     if (pf == 5) return fail(HV_ERR_ARG, "pf 5 builds its operands in place: time it through hv_simulate");
     if (pf == 0) pf = herk_prefetch(mode, tile) == 5 ? 4 : herk_prefetch(mode, tile);   // same MFMA loop
     std::lock_guard<std::mutex> lk(c->mu);

@@ -74,11 +74,16 @@ static int pf_choice(int mode) {
     return split_mode(mode) ? 5 : 1;
 }
 
-// Tile edge: 64 (FP32, FP64, small arrays), 128 (split modes above 64
-// antennas), 192 (split modes above 256 antennas; in-place builds only).
+// Tile edge: 64 (FP32, FP64, small arrays), 128 (split modes at 64 < N <= 128
+// and 192 < N <= 256), 192 (split modes at 128 < N <= 192 and above 256
+// antennas; in-place builds only).  Whole simulations on MI355X, split-BF16,
+// 2^20 sources: N = 160 258 TFLOP/s with 192-tiles against 120 with 128-tiles
+// (two 128-blocks, the second a quarter full); N = 128, 200, 256: 128-tiles
+// 208 / 164 / 258 against 168 / 112 / 183; N = 300: 248 against 176.
 // Overrides apply where the variant exists for the mode.
 int herk_tile(int mode, int nant) {
-    int t = split_mode(mode) ? (nant > 256 ? 192 : nant > 64 ? 128 : 64) : 64;
+    const bool t192 = nant > 256 || (nant > 128 && nant <= 192);
+    int t = split_mode(mode) ? (t192 ? 192 : nant > 64 ? 128 : 64) : 64;
     if (g_tile_override && (g_tile_override == 64 || split_mode(mode))) t = g_tile_override;
     if (t == 192 && pf_choice(mode) != 5) t = 128;
     return t;
@@ -2136,6 +2141,14 @@ int herk_prefetch(int mode, int tile) {
 
 // Contraction of operands already in HBM (the built-in variant, or with
 // pf 5, which builds operands in place, the LDS ring on a persistent grid).
+// Tile edge for contracting operands already in device memory (LDS-ring and
+// register-staged kernels): the in-place choice, with 192 (in place only)
+// taken as 128.
+int herk_tile_operands(int mode, int nant) {
+    const int t = herk_tile(mode, nant);
+    return t == 192 ? 128 : t;
+}
+
 hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {
     const int pf = herk_prefetch(mode, tile);
     return launch_herk_variant(mode, tile, pf == 5 ? 4 : pf, a, s);

@@ -213,6 +213,7 @@ hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s);
 bool herk_fused(int mode, int nant);   // does the built-in choice build operands in place?
 
 int herk_tile(int mode, int nant);
+int herk_tile_operands(int mode, int nant);
 
 hipError_t launch_coords(const CoordsArgs& a, hipStream_t s);
 hipError_t launch_phasor(int mode, const PhasorArgs& a, hipStream_t s);

@@ -325,8 +325,7 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.nfreq = nfreq;
     p.kpad = kpad;
     p.fused = herk_fused(mode, nant) && !operands;
-    p.tile = herk_tile(mode, nant);
-    if (!p.fused && p.tile == 192) p.tile = 128;   // 192-tiles exist only as in-place builds
+    p.tile = p.fused ? herk_tile(mode, nant) : herk_tile_operands(mode, nant);
     p.nb = (nant + p.tile - 1) / p.tile;
     p.ntiles = p.nb * (p.nb + 1) / 2;
     p.nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
@@ -1107,7 +1106,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     }
     SplitArgs sa{c->ws.buf[B_AUX].p, c->ws.buf[B_W].p, nsrc, kpad, nant, scale};
     HV_HIP(launch_split(mode, sa, c->stream));
-    const int tile = herk_tile(mode, nant);
+    const int tile = herk_tile_operands(mode, nant);
     const int nb = (nant + tile - 1) / tile;
     const int ntiles = nb * (nb + 1) / 2;
     const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);

@@ -71,3 +71,17 @@ def test_bad_precision_raises_assertion():
     with pytest.raises(AssertionError):
         vis_gpu(np.zeros((2, 3)), 0.15, np.zeros((1, 3, 3)), np.zeros((3, 4)), np.zeros(4),
                 np.zeros((2, 3, 3)), precision=2, gemm="split_bf16")
+
+
+@pytest.mark.parametrize("nant,tile", [(1, 64), (37, 64), (64, 64), (65, 128), (128, 128), (129, 192), (160, 192),
+                                       (192, 192), (193, 128), (256, 128), (257, 192), (350, 192)])
+def test_contraction_tile_choice(nant, tile):
+    # host-side choice, no GPU call: split modes take 192-tiles where their
+    # last block is fuller than a 128-tile's would be
+    import ctypes as C
+    lib = _lib.load()
+    t, p = C.c_int(), C.c_int()
+    assert lib.hv_herk_choice(_lib.MODES["split_bf16"], nant, C.byref(t), C.byref(p)) == 0
+    assert (t.value, p.value) == (tile, 5)
+    assert lib.hv_herk_choice(_lib.MODES["fp32"], nant, C.byref(t), C.byref(p)) == 0
+    assert t.value == 64

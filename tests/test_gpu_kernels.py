@@ -126,7 +126,8 @@ def test_phasor_realistic_vs_oracle(gemm):
 
 # ------------------------------------------------------- contraction
 @pytest.mark.parametrize("gemm", MODES)
-@pytest.mark.parametrize("N,K", [(1, 1), (5, 100), (37, 1000), (64, 8192), (65, 8193), (130, 20000)])
+@pytest.mark.parametrize("N,K", [(1, 1), (5, 100), (37, 1000), (64, 8192), (65, 8193), (130, 20000),
+                                 (300, 9000)])
 def test_herk_vs_numpy(gemm, N, K):
     rng = np.random.default_rng(N * 100003 + K)
     v = (rng.standard_normal((N, K)) + 1j * rng.standard_normal((N, K))) * rng.uniform(0.01, 3, K)

@@ -213,18 +213,20 @@ def test_batch_schedules_agree_bitwise(gemm, monkeypatch):
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("tile", [64, 128, 192])
-def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile):
+@pytest.mark.parametrize("tile,nant", [(64, 300), (128, 300), (192, 300), (192, 160), (192, 129)])
+def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile, nant):
     # pf 5 builds the operand stages inside the contraction from A sqrt(I);
     # pf 4 contracts the stages phasor_kernel wrote: same operand bits, same
     # MFMA sequence per 32x32 block, so the same visibilities bit for bit
     # (192-tiles exist only in place: they are checked against the 128-tile
     # ring, whose blocks run the same chains).  N = 300 leaves a partial
-    # last tile; K spans several 8192-source splits; 3 times x 4 frequencies
+    # last tile, N = 160 and 129 one diagonal 192-tile with five block rows
+    # (the default choice there); K spans several 8192-source splits; 3 times
+    # x 4 frequencies
     from hera_gpu_amd import _lib
     lib = _lib.load()
     w = configs.make_hera350(nsrc=3 * 8192 + 1000, ntimes=3, nfreq=4)
-    ap, bm = w.antpos[:300], w.bm_cube[:300]
+    ap, bm = w.antpos[:nant], w.bm_cube[:nant]
     outs = []
     try:
         for pf in (4, 5):
