@@ -213,7 +213,7 @@ def test_batch_schedules_agree_bitwise(gemm, monkeypatch):
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("tile,nant", [(64, 300), (128, 300), (192, 300), (192, 160), (192, 129)])
+@pytest.mark.parametrize("tile,nant", [(64, 300), (128, 300), (192, 300), (192, 160), (192, 129), (192, 400)])
 def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile, nant):
     # pf 5 builds the operand stages inside the contraction from A sqrt(I);
     # pf 4 contracts the stages phasor_kernel wrote: same operand bits, same
@@ -226,7 +226,14 @@ def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile, na
     from hera_gpu_amd import _lib
     lib = _lib.load()
     w = configs.make_hera350(nsrc=3 * 8192 + 1000, ntimes=3, nfreq=4)
-    ap, bm = w.antpos[:nant], w.bm_cube[:nant]
+    ap, bm = w.antpos, w.bm_cube
+    if nant > len(ap):
+        # N = 400: three 192-blocks, so full off-diagonal 192-tiles too; the
+        # extra antennas are copies of the first ones 500 m east
+        extra = nant - len(ap)
+        ap = np.concatenate([ap, ap[:extra] + np.array([500.0, 0.0, 0.0], dtype=ap.dtype)])
+        bm = np.concatenate([bm, bm[:extra]])
+    ap, bm = ap[:nant], bm[:nant]
     outs = []
     try:
         for pf in (4, 5):
