@@ -207,6 +207,7 @@ int hv_vis_pol(hv_ctx* ctx, int mode, int dtype, const void* antpos, int nant, c
  * Needs 5 (nants + nubls) + 256 values of LDS per workgroup (<= 160 KB); the
  * per-baseline arrays (weighted products, data weights) go to LDS when they
  * fit and at least two workgroups per CU stay resident, else to device memory.
+ * The device buffer stays with the context for later calls (as hv_medfilt2d).
  * ------------------------------------------------------------------------- */
 int hv_omnical(hv_ctx* ctx, int precision, const uint32_t* ggu_indices, int nbls, void* gains, int nants,
                void* ubls, int nubls, const void* data, const void* wgts, int64_t ndata, double conv_crit,
@@ -234,7 +235,9 @@ int hv_set_omnical_place(int place);
  * (r - wrows/2, c - wcols/2), zeros outside the image (the reference's zero
  * padding, :318).  Ranks follow IEEE totalOrder (-0 < +0, NaNs at the ends by
  * sign).  Window sides 1..127; odd sides up to 11 use compile-time selection
- * networks, other shapes a radix selection.  All images in one launch.
+ * networks, other shapes a radix selection.  All images in one launch; the
+ * device buffer (input and output of the batch) stays with the context for
+ * later calls, growing as needed, until hv_ctx_destroy.
  * ------------------------------------------------------------------------- */
 int hv_medfilt2d(hv_ctx* ctx, const float* images, float* out, int nimg, int nrows, int ncols, int wrows,
                  int wcols);
