@@ -1727,6 +1727,21 @@ __device__ __forceinline__ void t192_diag_mma(const lds_char* S, int lrow, int l
             t192_diag_row<MODE, IM, 3, 3, 4, 0>(S, lrow, gg, acc);
             t192_diag_row<MODE, IM, 4, 4, 4, 2>(S, lrow, gg, acc);
         }
+        // the 10 blocks of a diagonal tile with at most four block rows
+        // inside the array (N <= 128 at the last block), 3, 3, 2, 2 per SIMD
+        if constexpr (Q == 8) t192_diag_row<MODE, IM, 0, 0, 2, 0>(S, lrow, gg, acc);
+        if constexpr (Q == 9) {
+            t192_diag_row<MODE, IM, 0, 3, 3, 0>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 1, 1, 2, 1>(S, lrow, gg, acc);
+        }
+        if constexpr (Q == 10) {
+            t192_diag_row<MODE, IM, 1, 3, 3, 0>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 2, 2, 2, 1>(S, lrow, gg, acc);
+        }
+        if constexpr (Q == 11) {
+            t192_diag_row<MODE, IM, 2, 3, 3, 0>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 3, 3, 3, 1>(S, lrow, gg, acc);
+        }
     }
 }
 
@@ -1768,6 +1783,11 @@ __device__ __forceinline__ void t192_rect5_mma(const lds_char* As, const lds_cha
     if constexpr (Q == 1) t192_rect_mma<MODE, IM, 2, 3, 0, 3>(As, Bs, lrow, lh, acc);
     if constexpr (Q == 2) t192_rect_mma<MODE, IM, 4, 5, 0, 3>(As, Bs, lrow, lh, acc);
     if constexpr (Q == 3) t192_rect_mma<MODE, IM, 0, 5, 4, 4>(As, Bs, lrow, lh, acc);
+    // at most four B column blocks inside the array: 6 blocks per SIMD
+    if constexpr (Q == 4) t192_rect_mma<MODE, IM, 0, 1, 0, 2>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 5) t192_rect_mma<MODE, IM, 2, 3, 0, 2>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 6) t192_rect_mma<MODE, IM, 4, 5, 0, 2>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 7) t192_rect_mma<MODE, IM, 0, 5, 3, 3>(As, Bs, lrow, lh, acc);
 }
 
 // Block k of a wave's accumulator set -> (block row, block column) in the
@@ -1784,6 +1804,12 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
         case 7: br = k < 2 ? 3 : 4; bc = k < 2 ? 3 + k : 4; return k < 3;
         case 8: case 9: case 10: br = 2 * (dq - 8) + k / 4; bc = k % 4; return k < 8;
         case 11: br = k; bc = 4; return k < 6;
+        case 12: br = 0; bc = k; return k < 3;
+        case 13: br = k < 1 ? 0 : 1; bc = k < 1 ? 3 : k; return k < 3;
+        case 14: br = k < 1 ? 1 : 2; bc = k < 1 ? 3 : 2; return k < 2;
+        case 15: br = k < 1 ? 2 : 3; bc = 3; return k < 2;
+        case 16: case 17: case 18: br = 2 * (dq - 16) + k / 3; bc = k % 3; return k < 6;
+        case 19: br = k; bc = 3; return k < 6;
         default:
             br = (q >> 1) * 3 + k / 3;
             bc = (q & 1) * 3 + k % 3;
@@ -1795,7 +1821,9 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
 // q); DQ = 0..3: diagonal tile, block list DQ (21 blocks); DQ = 4..7:
 // diagonal tile with at most five block rows inside the array (15 blocks);
 // DQ = 8..11: off-diagonal tile with at most five B column blocks inside
-// the array (30 blocks).
+// the array (30 blocks); DQ = 12..15: diagonal tile with at most four block
+// rows inside (10 blocks); DQ = 16..19: off-diagonal tile with at most four
+// B column blocks inside (24 blocks).
 template <int MODE, bool IM, int DQ>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
                                            const double* antpos, int nant, int bi, int bj, const float* amp_t,
@@ -1803,8 +1831,10 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
                                            const int* ampoff, int crdoff, int lane, int wave, double il,
                                            f32x16 (&acc)[9]) {
     using L = T192;
-    constexpr bool DIAG = DQ >= 0 && DQ < 8;
-    constexpr bool RECT5 = DQ >= 8;
+    constexpr bool DIAG = (DQ >= 0 && DQ < 8) || (DQ >= 12 && DQ < 16);
+    constexpr bool RECT5 = (DQ >= 8 && DQ < 12) || DQ >= 16;
+    constexpr int QD = DQ < 8 ? DQ : DQ - 4;     // t192_diag_mma list
+    constexpr int QR = DQ < 12 ? DQ - 8 : DQ - 12;   // t192_rect5_mma list
     constexpr int NROW = DIAG ? L::TILE : L::ROWS;
     using RW = T192Rows<DIAG, IM>;
     constexpr int NR = RW::NR;
@@ -1831,8 +1861,8 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
                      ampoff, crdoff, wave, lane)
 #define HV_T192MMA(buf)                                                                   \
     do {                                                                                   \
-        if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? DQ : 0)>(buf, lrow, lh, acc);  \
-        else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? DQ - 8 : 0)>(buf, buf + L::OP, lrow, lh, acc); \
+        if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? QD : 0)>(buf, lrow, lh, acc);  \
+        else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, lrow, lh, acc); \
         else t192_quad_mma<MODE, IM>(buf, buf + L::OP, ra0, rb0, lh, acc);                 \
     } while (0)
 
@@ -1963,8 +1993,12 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         // of the tile hold antennas
         // off-diagonal tiles: the 30-block lists when at most five 32-column
         // blocks of the B block hold antennas
-        const bool five = a.nant - bj * TILE <= 160;
-        const int dq = diag ? (five ? 4 : 0) + q : (five ? 8 + q : -1);
+        // the lists that skip blocks past the last antenna: at most five or
+        // four 32-row blocks of the last block (diagonal tiles) or of the B
+        // block (off-diagonal tiles) hold antennas
+        const int rem = a.nant - bj * TILE;
+        const bool five = rem <= 160, four = rem <= 128;
+        const int dq = diag ? (four ? 12 : five ? 4 : 0) + q : (four ? 16 + q : five ? 8 + q : -1);
         if (dq < 0) HV_T192BOTH(-1);
         else if (dq == 0) HV_T192BOTH(0);
         else if (dq == 1) HV_T192BOTH(1);
@@ -1977,7 +2011,15 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         else if (dq == 8) HV_T192BOTH(8);
         else if (dq == 9) HV_T192BOTH(9);
         else if (dq == 10) HV_T192BOTH(10);
-        else HV_T192BOTH(11);
+        else if (dq == 11) HV_T192BOTH(11);
+        else if (dq == 12) HV_T192BOTH(12);
+        else if (dq == 13) HV_T192BOTH(13);
+        else if (dq == 14) HV_T192BOTH(14);
+        else if (dq == 15) HV_T192BOTH(15);
+        else if (dq == 16) HV_T192BOTH(16);
+        else if (dq == 17) HV_T192BOTH(17);
+        else if (dq == 18) HV_T192BOTH(18);
+        else HV_T192BOTH(19);
 #undef HV_T192BOTH
 #undef HV_T192SPLIT
 
