@@ -1253,9 +1253,11 @@ struct Frag21 {
     u32x4 ahi[2], alo[2], bhi, blo;
 };
 
+// NA: A row blocks read (2, or 1 for a wave that owns one block)
+template <int NA = 2>
 __device__ __forceinline__ void read_frag21(Frag21& f, const char* As, const char* Bs, int ra0, int rb0, int g) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NA; ++i) {
         f.ahi[i] = *reinterpret_cast<const u32x4*>(As + lds_off<true>(ra0 + 32 * i, 2 * g));
         f.alo[i] = *reinterpret_cast<const u32x4*>(As + lds_off<true>(ra0 + 32 * i, 2 * g + 1));
     }
@@ -1265,12 +1267,12 @@ __device__ __forceinline__ void read_frag21(Frag21& f, const char* As, const cha
 
 // Same chain per 32x32 block as frag_mma: Re += hi.hi + hi.lo + lo.hi and
 // Im += a.b' with the same operand order.
-template <int MODE>
+template <int MODE, int NA = 2>
 __device__ __forceinline__ void frag21_mma(const Frag21& f, f32x16 (&re)[2], f32x16 (&im)[2]) {
     // one B fragment per step: rotating B costs half of rotating the two A fragments
     const u32x4 bhr = rot_neg16(f.bhi), blr = rot_neg16(f.blo);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NA; ++i) {
         re[i] = mfma16<MODE>(f.ahi[i], f.bhi, re[i]);
         im[i] = mfma16<MODE>(f.ahi[i], bhr, im[i]);
         re[i] = mfma16<MODE>(f.ahi[i], f.blo, re[i]);
@@ -1341,33 +1343,51 @@ __device__ __forceinline__ void fused8_build(const char* in, char* op, int rbase
 }
 
 // The MFMAs of one stage for one wave: both 8-source steps, the second
-// step's fragments read while the first step's MFMAs run.
-template <int MODE>
+// step's fragments read while the first step's MFMAs run.  NA: the wave's
+// row blocks (1: only block row ra0).
+template <int MODE, int NA = 2>
 __device__ __forceinline__ void fused8_mma(const char* As, const char* Bs, int ra0, int rb0, int lh,
                                            f32x16 (&re)[2], f32x16 (&im)[2]) {
     Frag21 f0, f1;
-    read_frag21(f0, As, Bs, ra0, rb0, lh);
-    read_frag21(f1, As, Bs, ra0, rb0, 2 + lh);   // in flight during the first step's MFMAs
-    frag21_mma<MODE>(f0, re, im);
-    frag21_mma<MODE>(f1, re, im);
+    read_frag21<NA>(f0, As, Bs, ra0, rb0, lh);
+    read_frag21<NA>(f1, As, Bs, ra0, rb0, 2 + lh);   // in flight during the first step's MFMAs
+    frag21_mma<MODE, NA>(f0, re, im);
+    frag21_mma<MODE, NA>(f1, re, im);
 }
 
+// Diagonal 128-tiles: the 10 blocks (r, c), r <= c, of the 4x4 grid of
+// 32x32 blocks as 3, 3, 2, 2 per SIMD (SIMD = wave % 4) instead of the
+// quadrant layout's 2, 2, 4, 4 (a wave runs its blocks' MFMAs even when one
+// of its two is below the diagonal).  Wave -> block rows rp .. rp + nb - 1
+// of block column cb; nb = 0: no blocks.
+__device__ inline int fused8_diag_blocks(int wave, int& rp, int& cb) {
+    if (wave < 4) {                 // pairs (0-1, 1), (0-1, 2), (0-1, 3), (2-3, 3)
+        rp = wave == 3 ? 2 : 0;
+        cb = wave == 3 ? 3 : wave + 1;
+        return 2;
+    }
+    rp = wave == 5 ? 2 : 0;         // singles (0, 0) and (2, 2)
+    cb = rp;
+    return wave < 6 ? 1 : 0;
+}
+
+// KIND 0: build only; 1: MFMAs of the wave's two row blocks; 2: of its
+// first row block only.  ra0 / rb0: the lane's first A row / its B row.
 template <int MODE, int RPT, int KIND>
 __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, char* in3, char* op0, char* op1,
                                              const float* amp_t, const double* crd_t, size_t amp_stage,
                                              int s_begin, int n_st, const int* ampoff, int crdoff, int lane,
                                              int wave, const double (&ax)[RPT], const double (&ay)[RPT],
-                                             const double (&az)[RPT], bool diag, f32x16 (&re)[2],
-                                             f32x16 (&im)[2]) {
+                                             const double (&az)[RPT], bool diag, int ra0, int rb0,
+                                             f32x16 (&re)[2], f32x16 (&im)[2]) {
     using L = Fused8Lds;
     constexpr int TILE = 128;
     constexpr int PER = L::PER_STAGE;
     char* ins[FRING] = {in0, in1, in2, in3};
     char* ops[2] = {op0, op1};
     const int g = wave & 3, h = wave >> 2;
-    const int wr = wave >> 2, wc = wave & 3;
-    const int lrow = lane & 31, lh = lane >> 5;
-    const int ra0 = wr * 64 + lrow, rb0 = wc * 32 + lrow;
+    const int lh = lane >> 5;
+    constexpr int NA = KIND == 2 ? 1 : 2;
     const int boff = diag ? 0 : L::OP;
     // rows this lane builds: RPT = 2 (other tiles): h*128 + lane + 64 i;
     // RPT = 1 (diagonal tiles, A rows only): h*64 + lane
@@ -1404,9 +1424,9 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
             if (h == 0) {
                 fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az);
                 __builtin_amdgcn_sched_barrier(0);
-                if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
+                if constexpr (KIND != 0) fused8_mma<MODE, NA>(cur, cur + boff, ra0, rb0, lh, re, im);
             } else {
-                if constexpr (KIND != 0) fused8_mma<MODE>(cur, cur + boff, ra0, rb0, lh, re, im);
+                if constexpr (KIND != 0) fused8_mma<MODE, NA>(cur, cur + boff, ra0, rb0, lh, re, im);
                 __builtin_amdgcn_sched_barrier(0);
                 fused8_build<MODE, RPT>(ins[(k + 1) % FRING], nxt, rbase, g, ax, ay, az);
             }
@@ -1452,14 +1472,17 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
         const float* amp_t = a.amp + (size_t)(a.amp_per_unit ? unit : tb) * amp_time;
         const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
 
+        // the wave's blocks: block rows rp, rp + 1 (nb of them) of column cb
+        int rp = 2 * wr, cb = wc, nb = 2;
+        if (diag) nb = fused8_diag_blocks(wave, rp, cb);
         unsigned smask = 0;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const int r0 = wr * 64 + i * 32, c0 = wc * 32;
-            const bool on = (bi * TILE + r0 < a.nant) && (bj * TILE + c0 < a.nant) && !(diag && r0 > c0);
+            const bool on = i < nb && (bi * TILE + (rp + i) * 32 < a.nant) && (bj * TILE + cb * 32 < a.nant);
             smask |= (unsigned)on << i;
         }
         smask = __builtin_amdgcn_readfirstlane(smask);
+        const int ra0 = rp * 32 + (lane & 31), rb0 = cb * 32 + (lane & 31);
 
         // amplitude chunk c = (i*8 + wave)*64 + lane: row c/4, part c%4
         int ampoff[L::NAMP];
@@ -1478,12 +1501,15 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
             double ax[1], ay[1], az[1];
             const int ant = min(bi * TILE + h * 64 + lane, a.nant - 1);
             ax[0] = a.antpos[3 * ant] * il; ay[0] = a.antpos[3 * ant + 1] * il; az[0] = a.antpos[3 * ant + 2] * il;
-            if (smask)
+            if (smask == 3)
                 fused8_split<MODE, 1, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, true, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, true, ra0, rb0, re, im);
+            else if (smask)   // only block row rp (the second is off the list or past the antennas)
+                fused8_split<MODE, 1, 2>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
+                                         ampoff, crdoff, lane, wave, ax, ay, az, true, ra0, rb0, re, im);
             else
                 fused8_split<MODE, 1, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, true, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, true, ra0, rb0, re, im);
         } else {
             double ax[2], ay[2], az[2];
 #pragma unroll
@@ -1494,10 +1520,10 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
             }
             if (smask)
                 fused8_split<MODE, 2, 1>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, false, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, false, ra0, rb0, re, im);
             else
                 fused8_split<MODE, 2, 0>(in0, in1, in2, in3, op0, op1, amp_t, crd_t, amp_stage, s_begin, n_st,
-                                         ampoff, crdoff, lane, wave, ax, ay, az, false, re, im);
+                                         ampoff, crdoff, lane, wave, ax, ay, az, false, ra0, rb0, re, im);
         }
         if (!smask) continue;
 
@@ -1511,8 +1537,8 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
             for (int e = 0; e < 16; ++e) {
                 const int r = 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
                 const int c = lane & 31;
-                const int row = wr * 64 + i * 32 + r;
-                const int col = wc * 32 + c;
+                const int row = (rp + i) * 32 + r;
+                const int col = cb * 32 + c;
                 out[(size_t)row * TILE + col] = re[i][e];
                 out[TSZ + (size_t)row * TILE + col] = im[i][e];
             }
