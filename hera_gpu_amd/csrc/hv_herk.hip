@@ -78,8 +78,9 @@ static int pf_choice(int mode) {
 // and 192 < N <= 256), 192 (split modes at 128 < N <= 192 and above 256
 // antennas; in-place builds only).  Whole simulations on MI355X, split-BF16,
 // 2^20 sources: N = 160 258 TFLOP/s with 192-tiles against 120 with 128-tiles
-// (two 128-blocks, the second a quarter full); N = 128, 200, 256: 128-tiles
-// 208 / 164 / 258 against 168 / 112 / 183; N = 300: 248 against 176.
+// (two 128-blocks, the second a quarter full); N = 96, 128, 200, 256:
+// 128-tiles 148 / 240 / 186 / 271 against 108 / 190 / 118 / 192; N = 300:
+// 261 against 184; N = 37 and 64: 64-tiles 45 / 130 against 22 / 67.
 // Overrides apply where the variant exists for the mode.
 int herk_tile(int mode, int nant) {
     const bool t192 = nant > 256 || (nant > 128 && nant <= 192);
@@ -95,8 +96,9 @@ int herk_tile(int mode, int nant) {
 // imaginary part of a conj(b).  The 192-tile kernel rotates A instead where
 // that side has fewer fragments (rot_a16: the same two products in swapped
 // halves of each 32-bit pair); the matrix cores return the same bits either
-// way on gfx950, which the variant tests check (every split kernel's result is
-// bitwise equal to the others').
+// way on gfx950, which the variant tests check (the 128- and 192-tile
+// kernels' results are bitwise equal; 64-tiles fold their accumulators every
+// FOLD_SRC sources, so their bits differ at the 1e-6 level).
 __device__ inline u32x4 rot_neg16(u32x4 v) {
     u32x4 r;
 #pragma unroll
