@@ -150,6 +150,38 @@ def ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+_PAR_BYTES = 1 << 24        # arrays below 16 MB convert in the calling thread
+_NPAR = min(8, os.cpu_count() or 1)
+_pool = None
+
+
+def contiguous(a, dtype) -> np.ndarray:
+    """np.ascontiguousarray(a, dtype), with a large conversion (complex128 ->
+    complex64 data of a calibration call, say) split by rows over up to 8
+    threads: numpy's cast loops release the GIL, and one thread converts at
+    a fraction of the bandwidth that the later upload runs at."""
+    a = np.asarray(a)
+    dtype = np.dtype(dtype)
+    if (a.dtype == dtype and a.flags.c_contiguous) or a.ndim == 0 or a.shape[0] < 16 \
+            or a.size * max(a.itemsize, dtype.itemsize) < _PAR_BYTES:
+        return np.ascontiguousarray(a, dtype=dtype)
+    global _pool
+    with _lock:
+        if _pool is None:
+            from concurrent.futures import ThreadPoolExecutor
+            _pool = ThreadPoolExecutor(max_workers=_NPAR, thread_name_prefix="hv-cast")
+        pool = _pool
+    out = np.empty(a.shape, dtype=dtype)
+    n = _NPAR
+    edges = [a.shape[0] * i // n for i in range(n + 1)]
+
+    def cast(i):
+        out[edges[i]:edges[i + 1]] = a[edges[i]:edges[i + 1]]
+
+    list(pool.map(cast, range(n)))
+    return out
+
+
 class Context:
     """One device, stream and workspace (wraps hv_ctx)."""
 

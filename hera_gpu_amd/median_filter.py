@@ -25,7 +25,7 @@ def _window(kernel_size) -> tuple[int, int]:
 
 def medfilt2d(images, kernel_size=3, *, device=0) -> np.ndarray:
     """Filter a (N, M) image or an (B, N, M) batch; returns float32 of the same shape."""
-    x = np.ascontiguousarray(images, dtype=np.float32)     # median_filter.py:318 casts to float32
+    x = _lib.contiguous(images, np.float32)     # median_filter.py:318 casts to float32
     one = x.ndim == 2
     if one:
         x = x[None]
@@ -52,7 +52,7 @@ def MedianFilter(input=None, kernel_size=3, bw=32, bh=32):
 
 def bench_medfilt2d(images, kernel_size=3, reps=10, *, device=0) -> float:
     """Milliseconds per launch of the filter over a batch already in HBM."""
-    x = np.ascontiguousarray(images, dtype=np.float32)
+    x = _lib.contiguous(images, np.float32)
     if x.ndim == 2:
         x = x[None]
     wr, wc = _window(kernel_size)

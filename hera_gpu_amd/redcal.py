@@ -47,8 +47,8 @@ def omnical(ggu_indices, gains, ubls, data, wgts, conv_crit=1e-10, maxiter=50, c
     assert check_every > 1
     rd, cd = (np.float32, np.complex64) if precision == 1 else (np.float64, np.complex128)
     ggu = np.ascontiguousarray(ggu, dtype=np.uint32)
-    data = np.ascontiguousarray(data, dtype=cd)
-    wgts = np.ascontiguousarray(wgts, dtype=rd)
+    data = _lib.contiguous(data, cd)
+    wgts = _lib.contiguous(wgts, rd)
     g = np.array(gains, dtype=cd, order="C", copy=True)
     u = np.array(ubls, dtype=cd, order="C", copy=True)
     chisq = np.empty(ndata, dtype=rd)
