@@ -338,3 +338,27 @@ def test_per_frequency_beam_resident_and_checks():
         sky.close()
     with pytest.raises(AssertionError):
         vis_gpu(w.antpos, freqs[:1], w.eq2tops, w.crd_eq, w.I_sky, bm4)
+
+
+def test_diffuse_sky_config_vs_oracle():
+    # the diffuse-sky configuration (equal-area ring pixels, exponentiated
+    # Gaussian random field, HERA-350, frequencies across 100-200 MHz) at
+    # nside 16 (3072 pixels); every frequency of a time goes in one
+    # contraction launch (fewer launches than times when times batch too)
+    w = configs.make_diffuse(nside=16, ntimes=2, nfreq=4, lmax=24)
+    ref = vis_oracle(w.antpos, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube)
+    for gemm in ("split_bf16", "fp32"):
+        v = vis_gpu(w.antpos, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, gemm=gemm)
+        errs = [rel_err(v[f], ref[f]) for f in range(len(w.freqs))]
+        assert max(errs) < VIS_TOL[gemm], (gemm, errs)
+    sky = stages.Sky(w.crd_eq, w.I_sky)
+    beam = stages.Beam(w.bm_cube)
+    try:
+        a, st = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops, gemm="split_bf16", timing=True,
+                                dtype=np.float32)
+    finally:
+        beam.close()
+        sky.close()
+    assert 1 <= st["herk_n"] <= len(w.eq2tops)
+    np.testing.assert_array_equal(a, vis_gpu(w.antpos, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube,
+                                             gemm="split_bf16"))
