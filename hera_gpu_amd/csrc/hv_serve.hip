@@ -14,7 +14,10 @@
 // replay does exactly the arithmetic of a stand-alone run and results are
 // bitwise identical alone and in a mix.  A copier thread moves results from
 // pinned memory to the caller's array, so large copies do not hold up
-// admission.
+// admission.  Toucher threads pre-fault the caller's output pages from
+// submission on (one write per 4 KiB page), so the final copy, which would
+// otherwise pay for first-touch faults and page zeroing, runs at memory speed
+// while the device works on this or earlier jobs.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -74,6 +77,10 @@ struct SrvJob {
     int graph_hit = 0;
     int skipped = 0;           // times later jobs were admitted ahead of it
     std::string err;
+    // output pre-faulting (all under hv_server::touch_mu)
+    size_t touch_next = 0;     // next output byte handed to a toucher
+    int touch_inflight = 0;    // chunks being touched now
+    bool touch_stop = false;   // no further chunks: the result copy or the job's end is due
 };
 
 enum SlotState { FREE, RUNNING, COPYING };
@@ -118,7 +125,52 @@ struct hv_server {
     bool stopping = false;
     hv_server_stats_t stats{};
     std::thread worker, copier;
+    // output pre-faulting (lock order: mu, then touch_mu; touchers take only touch_mu)
+    std::mutex touch_mu;
+    std::condition_variable touch_cv;
+    std::deque<SrvJob*> touch_q;       // jobs with output pages left to pre-fault
+    std::vector<std::thread> touchers;
+    bool touch_quit = false;
 };
+
+constexpr size_t TOUCH_CHUNK = (size_t)32 << 20;   // bytes a toucher takes at a time
+constexpr size_t TOUCH_MIN = (size_t)8 << 20;      // smaller outputs are not pre-faulted
+
+// Write one byte per 4 KiB page of the chunks of queued jobs' outputs, front
+// job first (all touchers share its chunks).
+static void toucher_main(hv_server* s) {
+    std::unique_lock<std::mutex> lk(s->touch_mu);
+    for (;;) {
+        s->touch_cv.wait(lk, [&] { return s->touch_quit || !s->touch_q.empty(); });
+        if (s->touch_quit) break;
+        SrvJob* j = s->touch_q.front();
+        if (j->touch_stop || j->touch_next >= j->out_bytes) {
+            s->touch_q.pop_front();
+            continue;
+        }
+        const size_t off = j->touch_next;
+        const size_t n = std::min(TOUCH_CHUNK, j->out_bytes - off);
+        j->touch_next += n;
+        if (j->touch_next >= j->out_bytes) s->touch_q.pop_front();
+        ++j->touch_inflight;
+        volatile char* p = static_cast<char*>(j->out) + off;
+        lk.unlock();
+        for (size_t b = 0; b < n; b += 4096) p[b] = 0;
+        lk.lock();
+        if (--j->touch_inflight == 0) s->touch_cv.notify_all();
+    }
+}
+
+// End the pre-faulting of a job's output: no chunk is handed out after this
+// returns and none is being written.  Called before the result copy and
+// before a job is reported finished (its caller may then free the array).
+static void stop_touch(hv_server* s, SrvJob* j) {
+    std::unique_lock<std::mutex> lk(s->touch_mu);
+    j->touch_stop = true;
+    auto it = std::find(s->touch_q.begin(), s->touch_q.end(), j);
+    if (it != s->touch_q.end()) s->touch_q.erase(it);
+    s->touch_cv.wait(lk, [&] { return j->touch_inflight == 0; });
+}
 
 static size_t workspace_total(hv_server* s) {
     size_t t = 0;
@@ -247,6 +299,7 @@ static int launch_job(hv_server* s, Slot& sl, SrvJob& j) {
 }
 
 static void finish_locked(hv_server* s, SrvJob* j) {
+    stop_touch(s, j);
     j->t_done = now_s();
     s->finished[j->id] = j;
     s->finished_order.push_back(j->id);
@@ -394,6 +447,7 @@ static void copier_main(hv_server* s) {
         s->to_copy.pop_front();
         SrvJob* j = sl->job;
         lk.unlock();
+        stop_touch(s, j);
         if (j->out) copy_result(j->out, sl->pinned, j->out_bytes);
         lk.lock();
         finish_locked(s, j);
@@ -428,6 +482,9 @@ extern "C" int hv_server_create(hv_ctx* c, int nslots, size_t budget, int use_gr
     hv_server* raw = s.release();
     raw->worker = std::thread(worker_main, raw);
     raw->copier = std::thread(copier_main, raw);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    for (unsigned i = 0; i < std::min(4u, std::max(1u, hw / 2)); ++i)
+        raw->touchers.emplace_back(toucher_main, raw);
     *out = raw;
     return HV_OK;
 }
@@ -477,7 +534,13 @@ extern "C" int hv_server_submit(hv_server* s, const hv_job* job, int64_t* id) {
     j->id = s->next_id++;
     j->t_submit = now_s();
     *id = j->id;
-    s->pending.push_back(j.release());
+    SrvJob* raw = j.release();
+    if (raw->out && raw->out_bytes >= TOUCH_MIN) {
+        std::lock_guard<std::mutex> tl(s->touch_mu);
+        s->touch_q.push_back(raw);
+        s->touch_cv.notify_all();
+    }
+    s->pending.push_back(raw);
     s->stats.submitted++;
     s->cv.notify_one();
     return HV_OK;
@@ -566,6 +629,13 @@ extern "C" int hv_server_destroy(hv_server* s) {
         s->copy_cv.notify_all();
     }
     if (s->copier.joinable()) s->copier.join();
+    {
+        // every job has finished, which ended its pre-faulting
+        std::lock_guard<std::mutex> lk(s->touch_mu);
+        s->touch_quit = true;
+        s->touch_cv.notify_all();
+    }
+    for (auto& t : s->touchers) t.join();
     (void)hipSetDevice(s->ctx->device);
     for (auto& up : s->slots) {
         Slot& sl = *up;

@@ -300,7 +300,9 @@ typedef struct hv_job {
     int ntimes;
     void* vis_out;             /* host (F,T,N,N) complex64 (complex128 for FP64),
                                   or a baseline layout (flags), written before the
-                                  job is reported finished                         */
+                                  job is reported finished; from submission on,
+                                  its pages may be pre-faulted (zero bytes written)
+                                  until the result is copied in                     */
 } hv_job;
 
 typedef struct hv_job_info {

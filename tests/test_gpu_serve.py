@@ -152,3 +152,23 @@ def test_per_frequency_beam_jobs_equal_alone(catalogs):
             np.testing.assert_array_equal(r.vis, alone)
     finally:
         beam.close()
+
+
+def test_large_outputs_prefaulted_equal_alone_bitwise(catalogs):
+    # outputs of 42 MB (N = 128, 8 times x 40 frequencies, few sources): the
+    # server pre-faults their pages from submission on, in 32 MB chunks, while
+    # earlier jobs run; one job fails at admission with its output still being
+    # pre-faulted, and the others are unaffected
+    antpos, beam, sky = catalogs["disk128"]
+    freqs = np.linspace(100e6, 200e6, 40)
+    specs = [(i, 500 + 700 * i, configs.eq2tops_lst(8, lst0_deg=7.0 * i).astype(np.float64)) for i in range(6)]
+    with Server(nslots=2) as srv:
+        ids = {srv.submit(sky, beam, antpos, freqs, eq, nsrc=nsrc, gemm="split_bf16"): (i, nsrc, eq)
+               for i, nsrc, eq in specs}
+        bad = srv.submit(sky, beam, antpos, freqs, specs[0][2], nsrc=100, gemm="split_bf16", flags=FAIL_TEST)
+        assert srv.wait(bad, timeout_s=120).status != 0
+        done = {jid: srv.wait(jid, timeout_s=120) for jid in ids}
+    for jid, (i, nsrc, eq) in ids.items():
+        assert done[jid].status == 0, done[jid].error
+        alone, _ = stages.simulate(sky, beam, antpos, freqs, eq, gemm="split_bf16", nsrc=nsrc)
+        np.testing.assert_array_equal(done[jid].vis, alone, err_msg=f"job {i}")

@@ -362,3 +362,21 @@ def test_diffuse_sky_config_vs_oracle():
     assert 1 <= st["herk_n"] <= len(w.eq2tops)
     np.testing.assert_array_equal(a, vis_gpu(w.antpos, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube,
                                              gemm="split_bf16"))
+
+
+@pytest.mark.parametrize("gemm", P1_MODES + ["fp64"])
+def test_empty_and_unseen_skies_give_zero(gemm):
+    # no sources, and sources only at or below the horizon (the horizon test
+    # is strict z > 0, vis.py:75): every visibility is exactly zero
+    p = 2 if gemm == "fp64" else 1
+    antpos = np.random.default_rng(1).uniform(-50, 50, (NANT, 3))
+    eq2tops = np.array([np.identity(3)] * 3)
+    bm_cube = np.ones((NANT, BM_PIX, BM_PIX))
+    v = vis_gpu(antpos, 1.5e8, eq2tops, np.zeros((3, 0)), np.zeros(0), bm_cube, precision=p, gemm=gemm)
+    assert v.shape == (3, NANT, NANT)
+    np.testing.assert_array_equal(v, 0)
+    crd = np.zeros((3, 100))
+    crd[0] = np.linspace(-1, 1, 100)
+    crd[2, 50:] = -np.sqrt(1 - crd[0, 50:] ** 2)       # half on the horizon (z = 0), half below
+    v = vis_gpu(antpos, 1.5e8, eq2tops, crd, np.ones(100), bm_cube, precision=p, gemm=gemm)
+    np.testing.assert_array_equal(v, 0)
