@@ -1818,9 +1818,72 @@ __device__ __forceinline__ void t192_rect5_mma(const lds_char* As, const lds_cha
     if constexpr (Q == 7) t192_rect_mma<MODE, IM, 0, 5, 3, 3>(As, Bs, lrow, lh, acc);
 }
 
+// The two diagonal tiles of a two-block array as one item (As: the first
+// block's rows, Bs: the second's, whose last block column inside the array
+// is BC = 4 or 3): the 21 blocks r <= c of the first and the 15 or 10 of the
+// second, 9, 9, 9, 9 (8, 8, 8, 7) per SIMD, where the two tiles apart take
+// 6 and 4 (3) per SIMD in two items.
+template <int MODE, bool IM, int Q, int BC>
+__device__ __forceinline__ void t192_pair_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
+                                              f32x16 (&acc)[9]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int gg = 2 * s + lh;
+        if constexpr (Q == 0) {
+            t192_diag_row<MODE, IM, 0, 0, 5, 0>(As, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 3, 3, BC, 6>(Bs, lrow, gg, acc);
+            if constexpr (BC == 4) t192_diag_row<MODE, IM, 4, 4, 4, 8>(Bs, lrow, gg, acc);
+        }
+        if constexpr (Q == 1) {
+            t192_diag_row<MODE, IM, 1, 1, 5, 0>(As, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 1, 1, BC, 5>(Bs, lrow, gg, acc);
+        }
+        if constexpr (Q == 2) {
+            t192_diag_row<MODE, IM, 2, 2, 5, 0>(As, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 0, 0, BC, 4>(Bs, lrow, gg, acc);
+        }
+        if constexpr (Q == 3) {
+            t192_diag_row<MODE, IM, 3, 3, 5, 0>(As, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 4, 4, 5, 3>(As, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 5, 5, 5, 5>(As, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 2, 2, BC, 6>(Bs, lrow, gg, acc);
+        }
+    }
+}
+
 // Block k of a wave's accumulator set -> (block row, block column) in the
-// tile (list DQ of t192_diag_mma; DQ < 0: quadrant q), or false past the set.
-__device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
+// tile (list DQ of t192_split; DQ < 0: quadrant q), or false past the set;
+// `second`: the block lies in the second tile of a paired item (DQ >= 20).
+__device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc, int& second) {
+    second = 0;
+    if (dq >= 20) {
+        const int BC = dq < 24 ? 4 : 3;
+        const int Q = dq & 3;
+        if (Q == 0) {
+            if (k < 6) { br = 0; bc = k; return true; }
+            second = 1;
+            br = k < 8 ? 3 : 4;
+            bc = k < 8 ? 3 + (k - 6) : 4;
+            return BC == 4 ? k < 9 : k < 7;
+        }
+        if (Q == 1) {
+            if (k < 5) { br = 1; bc = 1 + k; return true; }
+            second = 1; br = 1; bc = 1 + (k - 5);
+            return k < 5 + BC;
+        }
+        if (Q == 2) {
+            if (k < 4) { br = 2; bc = 2 + k; return true; }
+            second = 1; br = 0; bc = k - 4;
+            return k < 5 + BC;
+        }
+        if (k < 6) {
+            br = k < 3 ? 3 : k < 5 ? 4 : 5;
+            bc = k < 3 ? 3 + k : k < 5 ? 4 + (k - 3) : 5;
+            return true;
+        }
+        second = 1; br = 2; bc = 2 + (k - 6);
+        return k < 5 + BC;
+    }
     switch (dq) {
         case 0: br = 0; bc = k; return k < 6;
         case 1: br = 1; bc = 1 + k; return k < 5;
@@ -1851,7 +1914,10 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc) {
 // DQ = 8..11: off-diagonal tile with at most five B column blocks inside
 // the array (30 blocks); DQ = 12..15: diagonal tile with at most four block
 // rows inside (10 blocks); DQ = 16..19: off-diagonal tile with at most four
-// B column blocks inside (24 blocks).
+// B column blocks inside (24 blocks); DQ = 20..23 / 24..27: the two diagonal
+// tiles of a two-block array as one item, at most five / four block rows of
+// the second inside (t192_pair_mma).  Paired items build both blocks' rows
+// as an off-diagonal tile does.
 template <int MODE, bool IM, int DQ>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
                                            const double* antpos, int nant, int bi, int bj, const float* amp_t,
@@ -1860,7 +1926,9 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
                                            f32x16 (&acc)[9]) {
     using L = T192;
     constexpr bool DIAG = (DQ >= 0 && DQ < 8) || (DQ >= 12 && DQ < 16);
-    constexpr bool RECT5 = (DQ >= 8 && DQ < 12) || DQ >= 16;
+    constexpr bool RECT5 = (DQ >= 8 && DQ < 12) || (DQ >= 16 && DQ < 20);
+    constexpr bool PAIR = DQ >= 20;
+    constexpr int QP = DQ & 3, BCP = DQ < 24 ? 4 : 3;   // t192_pair_mma list
     constexpr int QD = DQ < 8 ? DQ : DQ - 4;     // t192_diag_mma list
     constexpr int QR = DQ < 12 ? DQ - 8 : DQ - 12;   // t192_rect5_mma list
     constexpr int NROW = DIAG ? L::TILE : L::ROWS;
@@ -1891,6 +1959,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
     do {                                                                                   \
         if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? QD : 0)>(buf, lrow, lh, acc);  \
         else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, lrow, lh, acc); \
+        else if constexpr (PAIR) t192_pair_mma<MODE, IM, QP, BCP>(buf, buf + L::OP, lrow, lh, acc); \
         else t192_quad_mma<MODE, IM>(buf, buf + L::OP, ra0, rb0, lh, acc);                 \
     } while (0)
 
@@ -1952,6 +2021,14 @@ __host__ __device__ inline int t192_unit_group(int nunits, int64_t g) {
     return (nunits + nug - 1) / nug;
 }
 
+// Two-block arrays whose second block holds at most five 32-row blocks of
+// antennas run their two diagonal tiles as one item (work tile 0) beside the
+// off-diagonal tile (work tile 1): 2 items per (unit, split) instead of 3.
+__host__ __device__ inline bool t192_pairs(int nb, int nant) { return nb == 2 && nant - 192 <= 160; }
+__host__ __device__ inline int t192_work_tiles(int nb, int nant, int ntiles) {
+    return t192_pairs(nb, nant) ? 2 : ntiles;
+}
+
 __host__ __device__ inline int64_t t192_items(int nunits, int nsplit, int ntiles, int ug) {
     const int64_t nug = (nunits + ug - 1) / ug;
     const int64_t ncombo = nug * nsplit * ntiles;
@@ -1980,12 +2057,23 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     const size_t amp_time = (size_t)a.kpad * a.nant;
     constexpr int STAGES_SPLIT = KSPLIT / 16;
     const int ug = t192_unit_group(a.nunits, gridDim.x);
-    const int64_t nitems = t192_items(a.nunits, a.nsplit, a.ntiles, ug);
+    const bool pairs = t192_pairs(a.nb, a.nant);
+    const int ntw = pairs ? 2 : a.ntiles;
+    const int64_t nitems = t192_items(a.nunits, a.nsplit, ntw, ug);
 
     for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
         int unit, split, tile, bi, bj;
-        if (!t192_decode(a.nunits, a.nsplit, a.ntiles, ug, Lw, unit, split, tile)) continue;
-        decode_tile(tile, a.nb, bi, bj);
+        if (!t192_decode(a.nunits, a.nsplit, ntw, ug, Lw, unit, split, tile)) continue;
+        // a paired item (work tile 0 of a two-block array) builds both blocks'
+        // rows; its blocks are stored to the diagonal tiles 0 and 2
+        const bool pair = pairs && tile == 0;
+        if (pairs) {
+            bi = 0;
+            bj = 1;
+            tile = pair ? 0 : 1;
+        } else {
+            decode_tile(tile, a.nb, bi, bj);
+        }
         const bool diag = (bi == bj);
         const int s_begin = split * STAGES_SPLIT;
         const int s_end = min(s_begin + STAGES_SPLIT, kst);
@@ -2026,7 +2114,8 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         // block (off-diagonal tiles) hold antennas
         const int rem = a.nant - bj * TILE;
         const bool five = rem <= 160, four = rem <= 128;
-        const int dq = diag ? (four ? 12 : five ? 4 : 0) + q : (four ? 16 + q : five ? 8 + q : -1);
+        const int dq = pair ? (four ? 24 : 20) + q
+                            : diag ? (four ? 12 : five ? 4 : 0) + q : (four ? 16 + q : five ? 8 + q : -1);
         if (dq < 0) HV_T192BOTH(-1);
         else if (dq == 0) HV_T192BOTH(0);
         else if (dq == 1) HV_T192BOTH(1);
@@ -2047,18 +2136,29 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         else if (dq == 16) HV_T192BOTH(16);
         else if (dq == 17) HV_T192BOTH(17);
         else if (dq == 18) HV_T192BOTH(18);
-        else HV_T192BOTH(19);
+        else if (dq == 19) HV_T192BOTH(19);
+        else if (dq == 20) HV_T192BOTH(20);
+        else if (dq == 21) HV_T192BOTH(21);
+        else if (dq == 22) HV_T192BOTH(22);
+        else if (dq == 23) HV_T192BOTH(23);
+        else if (dq == 24) HV_T192BOTH(24);
+        else if (dq == 25) HV_T192BOTH(25);
+        else if (dq == 26) HV_T192BOTH(26);
+        else HV_T192BOTH(27);
 #undef HV_T192BOTH
 #undef HV_T192SPLIT
 
         constexpr size_t TSZ = (size_t)TILE * TILE;
-        float* out = reinterpret_cast<float*>(a.partials) +
-                     (((size_t)unit * a.nsplit + split) * a.ntiles + tile) * 2 * TSZ + (im ? TSZ : 0);
+        float* const out0 = reinterpret_cast<float*>(a.partials) +
+                            (((size_t)unit * a.nsplit + split) * a.ntiles) * 2 * TSZ + (im ? TSZ : 0);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            int br, bc;
-            if (!t192_block(dq, q, k, br, bc)) continue;
-            if (bi * TILE + br * 32 >= a.nant || bj * TILE + bc * 32 >= a.nant) continue;
+            int br, bc, second;
+            if (!t192_block(dq, q, k, br, bc, second)) continue;
+            // a paired item's blocks: tile 0 = (0, 0) or tile 2 = (1, 1)
+            const int ot = pair ? 2 * second : tile, ob0 = pair ? second : bi, ob1 = pair ? second : bj;
+            if (ob0 * TILE + br * 32 >= a.nant || ob1 * TILE + bc * 32 >= a.nant) continue;
+            float* out = out0 + (size_t)ot * 2 * TSZ;
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const int row = br * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
@@ -2129,7 +2229,8 @@ static hipError_t launch_t192_t(const FusedArgs& a, hipStream_t s) {
     if (a.kpad % 16) return hipErrorInvalidValue;
     const int64_t res = t192_resident<MODE>();
     const int64_t g0 = res > 0 ? res : 256;
-    const int64_t items = t192_items(a.nunits, a.nsplit, a.ntiles, t192_unit_group(a.nunits, g0));
+    const int64_t items = t192_items(a.nunits, a.nsplit, t192_work_tiles(a.nb, a.nant, a.ntiles),
+                                     t192_unit_group(a.nunits, g0));
     const int64_t grid = g0 < items ? g0 : items;
     hipLaunchKernelGGL((herk_t192_kernel<MODE>), dim3((unsigned)grid), dim3(512), 0, s, a);
     return hipGetLastError();
