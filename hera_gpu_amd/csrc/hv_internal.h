@@ -192,6 +192,7 @@ struct AmpArgs {
                              // else elements between frequency planes (amp per unit)
     int nfreq;               // frequencies per time of the launch (per-unit amp)
     int ants_per_wg;         // set by launch_amp (beams that fit in LDS)
+    int src_per_wg;          // set by launch_amp (sources per workgroup)
 };
 
 // Contraction of split-mode operands generated in place (see hv_herk.hip).
