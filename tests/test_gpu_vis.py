@@ -380,3 +380,16 @@ def test_empty_and_unseen_skies_give_zero(gemm):
     crd[2, 50:] = -np.sqrt(1 - crd[0, 50:] ** 2)       # half on the horizon (z = 0), half below
     v = vis_gpu(antpos, 1.5e8, eq2tops, crd, np.ones(100), bm_cube, precision=p, gemm=gemm)
     np.testing.assert_array_equal(v, 0)
+
+
+def test_amplitude_pass_chunking_bitwise():
+    # 16 times x 128 antennas x 40000 sources in one launch: the beam
+    # amplitude pass gives each workgroup 32768 sources (the launch keeps
+    # >= 1024 workgroups); one time alone gives it 8192.  The amplitudes,
+    # and so the visibilities, are the same bit for bit.
+    w = configs.make_disk128(nsrc=40000, ntimes=16)
+    freqs = np.array([150e6])
+    allt = vis_gpu(w.antpos, freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, gemm="split_bf16")
+    for t in (0, 7, 15):
+        one = vis_gpu(w.antpos, freqs, w.eq2tops[t:t + 1], w.crd_eq, w.I_sky, w.bm_cube, gemm="split_bf16")
+        np.testing.assert_array_equal(one[:, 0], allt[:, t])
