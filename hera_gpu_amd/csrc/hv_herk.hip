@@ -1818,11 +1818,11 @@ __device__ __forceinline__ void t192_rect5_mma(const lds_char* As, const lds_cha
     if constexpr (Q == 7) t192_rect_mma<MODE, IM, 0, 5, 3, 3>(As, Bs, lrow, lh, acc);
 }
 
-// The two diagonal tiles of a two-block array as one item (As: the first
-// block's rows, Bs: the second's, whose last block column inside the array
-// is BC = 4 or 3): the 21 blocks r <= c of the first and the 15 or 10 of the
-// second, 9, 9, 9, 9 (8, 8, 8, 7) per SIMD, where the two tiles apart take
-// 6 and 4 (3) per SIMD in two items.
+// The first and the last diagonal tile as one item (As: the first block's
+// rows, Bs: the last block's, whose last block column inside the array is
+// BC = 4 or 3): the 21 blocks r <= c of the first and the 15 or 10 of the
+// last, 9, 9, 9, 9 (8, 8, 8, 7) per SIMD, where the two tiles apart take 6
+// and 4 (3) per SIMD in two items.
 template <int MODE, bool IM, int Q, int BC>
 __device__ __forceinline__ void t192_pair_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
                                               f32x16 (&acc)[9]) {
@@ -1914,9 +1914,9 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc, int& s
 // DQ = 8..11: off-diagonal tile with at most five B column blocks inside
 // the array (30 blocks); DQ = 12..15: diagonal tile with at most four block
 // rows inside (10 blocks); DQ = 16..19: off-diagonal tile with at most four
-// B column blocks inside (24 blocks); DQ = 20..23 / 24..27: the two diagonal
-// tiles of a two-block array as one item, at most five / four block rows of
-// the second inside (t192_pair_mma).  Paired items build both blocks' rows
+// B column blocks inside (24 blocks); DQ = 20..23 / 24..27: the first and
+// the last diagonal tile as one item, at most five / four block rows of the
+// last inside (t192_pair_mma).  Paired items build both blocks' rows
 // as an off-diagonal tile does.
 template <int MODE, bool IM, int DQ>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
@@ -2021,12 +2021,13 @@ __host__ __device__ inline int t192_unit_group(int nunits, int64_t g) {
     return (nunits + nug - 1) / nug;
 }
 
-// Two-block arrays whose second block holds at most five 32-row blocks of
-// antennas run their two diagonal tiles as one item (work tile 0) beside the
-// off-diagonal tile (work tile 1): 2 items per (unit, split) instead of 3.
-__host__ __device__ inline bool t192_pairs(int nb, int nant) { return nb == 2 && nant - 192 <= 160; }
+// Arrays whose last block holds at most five 32-row blocks of antennas run
+// the first and the last diagonal tile as one item (work tile 0: tiles 0 and
+// ntiles - 1 of the decode order); work tile w = 1 .. ntiles - 2 is tile w.
+// Two blocks: 2 items per (unit, split) instead of 3.
+__host__ __device__ inline bool t192_pairs(int nb, int nant) { return nb >= 2 && nant - (nb - 1) * 192 <= 160; }
 __host__ __device__ inline int t192_work_tiles(int nb, int nant, int ntiles) {
-    return t192_pairs(nb, nant) ? 2 : ntiles;
+    return t192_pairs(nb, nant) ? ntiles - 1 : ntiles;
 }
 
 __host__ __device__ inline int64_t t192_items(int nunits, int nsplit, int ntiles, int ug) {
@@ -2058,19 +2059,19 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     constexpr int STAGES_SPLIT = KSPLIT / 16;
     const int ug = t192_unit_group(a.nunits, gridDim.x);
     const bool pairs = t192_pairs(a.nb, a.nant);
-    const int ntw = pairs ? 2 : a.ntiles;
+    const int ntw = t192_work_tiles(a.nb, a.nant, a.ntiles);
     const int64_t nitems = t192_items(a.nunits, a.nsplit, ntw, ug);
 
     for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
         int unit, split, tile, bi, bj;
         if (!t192_decode(a.nunits, a.nsplit, ntw, ug, Lw, unit, split, tile)) continue;
-        // a paired item (work tile 0 of a two-block array) builds both blocks'
-        // rows; its blocks are stored to the diagonal tiles 0 and 2
+        // a paired item (work tile 0) builds the rows of the first and the
+        // last block; its blocks are stored to the diagonal tiles 0 and
+        // ntiles - 1
         const bool pair = pairs && tile == 0;
-        if (pairs) {
+        if (pair) {
             bi = 0;
-            bj = 1;
-            tile = pair ? 0 : 1;
+            bj = a.nb - 1;
         } else {
             decode_tile(tile, a.nb, bi, bj);
         }
@@ -2155,8 +2156,9 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         for (int k = 0; k < 9; ++k) {
             int br, bc, second;
             if (!t192_block(dq, q, k, br, bc, second)) continue;
-            // a paired item's blocks: tile 0 = (0, 0) or tile 2 = (1, 1)
-            const int ot = pair ? 2 * second : tile, ob0 = pair ? second : bi, ob1 = pair ? second : bj;
+            // a paired item's blocks: tile 0 = (0, 0) or tile ntiles - 1 = (nb - 1, nb - 1)
+            const int ot = pair ? second * (a.ntiles - 1) : tile;
+            const int ob0 = pair ? second * (a.nb - 1) : bi, ob1 = pair ? second * (a.nb - 1) : bj;
             if (ob0 * TILE + br * 32 >= a.nant || ob1 * TILE + bc * 32 >= a.nant) continue;
             float* out = out0 + (size_t)ot * 2 * TSZ;
 #pragma unroll

@@ -213,7 +213,7 @@ def test_batch_schedules_agree_bitwise(gemm, monkeypatch):
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("tile,nant", [(64, 300), (128, 300), (192, 300), (192, 160), (192, 129), (192, 400), (192, 97), (192, 128), (192, 230), (128, 128), (128, 100), (128, 70), (192, 350)])
+@pytest.mark.parametrize("tile,nant", [(64, 300), (128, 300), (192, 300), (192, 160), (192, 129), (192, 400), (192, 97), (192, 128), (192, 230), (128, 128), (128, 100), (128, 70), (192, 350), (192, 520)])
 def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile, nant):
     # pf 5 builds the operand stages inside the contraction from A sqrt(I);
     # pf 4 contracts the stages phasor_kernel wrote: same operand bits, same
