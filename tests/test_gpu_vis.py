@@ -393,3 +393,23 @@ def test_amplitude_pass_chunking_bitwise():
     for t in (0, 7, 15):
         one = vis_gpu(w.antpos, freqs, w.eq2tops[t:t + 1], w.crd_eq, w.I_sky, w.bm_cube, gemm="split_bf16")
         np.testing.assert_array_equal(one[:, 0], allt[:, t])
+
+
+@pytest.mark.parametrize("gemm", P1_MODES + ["fp64"])
+def test_single_antenna_and_negative_intensity(gemm):
+    # N = 1 (autos only) against the oracle; a negative intensity gives NaN
+    # everywhere, as the reference's host-side sqrt does (vis.py:174), also
+    # through sources below the horizon (0 * NaN)
+    p = 2 if gemm == "fp64" else 1
+    d = load("realistic_small.npz")
+    I_sky = d["I_sky"].astype(np.float64) if p == 2 else d["I_sky"]
+    ap, bm = d["antpos"][:1], d["bm_cube"][:1]
+    v = vis_gpu(ap, d["freqs"], d["eq2tops"], d["crd_eq"], I_sky, bm, precision=p, gemm=gemm)
+    ref = vis_oracle(ap, d["freqs"], d["eq2tops"], d["crd_eq"], d["I_sky"], bm)
+    assert v.shape == ref.shape and rel_err(v, ref) < VIS_TOL[gemm]
+    assert np.all(v.imag == 0) and np.all(v.real > 0)
+    bad = I_sky.copy()
+    bad[3] = -1.0
+    with pytest.warns(RuntimeWarning):           # numpy's sqrt of a negative, as in the reference
+        v = vis_gpu(d["antpos"], d["freqs"], d["eq2tops"], d["crd_eq"], bad, d["bm_cube"], precision=p, gemm=gemm)
+    assert np.isnan(v).all()
