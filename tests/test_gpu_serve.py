@@ -172,3 +172,16 @@ def test_large_outputs_prefaulted_equal_alone_bitwise(catalogs):
         assert done[jid].status == 0, done[jid].error
         alone, _ = stages.simulate(sky, beam, antpos, freqs, eq, gemm="split_bf16", nsrc=nsrc)
         np.testing.assert_array_equal(done[jid].vis, alone, err_msg=f"job {i}")
+
+
+def test_empty_source_set_job_gives_zero(catalogs):
+    # a job over the first 0 sources of the catalog: the padded directions are
+    # zero (not above the horizon), so every visibility is exactly zero, also
+    # after a job whose sources filled the same slot
+    antpos, beam, sky, eq, freqs = job_inputs(catalogs, 3, "hex37", 2, 2)
+    with Server(nslots=1) as srv:
+        full = srv.wait(srv.submit(sky, beam, antpos, freqs, eq, nsrc=30000), 60)
+        empty = srv.wait(srv.submit(sky, beam, antpos, freqs, eq, nsrc=0), 60)
+    assert full.status == 0 and empty.status == 0, (full.error, empty.error)
+    assert np.abs(full.vis).max() > 0
+    np.testing.assert_array_equal(empty.vis, 0)
