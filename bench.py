@@ -48,7 +48,7 @@ PF_NAMES = {1: "register-staged, 1 stage ahead", 2: "register-staged, 2 stages a
 def variant_name(tile, pf):
     if tile == 192:
         return ("operands built in place from A sqrt(I), 8 waves (Re and Im wave pair per SIMD), "
-                "persistent XCD-grouped grid")
+                "first and last diagonal tiles as one item, persistent XCD-grouped grid")
     return PF_NAMES[pf]
 
 
