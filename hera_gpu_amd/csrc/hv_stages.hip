@@ -402,15 +402,22 @@ hipError_t launch_phasor_pol(int mode, const PhasorPolArgs& a, hipStream_t s) {
 // coordinates, the horizon test and the texel position and weights are
 // formed once and applied to each antenna's beam.  Grid (antenna group,
 // source chunk, time or unit).
-constexpr int AMP_ANT = 4;                        // antennas per workgroup, at most
+// antennas per workgroup, at most (their beams must fit the 160 KB of LDS),
+// and threads per workgroup: 8 antennas share each source's coordinates,
+// horizon test and texel weights, with 8 waves to cover the latency that
+// one 127 KB workgroup per CU leaves (measured on MI355X against 4 antennas
+// and 256 threads: 1.14 -> 1.02 ms for 128 antennas x 65536 sources x 64
+// times, 2.58 -> 2.24 ms for 350 x 2^20 x 2)
+constexpr int AMP_ANT = 8;
+constexpr int AMP_THREADS = 512;
 // sources per workgroup: the beam planes staged in LDS serve 4x the sources
-// of a phasor_kernel chunk (measured on MI355X: 1.30 -> 1.10 ms for 128
-// antennas x 65536 sources x 64 times, 2.84 -> 2.53 ms for 350 x 2^20 x 2),
-// unless that leaves fewer than 1024 workgroups (small jobs keep 8192)
+// of a phasor_kernel chunk (1.30 -> 1.10 ms and 2.84 -> 2.53 ms on the same
+// two launches), unless that leaves fewer than 1024 workgroups (small jobs
+// keep 8192)
 constexpr int AMP_SRC_PER_WG = 32768;
 
 template <typename BT>
-__global__ __launch_bounds__(PH_THREADS) void amp_kernel(AmpArgs a) {
+__global__ __launch_bounds__(AMP_THREADS) void amp_kernel(AmpArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     BT* sb = reinterpret_cast<BT*>(smem);                 // [AMP_ANT][P][P]
     const int64_t kact = eff_kpad(a.kpad, a.dyn_kpad);
@@ -425,7 +432,7 @@ __global__ __launch_bounds__(PH_THREADS) void amp_kernel(AmpArgs a) {
     const int P = a.beam_px;
     const int pp = P * P;
     const BT* gb = reinterpret_cast<const BT*>(a.beam) + fu * a.beam_fstride + (size_t)ant0 * pp;
-    for (int i = threadIdx.x; i < nq * pp; i += PH_THREADS) sb[i] = gb[i];
+    for (int i = threadIdx.x; i < nq * pp; i += AMP_THREADS) sb[i] = gb[i];
     __syncthreads();
     const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
     const double* cy = cx + a.kpad;
@@ -434,7 +441,7 @@ __global__ __launch_bounds__(PH_THREADS) void amp_kernel(AmpArgs a) {
     float* out = a.amp + (size_t)z * a.kpad * a.nant;
     const double pm = (double)(P - 1);
     for (int64_t s0 = wg0 + (int64_t)threadIdx.x * PH_SRC_PER_THREAD; s0 < wg1;
-         s0 += PH_THREADS * PH_SRC_PER_THREAD) {
+         s0 += AMP_THREADS * PH_SRC_PER_THREAD) {
         // texel corners and weights of each source (beam_bilinear's arithmetic)
         int o00[8], o01[8], o10[8], o11[8];
         BT fx[8], fy[8];
@@ -496,9 +503,9 @@ hipError_t launch_amp(const AmpArgs& args, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     if (a.in_f64)
-        hipLaunchKernelGGL(amp_kernel<double>, grid, dim3(PH_THREADS), lds, s, a);
+        hipLaunchKernelGGL(amp_kernel<double>, grid, dim3(AMP_THREADS), lds, s, a);
     else
-        hipLaunchKernelGGL(amp_kernel<float>, grid, dim3(PH_THREADS), lds, s, a);
+        hipLaunchKernelGGL(amp_kernel<float>, grid, dim3(AMP_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
