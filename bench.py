@@ -91,30 +91,51 @@ def run_timed(stages, sky, beam, w, gemm, steps, warmup):
     return st, wall
 
 
-def cpu_baseline(w, nsrc, gpu_unit):
-    """vis_cpu restated (float32, one call per frequency) on one unit of the workload."""
-    from oracle.cpu_ref import vis_cpu
-    from oracle.vis_oracle import vis_oracle_units
+def cpu_baseline(w, nsrc, gpu_unit, workers):
+    """vis_cpu restated (float32, one call per frequency) on one unit of the
+    workload at full N and the first `nsrc` sources (all K by default), with
+    the per-antenna beam and phasor rows on `workers` processes and the dot
+    products on the host BLAS; the FP64 oracle on the same unit checks the
+    CPU and GPU results.  Also the small 37-antenna workload in full."""
+    from hera_gpu_amd import configs
+    from oracle.cpu_ref import vis_cpu, vis_cpu_parallel
+    from oracle.vis_oracle import vis_oracle, vis_oracle_units
 
     N = w.antpos.shape[0]
+    T, F = w.eq2tops.shape[0], len(w.freqs)
     crd = w.crd_eq[:, :nsrc]
     I = w.I_sky[:nsrc]
     t0 = time.perf_counter()
-    v = vis_cpu(w.antpos, float(w.freqs[-1]), w.eq2tops[:1], crd, I, w.bm_cube, precision=1)
+    v = vis_cpu_parallel(w.antpos, float(w.freqs[-1]), w.eq2tops[:1], crd, I, w.bm_cube, precision=1,
+                         workers=workers)
     dt = time.perf_counter() - t0
-    # the FP64 oracle on the same unit checks the GPU result for it
     ref = vis_oracle_units(w.antpos, w.freqs[-1:], w.eq2tops[:1], crd, I, w.bm_cube, [(0, 0)])[0]
-    cores = len(os.sched_getaffinity(0))
+    per_unit = dt * w.crd_eq.shape[1] / nsrc
     res = {
-        "value": herk_flops(N, nsrc, 1) / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
-        "sample": f"1 unit (time 0, {w.freqs[-1] / 1e6:.0f} MHz), N={N}, first {nsrc} sources; "
-                  f"{dt:.2f} s; vis_cpu restated in float32 (scipy linear spline beam, BLAS "
-                  f"dot products on {cores} threads)",
+        "value": herk_flops(N, nsrc, 1) / dt / 1e12, "unit": "TFLOP/s", "cores": workers, "kind": "port",
+        "sample": f"1 unit (time 0, {w.freqs[-1] / 1e6:.0f} MHz) of the workload, N={N}, K={nsrc}"
+                  f"{'' if nsrc == w.crd_eq.shape[1] else ' (first sources)'}: {dt:.1f} s; vis_cpu restated in "
+                  f"float32 (scipy linear spline beam), its per-antenna beam and phasor rows on {workers} "
+                  f"processes, the per-row dot products on the host BLAS ({workers} threads)",
         "seconds": dt,
+        "whole_job_extrapolated_s": per_unit * T * F,
+        "extrapolation": f"x {T * F} units (T x F), linear in K: {per_unit * T * F / 3600:.1f} h for the whole "
+                         f"workload (extrapolated, not run)",
         "cpu_fp32_rel_err_vs_oracle": rel_err(v[0], ref),
     }
     for gemm, vg in gpu_unit.items():
         res[f"gpu_{gemm}_rel_err_vs_oracle"] = rel_err(vg, ref)
+    # the 37-antenna workload in full (10 times, 1000 sources, one frequency)
+    h = configs.make_hex37()
+    t1 = time.perf_counter()
+    vh = vis_cpu(h.antpos, float(h.freqs[0]), h.eq2tops, h.crd_eq, h.I_sky, h.bm_cube, precision=1)
+    dh = time.perf_counter() - t1
+    sh = h.shape
+    res["hex37_full"] = {"seconds": dh, "TFLOP/s": herk_flops(sh["N"], sh["K"], sh["T"] * sh["F"]) / dh / 1e12,
+                         "rel_err_vs_oracle": rel_err(vh, vis_oracle(h.antpos, h.freqs, h.eq2tops, h.crd_eq,
+                                                                      h.I_sky, h.bm_cube)[0]),
+                         "sample": "the whole 37-antenna workload (N=37, K=1000, T=10, F=1), vis_cpu restated, "
+                                   "one process"}
     return res
 
 
@@ -266,13 +287,19 @@ def medfilt_leg(reps=20, cpu_rows=256):
     return out
 
 
-def serving_leg(njobs, gemms, nslots, rate=None):
-    """The mixed-job serving stream: jobs with N in {37,128,350}, Nsrc log-uniform
-    in [1e3, 2^20], ntimes in {1,8,32}, Nfreq in {1,16,64}.  For each arithmetic the
-    server's capacity on this mix is measured first (every job submitted at t=0,
-    jobs / makespan); the Poisson stream then arrives at 70% of that capacity
-    (or at `rate`), so the latencies are those of a loaded, not an overloaded,
-    server."""
+SERVE_BUDGET = 200 << 30   # device bytes for the server's instances (clamped to what the context has left)
+
+
+def serving_leg(njobs, gemm, nslots, rate=None, burst_jobs=200, fp32_jobs=200, graphs=True):
+    """The fifth workload's serving load: a Poisson stream of jobs with N in
+    {37,128,350}, Nsrc log-uniform in [1e3, 2^20], ntimes in {1,8,32}, Nfreq
+    in {1,16,64}, beams and catalogs resident per layout.  The capacity of
+    each arithmetic is measured on the same t=0 burst (the stream's first
+    `burst_jobs` jobs, jobs / makespan), which also gives the like-for-like
+    split-BF16 / FP32 rate ratio; the split-BF16 stream then brings all
+    `njobs` jobs at 70% of its capacity (or `rate`), the FP32 stream its first
+    `fp32_jobs` at 70% of the FP32 capacity, so the latencies are those of a
+    loaded, not an overloaded, server."""
     import dataclasses
 
     from hera_gpu_amd import configs, serve, stages
@@ -282,34 +309,48 @@ def serving_leg(njobs, gemms, nslots, rate=None):
         antpos, beam, crd, flux = configs.serving_catalog(lay)
         layouts[lay] = (antpos, stages.Beam(beam), stages.Sky(crd, flux))
     shapes = configs.make_serving_stream(njobs, rate_hz=1.0)
-    mean_flops = float(np.mean([herk_flops(len(layouts[j.layout][0]), j.nsrc, j.ntimes * j.nfreq) for j in shapes]))
+    nant = {lay: len(v[0]) for lay, v in layouts.items()}
+
+    def flops(js):
+        return float(sum(herk_flops(nant[j.layout], j.nsrc, j.ntimes * j.nfreq) for j in js))
 
     def make_args(j):
         antpos, beam, sky = layouts[j.layout]
         eq, freqs = configs.job_times_freqs(j)
         return sky, beam, antpos, freqs, eq, j.nsrc
 
-    out = {"jobs": njobs, "mean_job_flops": mean_flops, "slots": nslots,
+    burst = [dataclasses.replace(j, arrival_s=0.0) for j in shapes[:burst_jobs]]
+    out = {"jobs": njobs, "slots": nslots, "graphs": graphs, "stream_flops": flops(shapes),
            "latency": "scheduled arrival to result in host memory",
-           "load": "Poisson arrivals at 70% of the capacity measured by a t=0 burst of the same jobs"}
-    for gemm in gemms:
-        burst = [dataclasses.replace(j, arrival_s=0.0) for j in shapes]
-        with serve.Server(nslots=nslots) as srv:
-            bsum, _ = serve.run_load(srv, burst, make_args, gemm=gemm, progress=log)
-        capacity = njobs / bsum["makespan_s"]
-        r = rate if rate else 0.7 * capacity
-        log(f"[bench] serving {gemm}: capacity {capacity:.2f} jobs/s ({bsum['tflops']:.1f} TFLOP/s); "
-            f"stream of {njobs} jobs at {r:.2f}/s")
-        jobs = configs.make_serving_stream(njobs, rate_hz=r)
-        with serve.Server(nslots=nslots) as srv:
-            summ, _ = serve.run_load(srv, jobs, make_args, gemm=gemm, progress=log)
+           "load": f"Poisson arrivals at 70% of the capacity measured on a t=0 burst of the stream's first "
+                   f"{burst_jobs} jobs",
+           "burst": {"jobs": len(burst), "flops": flops(burst)}}
+    gemms = [gemm] + (["fp32"] if fp32_jobs and gemm != "fp32" else [])
+    cap = {}
+    for g in gemms:
+        with serve.Server(nslots=nslots, budget_bytes=SERVE_BUDGET, graphs=graphs) as srv:
+            bsum, _ = serve.run_load(srv, burst, make_args, gemm=g, progress=log)
+        cap[g] = bsum
+        out["burst"][g] = {"jobs_per_s": len(burst) / bsum["makespan_s"], "tflops": bsum["tflops"],
+                           "makespan_s": bsum["makespan_s"], "graph_hit_rate": bsum["graph_hit_rate"]}
+        log(f"[bench] serving {g}: capacity {len(burst) / bsum['makespan_s']:.2f} jobs/s "
+            f"({bsum['tflops']:.1f} TFLOP/s) on the {len(burst)}-job burst")
+    if "fp32" in cap:
+        out["burst"]["ratio_tflops"] = cap[gemm]["tflops"] / cap["fp32"]["tflops"]
+    for g in gemms:
+        n = njobs if g == gemm else min(fp32_jobs, njobs)
+        r = rate if rate else 0.7 * len(burst) / cap[g]["makespan_s"]
+        jobs = configs.make_serving_stream(n, rate_hz=r)
+        log(f"[bench] serving {g}: stream of {n} jobs at {r:.2f}/s")
+        with serve.Server(nslots=nslots, budget_bytes=SERVE_BUDGET, graphs=graphs) as srv:
+            summ, _ = serve.run_load(srv, jobs, make_args, gemm=g, progress=log)
             summ["stats"] = srv.stats()
         summ["rate_hz"] = r
-        summ["capacity_jobs_per_s"] = capacity
-        summ["capacity_tflops"] = bsum["tflops"]
-        out[gemm] = summ
-        log(f"[bench] serving {gemm}: p50 {summ['p50_ms']:.0f} ms p99 {summ['p99_ms']:.0f} ms "
-            f"{summ['tflops']:.1f} TFLOP/s")
+        summ["stream"] = "all jobs of the stream" if n == njobs else f"the stream's first {n} jobs (labelled)"
+        summ["flops"] = flops(jobs)
+        out[g] = summ
+        log(f"[bench] serving {g}: p50 {summ['p50_ms']:.0f} ms p99 {summ['p99_ms']:.0f} ms "
+            f"{summ['tflops']:.1f} TFLOP/s, graph hit rate {summ['graph_hit_rate']:.2f}")
     for _, b, s_ in layouts.values():
         b.close()
         s_.close()
@@ -324,15 +365,24 @@ def main():
     ap.add_argument("--workload", default="hera350", choices=["hera350", "disk128", "diffuse", "hex37"])
     ap.add_argument("--gemm", default="split_bf16", choices=["split_bf16", "split_fp16", "fp32"])
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 comparison run")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the split-FP16 timing run")
+    ap.add_argument("--cmp-steps", type=int, default=2, help="timed steps of the FP32 and split-FP16 runs")
+    ap.add_argument("--cmp-warmup", type=int, default=1, help="warm-up steps of the FP32 and split-FP16 runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 sample comparison")
-    ap.add_argument("--cpu-nsrc", type=int, default=2 ** 16)
+    ap.add_argument("--cpu-nsrc", type=int, default=0, help="sources of the CPU baseline unit (0 = all)")
+    ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (0 = cores, at most 16)")
     ap.add_argument("--no-dropin", action="store_true", help="skip the host-buffer vis_gpu timing")
     ap.add_argument("--no-omnical", action="store_true", help="skip the Omnical calibration leg")
     ap.add_argument("--no-medfilt", action="store_true", help="skip the median filter leg")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the per-frequency beam, baseline-layout and polarised-beam leg")
-    ap.add_argument("--serve-jobs", type=int, default=120, help="serving stream length (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=2000, help="serving stream length (0 = skip)")
+    ap.add_argument("--serve-burst", type=int, default=200, help="jobs of the capacity burst")
+    ap.add_argument("--serve-fp32-jobs", type=int, default=200, help="jobs of the FP32 stream (0 = none)")
+    ap.add_argument("--serve-no-graphs", action="store_true", help="serve with direct launches (A/B)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="profiling target: only the serving leg (prints its JSON object)")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrivals per second (default: 70%% of the measured serving capacity)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -364,6 +414,15 @@ def main():
     sky = stages.Sky(w.crd_eq, w.I_sky)
     beam = stages.Beam(w.bm_cube)
     flops_job = herk_flops(N, K, units)
+
+    if args.serve_only:
+        sky.close()
+        beam.close()
+        res = serving_leg(args.serve_jobs, args.gemm, args.serve_slots, args.serve_rate,
+                          burst_jobs=min(args.serve_burst, args.serve_jobs),
+                          fp32_jobs=0 if args.no_fp32 else args.serve_fp32_jobs, graphs=not args.serve_no_graphs)
+        print(json.dumps({"serving": res}), flush=True)
+        return
 
     if args.kernels_only:
         # a short, fixed sequence of dispatches for rocprofv3 counter runs:
@@ -405,14 +464,19 @@ def main():
         phasor_bytes = elem * N * K * units * args.steps / st["phasor_n"]
     stage_ms = {k: st[k] / args.steps for k in ("coords_ms", "phasor_ms", "herk_ms", "reduce_ms")}
 
-    fp32 = None
-    if not args.no_fp32 and args.gemm != "fp32":
-        log("[bench] fp32 comparison: warmup 0, timed 1")
-        st32, _ = run_timed(stages, sky, beam, w, "fp32", 1, 0)
-        v32 = flops_job / (st32["total_ms"] / 1e3) / 1e12
-        fp32 = {"value": v32, "ms_per_step": st32["total_ms"],
-                "herk_tflops": flops_job / st32["herk_n"] / (st32["herk_ms"] / st32["herk_n"] / 1e3) / 1e12}
-        log(f"[bench] fp32: {st32['total_ms']:.1f} ms/step, {v32:.1f} TFLOP/s")
+    def compare_leg(gemm):
+        log(f"[bench] {gemm} comparison: warmup {args.cmp_warmup}, timed {args.cmp_steps}")
+        stc, _ = run_timed(stages, sky, beam, w, gemm, args.cmp_steps, args.cmp_warmup)
+        ms = stc["total_ms"] / args.cmp_steps
+        res = {"value": flops_job / (ms / 1e3) / 1e12, "ms_per_step": ms, "steps": args.cmp_steps,
+               "warmup": args.cmp_warmup,
+               "herk_tflops": flops_job * args.cmp_steps / stc["herk_n"] / (stc["herk_ms"] / stc["herk_n"] / 1e3)
+               / 1e12, "herk_avg_ms": stc["herk_ms"] / stc["herk_n"]}
+        log(f"[bench] {gemm}: {ms:.1f} ms/step, {res['value']:.1f} TFLOP/s")
+        return res
+
+    fp32 = compare_leg("fp32") if not args.no_fp32 and args.gemm != "fp32" else None
+    fp16 = compare_leg("split_fp16") if not args.no_fp16 and args.gemm != "split_fp16" else None
 
     dropin = None
     if not args.no_dropin:
@@ -430,33 +494,37 @@ def main():
 
     accuracy = None
     if not args.no_accuracy:
-        # FP64 path (pinned to the oracle by the tests) on a sample of units
+        # FP64 path (pinned to the oracle by the tests) on every frequency of
+        # the first and the last time, all sources
         tsel = sorted({0, T - 1})
-        fsel = sorted({0, F // 3, (2 * F) // 3, F - 1})
         beam64 = stages.Beam(w.bm_cube.astype(np.float64), dtype=np.float64)
-        ref, _ = stages.simulate(sky, beam64, w.antpos, w.freqs[fsel], w.eq2tops[tsel], gemm="fp64",
-                                 dtype=np.float32)
-        accuracy = {"sample": {"times": tsel, "freqs_mhz": [round(float(w.freqs[i]) / 1e6, 3) for i in fsel]},
+        ref, _ = stages.simulate(sky, beam64, w.antpos, w.freqs, w.eq2tops[tsel], gemm="fp64", dtype=np.float32)
+        accuracy = {"sample": {"times": tsel, "freqs": "all", "freqs_mhz": [round(float(x) / 1e6, 3) for x in
+                                                                         w.freqs]},
+                    "metric": "per frequency: ||V - V_fp64||_F / ||V_fp64||_F over the sampled times and all "
+                              "baselines",
                     "reference": "FP64 GPU path (pinned to the FP64 oracle by tests)"}
         for gemm in ("split_bf16", "split_fp16", "fp32"):
-            v, _ = stages.simulate(sky, beam, w.antpos, w.freqs[fsel], w.eq2tops[tsel], gemm=gemm,
-                                   dtype=np.float32)
-            per_f = [rel_err(v[i], ref[i]) for i in range(len(fsel))]
+            v, _ = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[tsel], gemm=gemm, dtype=np.float32)
+            per_f = [rel_err(v[i], ref[i]) for i in range(F)]
             accuracy[gemm] = {"max_rel_err": max(per_f), "per_freq": per_f}
-            log(f"[bench] rel err {gemm}: max {max(per_f):.2e}")
+            log(f"[bench] rel err {gemm}: max {max(per_f):.2e} over {F} frequencies")
+            del v
+        del ref
         beam64.close()
 
     cpu = None
     if not args.no_cpu:
-        nsrc = min(args.cpu_nsrc, K)
+        nsrc = min(args.cpu_nsrc, K) if args.cpu_nsrc else K
+        workers = args.cpu_workers or min(16, len(os.sched_getaffinity(0)))
         sub = stages.Sky(w.crd_eq[:, :nsrc], w.I_sky[:nsrc])
         gpu_unit = {}
         for gemm in ("split_bf16", "fp32"):
             v, _ = stages.simulate(sub, beam, w.antpos, w.freqs[-1:], w.eq2tops[:1], gemm=gemm, dtype=np.float32)
             gpu_unit[gemm] = v[0, 0]
         sub.close()
-        log(f"[bench] cpu baseline on {nsrc} sources")
-        cpu = cpu_baseline(w, nsrc, gpu_unit)
+        log(f"[bench] cpu baseline: 1 unit, {nsrc} sources, {workers} processes")
+        cpu = cpu_baseline(w, nsrc, gpu_unit, workers)
         log(f"[bench] cpu: {cpu['seconds']:.1f} s, {cpu['value']:.4f} TFLOP/s")
 
     variants = None if args.no_variants else variants_leg(w, args.gemm)
@@ -465,8 +533,10 @@ def main():
 
     serving = None
     if args.serve_jobs > 0:
-        gemms = [args.gemm] + (["fp32"] if args.gemm != "fp32" and not args.no_fp32 else [])
-        serving = serving_leg(args.serve_jobs, gemms, args.serve_slots, args.serve_rate)
+        serving = serving_leg(args.serve_jobs, args.gemm, args.serve_slots, args.serve_rate,
+                              burst_jobs=min(args.serve_burst, args.serve_jobs),
+                              fp32_jobs=0 if args.no_fp32 else args.serve_fp32_jobs,
+                              graphs=not args.serve_no_graphs)
 
     peak = PEAK_TFLOPS[args.gemm]
     executed = {}
@@ -503,6 +573,7 @@ def main():
                    "GBps": phasor_bytes / (phasor_avg_ms / 1e3) / 1e9, "hbm_frac": phasor_bytes /
                    (phasor_avg_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
         "fp32": fp32,
+        "split_fp16": fp16,
         "speedup_vs_fp32": (value / fp32["value"]) if fp32 else None,
         "accuracy": accuracy,
         "dropin_pcie_inclusive": dropin,
@@ -513,6 +584,8 @@ def main():
         "medfilt": medf,
         "p99_latency_ms": serving[args.gemm]["p99_ms"] if serving else None,
         "p50_latency_ms": serving[args.gemm]["p50_ms"] if serving else None,
+        "serving_tflops": serving[args.gemm]["tflops"] if serving else None,
+        "serving_ratio_vs_fp32": serving["burst"].get("ratio_tflops") if serving else None,
         "wall_s_timed": wall,
     }
     print(json.dumps(out), flush=True)

@@ -21,6 +21,7 @@ HV_ERR_HIP = -2
 HV_ERR_NOMEM = -3
 HV_ERR_STATE = -4
 HV_ERR_BUSY = -5
+HV_ERR_CANCELED = -6
 
 HV_F32 = 0
 HV_F64 = 1
@@ -62,7 +63,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
                 ("graph_captures", C.c_int64), ("graph_launches", C.c_int64), ("direct_launches", C.c_int64),
                 ("peak_running", C.c_int64), ("pending", C.c_int64), ("running", C.c_int64),
-                ("workspace_bytes", C.c_size_t), ("budget_bytes", C.c_size_t)]
+                ("workspace_bytes", C.c_size_t), ("budget_bytes", C.c_size_t), ("cancelled", C.c_int64),
+                ("instances", C.c_int64), ("evictions", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,6 +107,7 @@ SIGNATURES = {
     "hv_server_poll": (_i, [_vp, C.POINTER(JobInfo), _i, C.POINTER(_i)]),
     "hv_server_wait": (_i, [_vp, _i64, _d, C.POINTER(JobInfo)]),
     "hv_server_stats": (_i, [_vp, C.POINTER(ServerStats)]),
+    "hv_server_cancel": (_i, [_vp, _i64]),
     "hv_server_destroy": (_i, [_vp]),
 }
 

@@ -4,8 +4,10 @@
 resident sky catalog and beam (``stages.Sky`` / ``stages.Beam``) and bring
 their own antenna positions, times and frequencies.  The native worker admits
 queued jobs into free slots as slots and device memory free up, runs them
-concurrently on per-slot streams and replays a hipGraph captured per slot and
-shape bucket.
+concurrently on per-slot streams.  Each job runs in an instance of its shape
+bucket (device buffers at fixed addresses plus the hipGraph captured from the
+first job that ran there); later jobs of the bucket replay that graph on
+whichever slot admits them.
 
 ``run_load`` drives a timed stream of jobs (arrival times in seconds) through
 a server and reports latency percentiles (arrival to result on the host) and
@@ -98,6 +100,10 @@ class Server:
         info = JobInfo()
         _lib.check(self.lib.hv_server_wait(self.handle, jid, timeout_s, C.byref(info)), "hv_server_wait")
         return self._done(info)
+
+    def cancel(self, jid: int) -> None:
+        """Cancel a queued or running job (it then finishes with HV_ERR_CANCELED)."""
+        _lib.check(self.lib.hv_server_cancel(self.handle, jid), "hv_server_cancel")
 
     def stats(self) -> dict:
         st = ServerStats()

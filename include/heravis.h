@@ -36,6 +36,7 @@ extern "C" {
 #define HV_ERR_NOMEM (-3)   /* request exceeds the HBM budget */
 #define HV_ERR_STATE (-4)   /* wrong object state (e.g. closed server) */
 #define HV_ERR_BUSY (-5)    /* non-blocking call would block */
+#define HV_ERR_CANCELED (-6) /* the job was cancelled (hv_server_cancel) */
 
 #define HV_F32 0
 #define HV_F64 1
@@ -274,11 +275,17 @@ int hv_set_herk_variant(int tile, int pf);
 int hv_herk_choice(int mode, int nant, int* tile, int* pf);
 
 /* ---------------------------------------------------------------------------
- * Serving (no reference counterpart: the reference has no request loop).
+ * Serving (no reference counterpart: the reference has no request loop; it
+ * recompiles and re-uploads on every call, vis.py:183-206).
  * A server admits queued jobs into slots as slots and device memory free up;
- * admitted jobs run concurrently on per-slot streams, replaying a hipGraph
- * captured once per slot and shape bucket.  Results are bitwise identical
- * to hv_simulate on the same inputs, whatever else is running.
+ * admitted jobs run concurrently on per-slot streams.  A job runs in an
+ * instance of its shape bucket (mode, N, P, T, F, beam type, FP16 scale,
+ * 2^ceil(log2 Kpad), layout): the device buffers of such a job at fixed
+ * addresses and the hipGraph captured from the first job that ran in it.
+ * Instances outlive their jobs, so a later job of the bucket replays the
+ * graph on whichever slot admits it; idle instances are evicted least
+ * recently used first when a new one does not fit the budget.  Results are
+ * bitwise identical to hv_simulate on the same inputs, whatever else runs.
  * ------------------------------------------------------------------------- */
 typedef struct hv_server hv_server;
 
@@ -302,7 +309,10 @@ typedef struct hv_job {
                                   or a baseline layout (flags), written before the
                                   job is reported finished; from submission on,
                                   its pages may be pre-faulted (zero bytes written)
-                                  until the result is copied in                     */
+                                  until the result is copied in.  Undefined after a
+                                  failed or cancelled job (some pages may hold
+                                  zeros); untouched by the server once the job is
+                                  reported finished                                 */
 } hv_job;
 
 typedef struct hv_job_info {
@@ -318,14 +328,18 @@ typedef struct hv_job_info {
 } hv_job_info;
 
 typedef struct hv_server_stats_t {
-    int64_t submitted, completed, failed;
+    int64_t submitted, completed, failed;   /* failed includes cancelled jobs      */
     int64_t graph_captures, graph_launches, direct_launches;
     int64_t peak_running, pending, running;
-    size_t workspace_bytes, budget_bytes;
+    size_t workspace_bytes, budget_bytes;    /* device bytes of all instances      */
+    int64_t cancelled;                       /* hv_server_cancel calls that took    */
+    int64_t instances;                       /* bucket instances held now           */
+    int64_t evictions;                       /* idle instances dropped for room     */
 } hv_server_stats_t;
 
-/* nslots in [1,16]; budget = device bytes for slot workspaces (0 = half of
- * what the context has left); use_graphs = 0 launches kernels directly. */
+/* nslots in [1,16]; budget = device bytes for instances (0 = half of what
+ * the context has left); use_graphs = 0 launches kernels directly (same
+ * instances, no capture). */
 int hv_server_create(hv_ctx* ctx, int nslots, size_t budget, int use_graphs, hv_server** out);
 /* Queue a job (non-blocking).  Shape errors return HV_ERR_ARG; a job that can
  * never fit the budget returns HV_ERR_NOMEM. */
@@ -334,6 +348,11 @@ int hv_server_submit(hv_server* srv, const hv_job* job, int64_t* id);
 int hv_server_poll(hv_server* srv, hv_job_info* infos, int max, int* n);
 /* Block until job id finishes (timeout_s < 0: no limit); HV_ERR_BUSY on timeout. */
 int hv_server_wait(hv_server* srv, int64_t id, double timeout_s, hv_job_info* info);
+/* Cancel job id.  A queued job finishes at once with HV_ERR_CANCELED; a running
+ * one finishes with HV_ERR_CANCELED when its device work ends (the kernels are
+ * not interrupted) and its result is not copied.  HV_ERR_ARG if the job is not
+ * queued or running (finished, collected or unknown). */
+int hv_server_cancel(hv_server* srv, int64_t id);
 int hv_server_stats(hv_server* srv, hv_server_stats_t* out);
 /* Fail queued jobs, wait for running ones, release everything. */
 int hv_server_destroy(hv_server* srv);

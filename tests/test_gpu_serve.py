@@ -67,16 +67,18 @@ def test_graph_replay_same_bucket(catalogs):
     antpos, beam, sky, eq, freqs = job_inputs(catalogs, 3, "disk128", 2, 2)
     eq2 = configs.eq2tops_lst(2, lst0_deg=77.0).astype(np.float64)
     eq3 = configs.eq2tops_lst(2, lst0_deg=151.0).astype(np.float64)
-    with Server(nslots=1) as srv:
-        # one bucket (16384 < Kpad <= 32768): the first job runs direct, the
-        # second is captured, the third replays the graph
+    with Server(nslots=2) as srv:
+        # one bucket (16384 < Kpad <= 32768): the first job captures the
+        # graph of its instance; the next ones, one at a time so they find the
+        # instance idle, replay it on whichever slot takes them
         a = srv.wait(srv.submit(sky, beam, antpos, freqs, eq, nsrc=20000), 120)
         b = srv.wait(srv.submit(sky, beam, antpos, freqs, eq2, nsrc=31000), 120)
         c = srv.wait(srv.submit(sky, beam, antpos, freqs, eq3, nsrc=17000), 120)
         st = srv.stats()
     assert a.status == 0 and b.status == 0 and c.status == 0
-    assert not a.graph_hit and not b.graph_hit and c.graph_hit
-    assert st["direct_launches"] == 1 and st["graph_captures"] == 1 and st["graph_launches"] == 2
+    assert not a.graph_hit and b.graph_hit and c.graph_hit
+    assert st["direct_launches"] == 0 and st["graph_captures"] == 1 and st["graph_launches"] == 3
+    assert st["instances"] == 1
     for d, e, n in ((a, eq, 20000), (b, eq2, 31000), (c, eq3, 17000)):
         alone, _ = stages.simulate(sky, beam, antpos, freqs, e, nsrc=n, gemm="split_bf16")
         np.testing.assert_array_equal(d.vis, alone)
@@ -185,3 +187,104 @@ def test_empty_source_set_job_gives_zero(catalogs):
     assert full.status == 0 and empty.status == 0, (full.error, empty.error)
     assert np.abs(full.vis).max() > 0
     np.testing.assert_array_equal(empty.vis, 0)
+
+
+def test_cancel_queued_and_running(catalogs):
+    # one slot: the first job runs, the others queue behind it; cancelling a
+    # queued job finishes it at once, cancelling the running one drops its
+    # result when its kernels end; the rest equal their stand-alone runs
+    antpos, beam, sky, eq, freqs = job_inputs(catalogs, 4, "disk128", 3, 5)
+    with Server(nslots=1) as srv:
+        ids = [srv.submit(sky, beam, antpos, freqs, eq, nsrc=39000 - 1000 * k) for k in range(4)]
+        srv.cancel(ids[2])
+        srv.cancel(ids[0])
+        res = {i: srv.wait(i, 120) for i in ids}
+        with pytest.raises(AssertionError):
+            srv.cancel(ids[1])            # finished and collected
+        st = srv.stats()
+    assert res[ids[2]].status == -6 and "before it started" in res[ids[2]].error
+    assert res[ids[0]].status in (0, -6)  # it may have finished before the request
+    assert st["cancelled"] >= 1
+    for k in (1, 3):
+        d = res[ids[k]]
+        assert d.status == 0, d.error
+        alone, _ = stages.simulate(sky, beam, antpos, freqs, eq, nsrc=39000 - 1000 * k, gemm="split_bf16")
+        np.testing.assert_array_equal(d.vis, alone)
+
+
+# ---------------------------------------------------------------------------
+# The serving load of the fifth workload: jobs drawn from
+# configs.make_serving_stream (N in {37, 128, 350}, Nsrc log-uniform in
+# [10^3, 2^20], T in {1, 8, 32}, F in {1, 16, 64}, beams shared per layout).
+# Selection from the first 400 jobs of the seed-5 stream, in stream order:
+# the first HERA-350 job with Nsrc >= 2^19, the first HERA-350 job with
+# T = 32, the first with F = 64, the first with T = 8 and F = 16, then the
+# remaining jobs in stream order whose output is at most 256 MB, up to 18.
+SERVE_TOL = {"split_bf16": 1e-5}   # relative Frobenius error vs the FP64 oracle
+SAMPLE_ANTS = {37: np.arange(37), 128: np.r_[0:20, 108:128],
+               350: np.r_[0:10, 160:170, 330:350]}   # 350: core, centre and the 20 outermost (outriggers)
+
+
+def _config5_jobs():
+    stream = configs.make_serving_stream(400, rate_hz=50.0)
+    nant = {"hex37": 37, "disk128": 128, "hera350": 350}
+    picks = []
+    rules = [lambda j: j.layout == "hera350" and j.nsrc >= 2 ** 19,
+             lambda j: j.layout == "hera350" and j.ntimes == 32,
+             lambda j: j.nfreq == 64,
+             lambda j: j.ntimes == 8 and j.nfreq == 16]
+    for rule in rules:
+        j = next(j for j in stream if rule(j) and j not in picks)
+        picks.append(j)
+    for j in stream:
+        if len(picks) >= 18:
+            break
+        out = nant[j.layout] ** 2 * j.ntimes * j.nfreq * 8
+        if j not in picks and out <= 256 << 20:
+            picks.append(j)
+    return sorted(picks, key=lambda j: j.job_id), nant
+
+
+@pytest.fixture(scope="module")
+def full_catalogs():
+    out = {}
+    for lay in ("hex37", "disk128", "hera350"):
+        antpos, beam, crd, flux = configs.serving_catalog(lay)
+        out[lay] = (antpos, beam, crd, flux, stages.Beam(beam), stages.Sky(crd, flux))
+    yield out
+    for v in out.values():
+        v[4].close()
+        v[5].close()
+
+
+def test_config5_mix_equals_alone_and_oracle(full_catalogs):
+    from oracle.vis_oracle import rel_err, vis_oracle_units
+
+    jobs, nant = _config5_jobs()
+    assert sum(j.layout == "hera350" for j in jobs) >= 3
+    assert any(j.layout == "hera350" and j.nsrc >= 2 ** 19 for j in jobs)
+    assert {8, 32} <= {j.ntimes for j in jobs} and {16, 64} <= {j.nfreq for j in jobs}
+    with Server(nslots=4) as srv:
+        ids = {}
+        for j in jobs:
+            antpos, _, _, _, beam, sky = full_catalogs[j.layout]
+            eq, freqs = configs.job_times_freqs(j)
+            ids[srv.submit(sky, beam, antpos, freqs, eq, nsrc=j.nsrc, gemm="split_bf16")] = j
+        done = {jid: srv.wait(jid, 300) for jid in ids}
+        st = srv.stats()
+    assert st["peak_running"] >= 2 and st["failed"] == 0
+    for jid, j in ids.items():
+        d = done[jid]
+        assert d.status == 0, d.error
+        antpos, bm, crd, flux, beam, sky = full_catalogs[j.layout]
+        eq, freqs = configs.job_times_freqs(j)
+        alone, _ = stages.simulate(sky, beam, antpos, freqs, eq, nsrc=j.nsrc, gemm="split_bf16")
+        np.testing.assert_array_equal(d.vis, alone, err_msg=f"job {j.job_id}")
+        # first and last (frequency, time) unit on the antenna sample
+        S = SAMPLE_ANTS[nant[j.layout]]
+        units = sorted({(0, 0), (j.nfreq - 1, j.ntimes - 1)})
+        ref = vis_oracle_units(antpos[S], freqs, eq, crd[:, :j.nsrc], flux[:j.nsrc], bm[S], units)
+        for k, (f, t) in enumerate(units):
+            err = rel_err(d.vis[f, t][np.ix_(S, S)], ref[k])
+            assert err < SERVE_TOL["split_bf16"], f"job {j.job_id} unit ({f}, {t}): rel err {err:.2e}"
+        del d.vis, alone
