@@ -78,6 +78,7 @@ SIGNATURES = {
     "hv_ctx_destroy": (_i, [_vp]),
     "hv_vis": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _vp]),
     "hv_vis_ex": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _i, _i, _vp]),
+    "hv_vis_device_noalloc": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _vp, _vp]),
     "hv_vis_pol": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i, _vp]),
     "hv_omnical": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, _i64, C.c_double, _i, _i, _i, C.c_double,
                         _vp, _vp, _vp]),

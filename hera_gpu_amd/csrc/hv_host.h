@@ -99,6 +99,9 @@ void release(hv_ctx* c, Workspace& ws);
 // serialised by its lock.
 int copy_h2d(hv_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s);
 int copy_d2h(hv_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t s);
+// Device address of the no-allocation entry point's scratch array on the
+// current device (looked up once per device).
+int noalloc_scratch(int device, void** ptr);
 
 float fp16_scale(double max_amp);
 
@@ -137,6 +140,8 @@ struct JobDev {
     const int64_t* dyn_kpad;    // optional device-side source count
     int layout = 0;             // output layout (Layout)
     size_t beam_fstride = 0;    // elements between per-frequency beam planes (0: one beam)
+    int t_base = 0;             // time index of this job's first time in `vis`
+    int ntimes_out = 0;         // times of the `vis` array (0: ntimes)
 };
 
 struct StageTimer {

@@ -423,10 +423,10 @@ static int enqueue_reduce(const JobDev& j, const Plan& p, void* part, int t0, in
     ReduceArgs ra{};
     ra.partials = part;
     ra.vis = j.vis;
-    ra.t0 = t0;
+    ra.t0 = j.t_base + t0;
     ra.f0 = f0;
     ra.nfreq_launch = nf;
-    ra.ntimes_total = j.ntimes;
+    ra.ntimes_total = j.ntimes_out ? j.ntimes_out : j.ntimes;
     ra.nant = j.nant;
     ra.nb = p.nb;
     ra.ntiles = p.ntiles;

@@ -176,6 +176,32 @@ int hv_simulate(hv_ctx* ctx, int mode, const hv_sky* sky, int64_t nsrc_used,
                 void* vis_out, int reps, hv_stage_times* times);
 
 /* ---------------------------------------------------------------------------
+ * The whole simulation on device arrays, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels as hv_vis and results bitwise equal to hv_vis on
+ * the same values.
+ *   mode   : HV_MODE_FP32 or HV_MODE_SPLIT_BF16;
+ *   antpos (N,3) m, freqs (F) Hz, eq2tops (T,3,3), crd_eq (3,nsrc) and isqrt
+ *            (nsrc): device float64; bm_cube (N,P,P): device float32;
+ *   vis_out: device (F,T,N,N) complex64;
+ *   nsrc   : a multiple of 64 (pad with zero directions: below the horizon,
+ *            they contribute nothing);
+ *   stream : a hipStream_t (NULL: the default stream).  The call only
+ *            enqueues work on it; the arrays must stay valid until it ends.
+ * Working memory (rotated coordinates, A sqrt(I) or operands, partial tiles)
+ * is a static device array of HV_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; times are processed in batches that fit it, and a call
+ * whose single time does not fit fails with HV_ERR_NOMEM (e.g. N = 350 fits up
+ * to ~1.5e5 sources, N = 37 up to ~1.5e6).  All calls on one GPU share that
+ * array: calls that overlap on one GPU race.
+ * ------------------------------------------------------------------------- */
+#define HV_NOALLOC_SCRATCH_BYTES (256u << 20)
+int hv_vis_device_noalloc(hv_ctx* ctx, int mode, const double* antpos, int nant, const double* freqs, int nfreq,
+                          const double* eq2tops, int ntimes, const double* crd_eq, const double* isqrt,
+                          int64_t nsrc, const float* bm_cube, int beam_px, void* vis_out, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Polarised (Jones) beams.  No counterpart in the reference, whose beam is a
  * real scalar per antenna (vis.py:162-175, "XXX complex?" at vis.py:175).
  * jones: complex [nant][2 feeds p][2 sky axes k][beam_px][beam_px], (re, im)
