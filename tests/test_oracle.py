@@ -258,3 +258,18 @@ def test_pins_catch_wrong_frequency_plane():
     ref = brute_force(antpos, freqs, eq, crd, I, bm4)
     for wrong in (np.stack([bm4[0], bm4[0]]), bm4[::-1]):
         assert vo.rel_err(vo.vis_oracle(antpos, freqs, eq, crd, I, wrong), ref) > 1e-3
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_vis_cpu_parallel_equals_vis_cpu_bitwise(precision):
+    # the CPU baseline's process-parallel form (per-antenna rows on spawned
+    # workers, shared memory) against the restated vis_cpu: same bits
+    from hera_gpu_amd import configs
+    from oracle.cpu_ref import vis_cpu_parallel
+
+    w = configs.make_hex37()
+    a = vis_cpu(w.antpos, 150e6, w.eq2tops[:3], w.crd_eq, w.I_sky, w.bm_cube, precision=precision)
+    b = vis_cpu_parallel(w.antpos, 150e6, w.eq2tops[:3], w.crd_eq, w.I_sky, w.bm_cube, precision=precision,
+                         workers=3)
+    assert a.dtype == b.dtype
+    np.testing.assert_array_equal(a, b)
