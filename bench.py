@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE x1.0.
 MEASURED_TRAFFIC = {
     # 192-tile in-place build (herk_t192_kernel), two times (128 units) per launch
-    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 6441098, "write_kib": 8666624, "fetch_scale": 2.0,
+    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 6208934, "write_kib": 8660992, "fetch_scale": 2.0,
                                           "write_scale": 1.0},
     # 128-tile in-place build (--herk-variant 128,5), one time per launch
     ("hera350", "split_bf16", 64, 128): {"fetch_kib": 138366362, "write_kib": 4325376, "fetch_scale": 2.0,

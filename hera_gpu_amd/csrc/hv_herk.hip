@@ -1620,39 +1620,73 @@ struct T192Rows {
     __device__ static int row0(int wave) { return DIAG ? (IM ? 128 + 16 * (wave - 4) : 32 * wave) : 48 * wave; }
 };
 
-// Operand rows of one lane (T192Rows) for its source group of the stage,
-// with the arithmetic of fused8_build; ax/ay/az: the rows' antenna positions.
+// Operand buffers of the 192-tile kernel (its own LDS layout; no other kernel
+// reads it).  Row r (A block rows 0-191, then B block rows, 128 bytes each)
+// holds the stage's 16 sources as four 16-byte chunks of 4 sources, hi and lo
+// halves: chunk 2 g + h for source group g.  A chunk holds its 4 sources
+// planar, (x0 x1 x2 x3 y0 y1 y2 y3) as 16-bit values: both operands of every
+// MFMA use the same order, so each output sums the products of the
+// interleaved order, and a rotation (y, -x) is a swap of word pairs and two
+// sign flips.  The chunk index is XOR-ed with (r >> 1) & 7, which spreads the
+// 16 rows of a 16-lane phase of ds_read_b128 / ds_write_b128 over all 64
+// banks; every term but the row's 32-row block is then fixed per lane, so
+// block offsets ride in the instructions' immediate offsets.
+__device__ __forceinline__ int t192_off(int r, int q) { return r * STAGE_BYTES + ((q ^ ((r >> 1) & 7)) << 4); }
+
+// hi and lo words of two values a = am_a c_a, b = am_b c_b, a in the low half:
+// hi = RNE(round(am c)) and lo = RNE(fma(am, c, -hi)) per value, the
+// arithmetic of split_phasor (no contraction), so the halves are those of
+// every other operand builder.
+template <int MODE>
+__device__ __forceinline__ void split_two(float am_a, float c_a, float am_b, float c_b, uint32_t& hi, uint32_t& lo) {
+#pragma clang fp contract(off)
+    const float va = am_a * c_a, vb = am_b * c_b;
+    if constexpr (MODE == MODE_SPLIT_BF16) {
+        hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((hv_f32x2){va, vb}, hv_bf16x2));
+        const float ha = __uint_as_float(hi << 16), hb = __uint_as_float(hi & 0xffff0000u);
+        lo = __builtin_bit_cast(uint32_t,
+                                __builtin_convertvector((hv_f32x2){fmaf(am_a, c_a, -ha), fmaf(am_b, c_b, -hb)}, hv_bf16x2));
+    } else {
+        const hv_f16x2 h = __builtin_convertvector((hv_f32x2){va, vb}, hv_f16x2);
+        hi = __builtin_bit_cast(uint32_t, h);
+        const hv_f32x2 b = __builtin_convertvector(h, hv_f32x2);
+        lo = __builtin_bit_cast(uint32_t,
+                                __builtin_convertvector((hv_f32x2){fmaf(am_a, c_a, -b[0]), fmaf(am_b, c_b, -b[1])}, hv_f16x2));
+    }
+}
+
+// Operand rows of one lane (T192Rows) for its source group of the stage;
+// ax/ay/az: the rows' antenna positions in wavelengths.  Phases and halves
+// are those of fused8_build; only their places in LDS differ.
 template <int MODE, int NR>
 __device__ __forceinline__ void t192_build(const lds_char* in, lds_char* op, int row0, int lane,
                                            const double (&ax)[3], const double (&ay)[3], const double (&az)[3]) {
     using L = T192;
-    const int g = lane >> 4;
+    const int g = lane >> 4, l16 = lane & 15;
+    const lds_char* cin = in + L::AMP + 32 * g;                                     // x; y at +128, z at +256
+    const lds_char* ain = in + (row0 + l16) * 64 + ((g ^ ((l16 >> 1) & 3)) << 4);   // +1024 per 16 rows
+    lds_char* wop = op + t192_off(row0 + l16, 2 * g);                               // +2048 per 16 rows
+    const int wlo = (((2 * g + 1) ^ ((l16 >> 1) & 7)) - ((2 * g) ^ ((l16 >> 1) & 7))) << 4;
     double x[4], y[4], z[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        x[j] = lds_ld<double>(in + L::AMP + 32 * g + 8 * j);
-        y[j] = lds_ld<double>(in + L::AMP + 128 + 32 * g + 8 * j);
-        z[j] = lds_ld<double>(in + L::AMP + 256 + 32 * g + 8 * j);
+        x[j] = lds_ld<double>(cin + 8 * j);
+        y[j] = lds_ld<double>(cin + 128 + 8 * j);
+        z[j] = lds_ld<double>(cin + 256 + 8 * j);
     }
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const int r = row0 + (lane & 15) + 16 * k;
-        const f32x4 am = lds_ld<f32x4>(in + r * 64 + ((g ^ amp_swz(r)) << 4));
-        u32x4 hi, lo;
+        const f32x4 am = lds_ld<f32x4>(ain + 1024 * k);
+        float sn[4], cs[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float sn, cs;
-            sincos_turns_hw(phase_turns(ax[k], ay[k], az[k], x[j], y[j], z[j]), &sn, &cs);
-            uint32_t hw, lw;
-            split_phasor<MODE>(am[j], cs, sn, hw, lw);
-            hi[j] = hw;
-            lo[j] = lw;
-        }
-        const bool bblk = r >= L::TILE;
-        lds_char* base = bblk ? op + L::OP : op;
-        const int rr = bblk ? r - L::TILE : r;
-        lds_st(base + lds_off<true>(rr, 2 * g), hi);
-        lds_st(base + lds_off<true>(rr, 2 * g + 1), lo);
+        for (int j = 0; j < 4; ++j) sincos_turns_hw(phase_turns(ax[k], ay[k], az[k], x[j], y[j], z[j]), &sn[j], &cs[j]);
+        uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
+        split_two<MODE>(am[0], cs[0], am[1], cs[1], h0, l0);
+        split_two<MODE>(am[2], cs[2], am[3], cs[3], h1, l1);
+        split_two<MODE>(am[0], sn[0], am[1], sn[1], h2, l2);
+        split_two<MODE>(am[2], sn[2], am[3], sn[3], h3, l3);
+        lds_st(wop + 2048 * k, u32x4{h0, h1, h2, h3});
+        lds_st(wop + 2048 * k + wlo, u32x4{l0, l1, l2, l3});
     }
 }
 
@@ -1665,48 +1699,63 @@ __device__ __forceinline__ void t192_mac(const u32x4& ahi, const u32x4& alo, con
     acc = mfma16<MODE>(alo, bhi, acc);
 }
 
-// a' with a' . b = a . rot(b) = a_y b_x - a_x b_y: (a_y, -a_x) per 16-bit pair.
-__device__ inline u32x4 rot_a16(u32x4 v) {
-    u32x4 r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = __builtin_amdgcn_alignbit(v[i], v[i], 16) ^ 0x80000000u;
-    return r;
+// Planar chunks (x0 x1 x2 x3 y0 y1 y2 y3): a' = (a_y, -a_x), with
+// a' . b = a . rot(b) = a_y b_x - a_x b_y (rotating the A side), and
+// b' = (-b_y, b_x) (rotating the B side).  The same two products per source
+// land in the same MFMA lanes either way.
+__device__ __forceinline__ u32x4 t192_rot_a(u32x4 v) {
+    return u32x4{v[2], v[3], v[0] ^ 0x80008000u, v[1] ^ 0x80008000u};
+}
+__device__ __forceinline__ u32x4 t192_rot_b(u32x4 v) {
+    return u32x4{v[2] ^ 0x80008000u, v[3] ^ 0x80008000u, v[0], v[1]};
 }
 
-// Fragment of one 32-row block; ROT 1: rotated as B (rot_neg16), 2: as A (rot_a16).
+// Fragment of 32-row block R (rows 32 R + lane % 32) for one step; fo: the
+// lane's byte offsets of its hi and lo chunks inside a block (T192Frag).
+// ROT 1: rotated as B, 2: as A.
+struct T192Frag {
+    int hi[2], lo[2];   // per step s
+    __device__ explicit T192Frag(int lane) {
+        const int lrow = lane & 31, lh = lane >> 5;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            hi[s] = t192_off(lrow, 2 * (2 * s + lh));
+            lo[s] = t192_off(lrow, 2 * (2 * s + lh) + 1);
+        }
+    }
+};
 template <int ROT>
-__device__ __forceinline__ void t192_frag(const lds_char* S, int row, int gg, u32x4& hi, u32x4& lo) {
-    hi = lds_ld<u32x4>(S + lds_off<true>(row, 2 * gg));
-    lo = lds_ld<u32x4>(S + lds_off<true>(row, 2 * gg + 1));
+__device__ __forceinline__ void t192_frag(const lds_char* S, int R, int fh, int fl, u32x4& hi, u32x4& lo) {
+    hi = lds_ld<u32x4>(S + fh + 32 * STAGE_BYTES * R);
+    lo = lds_ld<u32x4>(S + fl + 32 * STAGE_BYTES * R);
     if constexpr (ROT == 1) {
-        hi = rot_neg16(hi);
-        lo = rot_neg16(lo);
+        hi = t192_rot_b(hi);
+        lo = t192_rot_b(lo);
     } else if constexpr (ROT == 2) {
-        hi = rot_a16(hi);
-        lo = rot_a16(lo);
+        hi = t192_rot_a(hi);
+        lo = t192_rot_a(lo);
     }
 }
 
-// Off-diagonal tiles: the 3x3 blocks of quadrant (wr, wc), one component,
-// both steps of a stage.
+// Off-diagonal tiles: the 3x3 blocks of one quadrant (As, Bs: its first A and
+// B block rows), one component, both steps of a stage.
 template <int MODE, bool IM>
-__device__ __forceinline__ void t192_quad_mma(const lds_char* As, const lds_char* Bs, int ra0, int rb0, int lh,
+__device__ __forceinline__ void t192_quad_mma(const lds_char* As, const lds_char* Bs, const T192Frag& fo,
                                               f32x16 (&acc)[9]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int gg = 2 * s + lh;
         // fragments in order of first use (LDS reads return in order)
         u32x4 bhi[3], blo[3];
         u32x4 ahi, alo;
-        t192_frag<0>(As, ra0, gg, ahi, alo);
+        t192_frag<0>(As, 0, fo.hi[s], fo.lo[s], ahi, alo);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            t192_frag<IM ? 1 : 0>(Bs, rb0 + 32 * j, gg, bhi[j], blo[j]);
+            t192_frag<IM ? 1 : 0>(Bs, j, fo.hi[s], fo.lo[s], bhi[j], blo[j]);
             t192_mac<MODE>(ahi, alo, bhi[j], blo[j], acc[j]);
         }
 #pragma unroll
         for (int i = 1; i < 3; ++i) {
-            t192_frag<0>(As, ra0 + 32 * i, gg, ahi, alo);
+            t192_frag<0>(As, i, fo.hi[s], fo.lo[s], ahi, alo);
 #pragma unroll
             for (int j = 0; j < 3; ++j) t192_mac<MODE>(ahi, alo, bhi[j], blo[j], acc[3 * i + j]);
         }
@@ -1715,60 +1764,60 @@ __device__ __forceinline__ void t192_quad_mma(const lds_char* As, const lds_char
 
 // Diagonal tiles: block row R, columns C0..C1 into acc[K0 ...], one step.
 template <int MODE, bool IM, int R, int C0, int C1, int K0>
-__device__ __forceinline__ void t192_diag_row(const lds_char* S, int lrow, int gg, f32x16 (&acc)[9]) {
+__device__ __forceinline__ void t192_diag_row(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
     // Im: the one A fragment of the row is rotated instead of its B fragments
     constexpr int RA = IM ? 2 : 0;
     u32x4 ahi, alo;
-    t192_frag<RA>(S, 32 * R + lrow, gg, ahi, alo);
+    t192_frag<RA>(S, R, fh, fl, ahi, alo);
 #pragma unroll
     for (int c = C0; c <= C1; ++c) {
         u32x4 bhi, blo;
-        t192_frag<0>(S, 32 * c + lrow, gg, bhi, blo);
+        t192_frag<0>(S, c, fh, fl, bhi, blo);
         t192_mac<MODE>(ahi, alo, bhi, blo, acc[K0 + c - C0]);
     }
 }
 
 template <int MODE, bool IM, int Q>
-__device__ __forceinline__ void t192_diag_mma(const lds_char* S, int lrow, int lh, f32x16 (&acc)[9]) {
+__device__ __forceinline__ void t192_diag_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int gg = 2 * s + lh;
-        if constexpr (Q == 0) t192_diag_row<MODE, IM, 0, 0, 5, 0>(S, lrow, gg, acc);
-        if constexpr (Q == 1) t192_diag_row<MODE, IM, 1, 1, 5, 0>(S, lrow, gg, acc);
+        const int fh = fo.hi[s], fl = fo.lo[s];
+        if constexpr (Q == 0) t192_diag_row<MODE, IM, 0, 0, 5, 0>(S, fh, fl, acc);
+        if constexpr (Q == 1) t192_diag_row<MODE, IM, 1, 1, 5, 0>(S, fh, fl, acc);
         if constexpr (Q == 2) {
-            t192_diag_row<MODE, IM, 2, 2, 5, 0>(S, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 5, 5, 5, 4>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 2, 2, 5, 0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, 5, 5, 5, 4>(S, fh, fl, acc);
         }
         if constexpr (Q == 3) {
-            t192_diag_row<MODE, IM, 3, 3, 5, 0>(S, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 4, 4, 5, 3>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 3, 3, 5, 0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, 4, 4, 5, 3>(S, fh, fl, acc);
         }
         // the 15 blocks of a diagonal tile with five block rows inside the
         // array (the last tile at N = 321..352), 4, 4, 4, 3 per SIMD
-        if constexpr (Q == 4) t192_diag_row<MODE, IM, 0, 0, 3, 0>(S, lrow, gg, acc);
-        if constexpr (Q == 5) t192_diag_row<MODE, IM, 1, 1, 4, 0>(S, lrow, gg, acc);
+        if constexpr (Q == 4) t192_diag_row<MODE, IM, 0, 0, 3, 0>(S, fh, fl, acc);
+        if constexpr (Q == 5) t192_diag_row<MODE, IM, 1, 1, 4, 0>(S, fh, fl, acc);
         if constexpr (Q == 6) {
-            t192_diag_row<MODE, IM, 2, 2, 4, 0>(S, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 0, 4, 4, 3>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 2, 2, 4, 0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, 0, 4, 4, 3>(S, fh, fl, acc);
         }
         if constexpr (Q == 7) {
-            t192_diag_row<MODE, IM, 3, 3, 4, 0>(S, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 4, 4, 4, 2>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 3, 3, 4, 0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, 4, 4, 4, 2>(S, fh, fl, acc);
         }
         // the 10 blocks of a diagonal tile with at most four block rows
         // inside the array (N <= 128 at the last block), 3, 3, 2, 2 per SIMD
-        if constexpr (Q == 8) t192_diag_row<MODE, IM, 0, 0, 2, 0>(S, lrow, gg, acc);
+        if constexpr (Q == 8) t192_diag_row<MODE, IM, 0, 0, 2, 0>(S, fh, fl, acc);
         if constexpr (Q == 9) {
-            t192_diag_row<MODE, IM, 0, 3, 3, 0>(S, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 1, 1, 2, 1>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 0, 3, 3, 0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, 1, 1, 2, 1>(S, fh, fl, acc);
         }
         if constexpr (Q == 10) {
-            t192_diag_row<MODE, IM, 1, 3, 3, 0>(S, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 2, 2, 2, 1>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 1, 3, 3, 0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, 2, 2, 2, 1>(S, fh, fl, acc);
         }
         if constexpr (Q == 11) {
-            t192_diag_row<MODE, IM, 2, 3, 3, 0>(S, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 3, 3, 3, 1>(S, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 2, 3, 3, 0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, 3, 3, 3, 1>(S, fh, fl, acc);
         }
     }
 }
@@ -1778,7 +1827,7 @@ __device__ __forceinline__ void t192_diag_mma(const lds_char* S, int lrow, int l
 // C0..C1 per SIMD (8, 8, 8, 6 blocks instead of 9, 9, 9, 9 with 6 wasted),
 // B fragments held, A streamed.
 template <int MODE, bool IM, int R0, int R1, int C0, int C1>
-__device__ __forceinline__ void t192_rect_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
+__device__ __forceinline__ void t192_rect_mma(const lds_char* As, const lds_char* Bs, const T192Frag& fo,
                                               f32x16 (&acc)[9]) {
     constexpr int NC = C1 - C0 + 1;
     // Im: rotate the side with fewer fragments per step
@@ -1786,18 +1835,18 @@ __device__ __forceinline__ void t192_rect_mma(const lds_char* As, const lds_char
     constexpr int RA = (IM && ROTA) ? 2 : 0, RB = (IM && !ROTA) ? 1 : 0;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int gg = 2 * s + lh;
+        const int fh = fo.hi[s], fl = fo.lo[s];
         u32x4 bhi[NC], blo[NC];
         u32x4 ahi, alo;
-        t192_frag<RA>(As, 32 * R0 + lrow, gg, ahi, alo);
+        t192_frag<RA>(As, R0, fh, fl, ahi, alo);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            t192_frag<RB>(Bs, 32 * (C0 + c) + lrow, gg, bhi[c], blo[c]);
+            t192_frag<RB>(Bs, C0 + c, fh, fl, bhi[c], blo[c]);
             t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[c]);
         }
 #pragma unroll
         for (int r = R0 + 1; r <= R1; ++r) {
-            t192_frag<RA>(As, 32 * r + lrow, gg, ahi, alo);
+            t192_frag<RA>(As, r, fh, fl, ahi, alo);
 #pragma unroll
             for (int c = 0; c < NC; ++c) t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[(r - R0) * NC + c]);
         }
@@ -1805,17 +1854,17 @@ __device__ __forceinline__ void t192_rect_mma(const lds_char* As, const lds_char
 }
 
 template <int MODE, bool IM, int Q>
-__device__ __forceinline__ void t192_rect5_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
+__device__ __forceinline__ void t192_rect5_mma(const lds_char* As, const lds_char* Bs, const T192Frag& fo,
                                                f32x16 (&acc)[9]) {
-    if constexpr (Q == 0) t192_rect_mma<MODE, IM, 0, 1, 0, 3>(As, Bs, lrow, lh, acc);
-    if constexpr (Q == 1) t192_rect_mma<MODE, IM, 2, 3, 0, 3>(As, Bs, lrow, lh, acc);
-    if constexpr (Q == 2) t192_rect_mma<MODE, IM, 4, 5, 0, 3>(As, Bs, lrow, lh, acc);
-    if constexpr (Q == 3) t192_rect_mma<MODE, IM, 0, 5, 4, 4>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 0) t192_rect_mma<MODE, IM, 0, 1, 0, 3>(As, Bs, fo, acc);
+    if constexpr (Q == 1) t192_rect_mma<MODE, IM, 2, 3, 0, 3>(As, Bs, fo, acc);
+    if constexpr (Q == 2) t192_rect_mma<MODE, IM, 4, 5, 0, 3>(As, Bs, fo, acc);
+    if constexpr (Q == 3) t192_rect_mma<MODE, IM, 0, 5, 4, 4>(As, Bs, fo, acc);
     // at most four B column blocks inside the array: 6 blocks per SIMD
-    if constexpr (Q == 4) t192_rect_mma<MODE, IM, 0, 1, 0, 2>(As, Bs, lrow, lh, acc);
-    if constexpr (Q == 5) t192_rect_mma<MODE, IM, 2, 3, 0, 2>(As, Bs, lrow, lh, acc);
-    if constexpr (Q == 6) t192_rect_mma<MODE, IM, 4, 5, 0, 2>(As, Bs, lrow, lh, acc);
-    if constexpr (Q == 7) t192_rect_mma<MODE, IM, 0, 5, 3, 3>(As, Bs, lrow, lh, acc);
+    if constexpr (Q == 4) t192_rect_mma<MODE, IM, 0, 1, 0, 2>(As, Bs, fo, acc);
+    if constexpr (Q == 5) t192_rect_mma<MODE, IM, 2, 3, 0, 2>(As, Bs, fo, acc);
+    if constexpr (Q == 6) t192_rect_mma<MODE, IM, 4, 5, 0, 2>(As, Bs, fo, acc);
+    if constexpr (Q == 7) t192_rect_mma<MODE, IM, 0, 5, 3, 3>(As, Bs, fo, acc);
 }
 
 // The first and the last diagonal tile as one item (As: the first block's
@@ -1824,29 +1873,29 @@ __device__ __forceinline__ void t192_rect5_mma(const lds_char* As, const lds_cha
 // last, 9, 9, 9, 9 (8, 8, 8, 7) per SIMD, where the two tiles apart take 6
 // and 4 (3) per SIMD in two items.
 template <int MODE, bool IM, int Q, int BC>
-__device__ __forceinline__ void t192_pair_mma(const lds_char* As, const lds_char* Bs, int lrow, int lh,
+__device__ __forceinline__ void t192_pair_mma(const lds_char* As, const lds_char* Bs, const T192Frag& fo,
                                               f32x16 (&acc)[9]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int gg = 2 * s + lh;
+        const int fh = fo.hi[s], fl = fo.lo[s];
         if constexpr (Q == 0) {
-            t192_diag_row<MODE, IM, 0, 0, 5, 0>(As, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 3, 3, BC, 6>(Bs, lrow, gg, acc);
-            if constexpr (BC == 4) t192_diag_row<MODE, IM, 4, 4, 4, 8>(Bs, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 0, 0, 5, 0>(As, fh, fl, acc);
+            t192_diag_row<MODE, IM, 3, 3, BC, 6>(Bs, fh, fl, acc);
+            if constexpr (BC == 4) t192_diag_row<MODE, IM, 4, 4, 4, 8>(Bs, fh, fl, acc);
         }
         if constexpr (Q == 1) {
-            t192_diag_row<MODE, IM, 1, 1, 5, 0>(As, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 1, 1, BC, 5>(Bs, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 1, 1, 5, 0>(As, fh, fl, acc);
+            t192_diag_row<MODE, IM, 1, 1, BC, 5>(Bs, fh, fl, acc);
         }
         if constexpr (Q == 2) {
-            t192_diag_row<MODE, IM, 2, 2, 5, 0>(As, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 0, 0, BC, 4>(Bs, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 2, 2, 5, 0>(As, fh, fl, acc);
+            t192_diag_row<MODE, IM, 0, 0, BC, 4>(Bs, fh, fl, acc);
         }
         if constexpr (Q == 3) {
-            t192_diag_row<MODE, IM, 3, 3, 5, 0>(As, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 4, 4, 5, 3>(As, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 5, 5, 5, 5>(As, lrow, gg, acc);
-            t192_diag_row<MODE, IM, 2, 2, BC, 6>(Bs, lrow, gg, acc);
+            t192_diag_row<MODE, IM, 3, 3, 5, 0>(As, fh, fl, acc);
+            t192_diag_row<MODE, IM, 4, 4, 5, 3>(As, fh, fl, acc);
+            t192_diag_row<MODE, IM, 5, 5, 5, 5>(As, fh, fl, acc);
+            t192_diag_row<MODE, IM, 2, 2, BC, 6>(Bs, fh, fl, acc);
         }
     }
 }
@@ -1946,8 +1995,9 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         az[k] = antpos[3 * ant + 2] * il;
     }
     const int q = wave & 3;
-    const int lrow = lane & 31, lh = lane >> 5;
-    const int ra0 = (q >> 1) * 96 + lrow, rb0 = (q & 1) * 96 + lrow;
+    const T192Frag fo(lane);
+    // off-diagonal quadrant (q / 2, q % 2): its first A and B block rows
+    const int qa = (q >> 1) * 96 * STAGE_BYTES, qb = L::OP + (q & 1) * 96 * STAGE_BYTES;
 #pragma unroll
     for (int k = 0; k < 9; ++k)
 #pragma unroll
@@ -1957,10 +2007,10 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
                      ampoff, crdoff, wave, lane)
 #define HV_T192MMA(buf)                                                                   \
     do {                                                                                   \
-        if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? QD : 0)>(buf, lrow, lh, acc);  \
-        else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, lrow, lh, acc); \
-        else if constexpr (PAIR) t192_pair_mma<MODE, IM, QP, BCP>(buf, buf + L::OP, lrow, lh, acc); \
-        else t192_quad_mma<MODE, IM>(buf, buf + L::OP, ra0, rb0, lh, acc);                 \
+        if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? QD : 0)>(buf, fo, acc);          \
+        else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, fo, acc); \
+        else if constexpr (PAIR) t192_pair_mma<MODE, IM, QP, BCP>(buf, buf + L::OP, fo, acc); \
+        else t192_quad_mma<MODE, IM>(buf + qa, buf + qb, fo, acc);                             \
     } while (0)
 
     HV_T192ISSUE(in0, 0);
