@@ -69,7 +69,7 @@ extern "C" int hv_vis_device_noalloc(hv_ctx* c, int mode, const double* antpos, 
     if (nsrc <= 0 || nsrc % KALIGN != 0)
         return fail(HV_ERR_ARG, "nsrc must be a positive multiple of 64 (pad with zero directions, which "
                                 "are below the horizon and contribute nothing)");
-    if (beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "beam_px must be in [1, 128]");
+    if (beam_px < 1 || beam_px > 2048) return fail(HV_ERR_ARG, "beam_px must be in [1, 2048]");
     HV_HIP(hipSetDevice(c->device));
     void* base = nullptr;
     HV_CHECK(noalloc_scratch(c->device, &base));

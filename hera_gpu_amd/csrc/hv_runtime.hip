@@ -571,6 +571,11 @@ static int run_job(hv_ctx* c, const JobDev& j, StageTimer& tm) {
 }
 
 // ---------------------------------------------------------- argument checks
+// Beams up to 2048 pixels a side: planes that fit in LDS are staged there
+// (P <= 202 for float32, 143 for float64; polarised beams stage four planes,
+// P <= 101 / 71), larger ones are read from global memory by the same
+// bilinear arithmetic.
+constexpr int MAX_BEAM_PX = 2048;
 static int check_job_args(int mode, int dtype, int nant, int nfreq, int ntimes, int64_t nsrc, int beam_px) {
     HV_CHECK(check_mode(mode));
     HV_CHECK(check_dtype(dtype));
@@ -578,7 +583,7 @@ static int check_job_args(int mode, int dtype, int nant, int nfreq, int ntimes, 
     if (nfreq <= 0) return fail(HV_ERR_ARG, "nfreq must be positive");
     if (ntimes <= 0) return fail(HV_ERR_ARG, "ntimes must be positive");
     if (nsrc < 0) return fail(HV_ERR_ARG, "nsrc must be non-negative");
-    if (beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "beam_px must be in [1, 128]");
+    if (beam_px < 1 || beam_px > MAX_BEAM_PX) return fail(HV_ERR_ARG, "beam_px must be in [1, 2048]");
     if (nsrc > ((int64_t)1 << 36)) return fail(HV_ERR_ARG, "nsrc too large");
     return HV_OK;
 }
@@ -737,9 +742,6 @@ extern "C" int hv_vis_pol(hv_ctx* c, int mode, int dtype, const void* antpos, in
                           int64_t nsrc, const void* jones, int beam_px, void* vis_out) {
     if (!c) return fail(HV_ERR_ARG, "ctx is NULL");
     HV_CHECK(check_job_args(mode, dtype, nant, nfreq, ntimes, nsrc, beam_px));
-    if (beam_px > (dtype == HV_F64 ? 71 : 101))
-        return fail(HV_ERR_ARG, "polarised beams: the four planes of a feed must fit in LDS (beam_px <= 101, "
-                                "71 for float64)");
     if (!antpos || !freqs || !eq2tops || !jones || !vis_out || (nsrc > 0 && (!crd_eq || !isqrt)))
         return fail(HV_ERR_ARG, "NULL array argument");
     std::lock_guard<std::mutex> lk(c->mu);
@@ -835,7 +837,7 @@ extern "C" int hv_beam_upload_freq(hv_ctx* c, int dtype, const void* bm_cube, in
     if (!c || !out || !bm_cube) return fail(HV_ERR_ARG, "NULL argument");
     *out = nullptr;
     HV_CHECK(check_dtype(dtype));
-    if (nant <= 0 || nfreq <= 0 || beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "bad beam shape");
+    if (nant <= 0 || nfreq <= 0 || beam_px < 1 || beam_px > MAX_BEAM_PX) return fail(HV_ERR_ARG, "bad beam shape");
     std::lock_guard<std::mutex> lk(c->mu);
     HV_CHECK(set_device(c));
     const size_t n = (size_t)nfreq * nant * beam_px * beam_px;
@@ -982,7 +984,7 @@ extern "C" int hv_k_beam(hv_ctx* c, int dtype, const void* bm_cube, int nant, in
                          int64_t nsrc, void* A_out) {
     if (!c || !bm_cube || !crd_top || !A_out) return fail(HV_ERR_ARG, "NULL argument");
     HV_CHECK(check_dtype(dtype));
-    if (nant <= 0 || nsrc <= 0 || beam_px < 1 || beam_px > 128) return fail(HV_ERR_ARG, "bad shape");
+    if (nant <= 0 || nsrc <= 0 || beam_px < 1 || beam_px > MAX_BEAM_PX) return fail(HV_ERR_ARG, "bad shape");
     std::lock_guard<std::mutex> lk(c->mu);
     HV_CHECK(set_device(c));
     const size_t es = dtype == HV_F64 ? 8 : 4;

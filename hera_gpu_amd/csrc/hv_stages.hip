@@ -67,6 +67,9 @@ hipError_t launch_coords(const CoordsArgs& a, hipStream_t s) {
 // per workgroup; each thread then handles 8 consecutive sources: beam
 // amplitude A sqrt(I) (zero below the horizon) and FP64 path difference are
 // formed once and reused for every frequency of the launch.
+// LDS of one workgroup on gfx950; beam planes that do not fit are read from
+// global memory instead (the same taps through L2).
+constexpr size_t LDS_BYTES = (size_t)160 * 1024;
 constexpr int PH_THREADS = 256;
 constexpr int PH_SRC_PER_THREAD = 8;
 constexpr int PH_SRC_PER_WG = 8192;
@@ -105,11 +108,12 @@ __device__ __forceinline__ void phasor_store(char* dst, const R* amp, const doub
     }
 }
 
-template <int MODE, typename BT>
+// GB: the beam plane does not fit in LDS; the taps come from global memory.
+template <int MODE, typename BT, bool GB>
 __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     using R = typename std::conditional<MODE == MODE_FP64, double, float>::type;
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    BT* sb = reinterpret_cast<BT*>(smem);
+    const BT* sb = reinterpret_cast<const BT*>(smem);
     const int64_t kact = eff_kpad(a.kpad, a.dyn_kpad);
     // antenna is the fastest grid index: the workgroups of one source chunk
     // run side by side and share its coordinates in L2
@@ -119,8 +123,13 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
     const int tb = blockIdx.z;
     const int P = a.beam_px;
     const BT* gb = reinterpret_cast<const BT*>(a.beam) + (size_t)ant * P * P;
-    for (int i = threadIdx.x; i < P * P; i += PH_THREADS) sb[i] = gb[i];
-    __syncthreads();
+    if constexpr (GB) {
+        sb = gb;
+    } else {
+        BT* st = reinterpret_cast<BT*>(smem);
+        for (int i = threadIdx.x; i < P * P; i += PH_THREADS) st[i] = gb[i];
+        __syncthreads();
+    }
 
     const double ax = a.antpos[3 * ant], ay = a.antpos[3 * ant + 1], az = a.antpos[3 * ant + 2];
     const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
@@ -137,10 +146,15 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_kernel(PhasorArgs a) {
         // LDS in turn, the same arithmetic per element as below
         for (int u = 0; u < a.nfreq; ++u) {
             if (u > 0) {
-                __syncthreads();                       // every thread is done with plane u-1
                 const BT* gu = reinterpret_cast<const BT*>(a.beam) + u * a.beam_fstride + (size_t)ant * P * P;
-                for (int i = threadIdx.x; i < P * P; i += PH_THREADS) sb[i] = gu[i];
-                __syncthreads();
+                if constexpr (GB) {
+                    sb = gu;
+                } else {
+                    __syncthreads();                   // every thread is done with plane u-1
+                    BT* st = reinterpret_cast<BT*>(smem);
+                    for (int i = threadIdx.x; i < P * P; i += PH_THREADS) st[i] = gu[i];
+                    __syncthreads();
+                }
             }
             const double il = a.inv_lambda[u];
             const double il4 = 4.0 * il;
@@ -204,12 +218,16 @@ static hipError_t launch_phasor_t(const PhasorArgs& a, hipStream_t s) {
     const int64_t nchunk = (a.kpad + PH_SRC_PER_WG - 1) / PH_SRC_PER_WG;
     if (nchunk > 65535 || a.ntime > 65535) return hipErrorInvalidConfiguration;
     dim3 grid((unsigned)a.nant, (unsigned)nchunk, (unsigned)a.ntime);
+    if (lds > LDS_BYTES) {   // the antenna's beam plane does not fit: taps from global memory
+        hipLaunchKernelGGL((phasor_kernel<MODE, BT, true>), grid, dim3(PH_THREADS), 0, s, a);
+        return hipGetLastError();
+    }
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)phasor_kernel<MODE, BT>,
+        hipError_t e = hipFuncSetAttribute((const void*)phasor_kernel<MODE, BT, false>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((phasor_kernel<MODE, BT>), grid, dim3(PH_THREADS), lds, s, a);
+    hipLaunchKernelGGL((phasor_kernel<MODE, BT, false>), grid, dim3(PH_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
@@ -240,8 +258,13 @@ hipError_t launch_phasor(int mode, const PhasorArgs& a, hipStream_t s) {
 // turns otherwise) and the complex product J sqrt(I) exp(i phase).
 constexpr int PP_SRC_PER_WG = 4096;
 
-template <typename BT>
-__device__ inline void beam_bilinear4(const BT* __restrict__ b, int P, double x, double y, float (&out)[4]) {
+// Four planes at once with one set of texel corners and weights (the
+// arithmetic of beam_bilinear).  CPLX false: planes b + c P^2 (real, as
+// staged in LDS); true: the two complex planes of a feed in global memory,
+// (re, im) interleaved, plane c = real (c even) or imaginary part of complex
+// plane c / 2.  The values, and so the results, are the same either way.
+template <typename BT, bool CPLX = false, typename OUT>
+__device__ inline void beam_bilinear4(const BT* __restrict__ b, int P, double x, double y, OUT (&out)[4]) {
     const double bx = (double)(P - 1) * (0.5 * x + 0.5);
     const double by = (double)(P - 1) * (0.5 * y + 0.5);
     const double px = floor(bx), py = floor(by);
@@ -250,34 +273,15 @@ __device__ inline void beam_bilinear4(const BT* __restrict__ b, int P, double x,
     const int ix0 = (int)clampr(px, 0.0, pm), ix1 = (int)clampr(px + 1.0, 0.0, pm);
     const int iy0 = (int)clampr(py, 0.0, pm), iy1 = (int)clampr(py + 1.0, 0.0, pm);
     const size_t pl = (size_t)P * P;
+    constexpr size_t ST = CPLX ? 2 : 1;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const BT* q = b + c * pl;
-        const BT v00 = q[iy0 * P + ix0], v01 = q[iy0 * P + ix1];
-        const BT v10 = q[iy1 * P + ix0], v11 = q[iy1 * P + ix1];
+        const BT* q = CPLX ? b + (size_t)(c >> 1) * 2 * pl + (c & 1) : b + c * pl;
+        const BT v00 = q[(iy0 * P + ix0) * ST], v01 = q[(iy0 * P + ix1) * ST];
+        const BT v10 = q[(iy1 * P + ix0) * ST], v11 = q[(iy1 * P + ix1) * ST];
         const BT top = fma(fx, v01 - v00, v00);
         const BT bot = fma(fx, v11 - v10, v10);
-        out[c] = (float)fma(fy, bot - top, top);
-    }
-}
-template <typename BT>
-__device__ inline void beam_bilinear4(const BT* __restrict__ b, int P, double x, double y, double (&out)[4]) {
-    const double bx = (double)(P - 1) * (0.5 * x + 0.5);
-    const double by = (double)(P - 1) * (0.5 * y + 0.5);
-    const double px = floor(bx), py = floor(by);
-    const BT fx = (BT)(bx - px), fy = (BT)(by - py);
-    const double pm = (double)(P - 1);
-    const int ix0 = (int)clampr(px, 0.0, pm), ix1 = (int)clampr(px + 1.0, 0.0, pm);
-    const int iy0 = (int)clampr(py, 0.0, pm), iy1 = (int)clampr(py + 1.0, 0.0, pm);
-    const size_t pl = (size_t)P * P;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const BT* q = b + c * pl;
-        const BT v00 = q[iy0 * P + ix0], v01 = q[iy0 * P + ix1];
-        const BT v10 = q[iy1 * P + ix0], v11 = q[iy1 * P + ix1];
-        const BT top = fma(fx, v01 - v00, v00);
-        const BT bot = fma(fx, v11 - v10, v10);
-        out[c] = (double)fma(fy, bot - top, top);
+        out[c] = (OUT)fma(fy, bot - top, top);
     }
 }
 
@@ -293,7 +297,9 @@ __device__ inline void phase_sincos(double ax, double ay, double az, double x, d
     }
 }
 
-template <int MODE, typename BT>
+// GB: the planes do not fit in LDS and the bilinear taps come from the
+// Jones array in global memory (same values, same arithmetic).
+template <int MODE, typename BT, bool GB>
 __global__ __launch_bounds__(PH_THREADS) void phasor_pol_kernel(PhasorPolArgs a) {
     using R = typename std::conditional<MODE == MODE_FP64, double, float>::type;
     extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -304,12 +310,14 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_pol_kernel(PhasorPolArgs a)
     const int P = a.beam_px;
     const size_t pl = (size_t)P * P;
     const BT* gj = reinterpret_cast<const BT*>(a.jones) + ((size_t)ant * 4 + (size_t)p * 2) * pl * 2;
-    for (size_t i = threadIdx.x; i < 2 * pl; i += PH_THREADS) {
-        const size_t k = i / pl, e = i - k * pl;          // sky axis k, element e
-        sb[(2 * k) * pl + e] = gj[(k * pl + e) * 2];
-        sb[(2 * k + 1) * pl + e] = gj[(k * pl + e) * 2 + 1];
+    if constexpr (!GB) {
+        for (size_t i = threadIdx.x; i < 2 * pl; i += PH_THREADS) {
+            const size_t k = i / pl, e = i - k * pl;      // sky axis k, element e
+            sb[(2 * k) * pl + e] = gj[(k * pl + e) * 2];
+            sb[(2 * k + 1) * pl + e] = gj[(k * pl + e) * 2 + 1];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
     const double ax = a.antpos[3 * ant], ay = a.antpos[3 * ant + 1], az = a.antpos[3 * ant + 2];
     const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
@@ -330,7 +338,10 @@ __global__ __launch_bounds__(PH_THREADS) void phasor_pol_kernel(PhasorPolArgs a)
             const int64_t s = s0 + j;
             const double x = cx[s], y = cy[s], z = cz[s];
             R bj[4] = {0, 0, 0, 0};
-            if (z > 0.0) beam_bilinear4<BT>(sb, P, x, y, bj);
+            if (z > 0.0) {
+                if constexpr (GB) beam_bilinear4<BT, true>(gj, P, x, y, bj);
+                else beam_bilinear4<BT>(sb, P, x, y, bj);
+            }
             jr[j][0] = bj[0]; ji[j][0] = bj[1]; jr[j][1] = bj[2]; ji[j][1] = bj[3];
             amp[j] = (R)a.isqrt[s] * (R)a.amp_scale;
             sx[j] = x;
@@ -365,14 +376,17 @@ static hipError_t launch_phasor_pol_t(const PhasorPolArgs& a, hipStream_t s) {
     const size_t lds = (size_t)4 * a.beam_px * a.beam_px * sizeof(BT);
     const int64_t nchunk = (a.kpad + PP_SRC_PER_WG - 1) / PP_SRC_PER_WG;
     if (nchunk > 65535 || a.ntime > 65535 || a.kpad % 4) return hipErrorInvalidConfiguration;
-    if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
     dim3 grid((unsigned)(2 * a.nant), (unsigned)nchunk, (unsigned)a.ntime);
+    if (lds > LDS_BYTES) {   // the four planes of a feed do not fit: taps from global memory
+        hipLaunchKernelGGL((phasor_pol_kernel<MODE, BT, true>), grid, dim3(PH_THREADS), 0, s, a);
+        return hipGetLastError();
+    }
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)phasor_pol_kernel<MODE, BT>,
+        hipError_t e = hipFuncSetAttribute((const void*)phasor_pol_kernel<MODE, BT, false>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((phasor_pol_kernel<MODE, BT>), grid, dim3(PH_THREADS), lds, s, a);
+    hipLaunchKernelGGL((phasor_pol_kernel<MODE, BT, false>), grid, dim3(PH_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
@@ -416,7 +430,9 @@ constexpr int AMP_THREADS = 512;
 // keep 8192)
 constexpr int AMP_SRC_PER_WG = 32768;
 
-template <typename BT>
+// GB: not even one antenna's beam plane fits in LDS; the taps come from
+// global memory (same values, same arithmetic).
+template <typename BT, bool GB>
 __global__ __launch_bounds__(AMP_THREADS) void amp_kernel(AmpArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     BT* sb = reinterpret_cast<BT*>(smem);                 // [AMP_ANT][P][P]
@@ -432,8 +448,10 @@ __global__ __launch_bounds__(AMP_THREADS) void amp_kernel(AmpArgs a) {
     const int P = a.beam_px;
     const int pp = P * P;
     const BT* gb = reinterpret_cast<const BT*>(a.beam) + fu * a.beam_fstride + (size_t)ant0 * pp;
-    for (int i = threadIdx.x; i < nq * pp; i += AMP_THREADS) sb[i] = gb[i];
-    __syncthreads();
+    if constexpr (!GB) {
+        for (int i = threadIdx.x; i < nq * pp; i += AMP_THREADS) sb[i] = gb[i];
+        __syncthreads();
+    }
     const double* cx = a.crd_top + (size_t)tb * 3 * a.kpad;
     const double* cy = cx + a.kpad;
     const double* cz = cx + 2 * a.kpad;
@@ -464,7 +482,7 @@ __global__ __launch_bounds__(AMP_THREADS) void amp_kernel(AmpArgs a) {
             is[j] = (float)a.isqrt[s];
         }
         for (int q = 0; q < nq; ++q) {
-            const BT* b = sb + q * pp;
+            const BT* b = (GB ? gb : sb) + q * pp;
             float amp[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -487,7 +505,8 @@ __global__ __launch_bounds__(AMP_THREADS) void amp_kernel(AmpArgs a) {
 hipError_t launch_amp(const AmpArgs& args, hipStream_t s) {
     AmpArgs a = args;
     const size_t plane = (size_t)a.beam_px * a.beam_px * (a.in_f64 ? 8 : 4);
-    a.ants_per_wg = (int)std::max<size_t>(1, std::min<size_t>(AMP_ANT, (size_t)(160 * 1024) / plane));
+    const bool gb = plane > LDS_BYTES;   // not one plane fits: taps from global memory
+    a.ants_per_wg = gb ? AMP_ANT : (int)std::max<size_t>(1, std::min<size_t>(AMP_ANT, LDS_BYTES / plane));
     const int64_t nz = a.beam_fstride ? (int64_t)a.ntime * a.nfreq : a.ntime;
     const int64_t ngroups = (a.nant + a.ants_per_wg - 1) / a.ants_per_wg;
     a.src_per_wg = ngroups * ((a.kpad + AMP_SRC_PER_WG - 1) / AMP_SRC_PER_WG) * nz >= 1024 ? AMP_SRC_PER_WG
@@ -495,32 +514,40 @@ hipError_t launch_amp(const AmpArgs& args, hipStream_t s) {
     const int64_t nchunk = (a.kpad + a.src_per_wg - 1) / a.src_per_wg;
     if (nchunk > 65535 || nz > 65535 || a.kpad % 16) return hipErrorInvalidConfiguration;
     dim3 grid((unsigned)((a.nant + a.ants_per_wg - 1) / a.ants_per_wg), (unsigned)nchunk, (unsigned)nz);
+    if (gb) {
+        if (a.in_f64) hipLaunchKernelGGL((amp_kernel<double, true>), grid, dim3(AMP_THREADS), 0, s, a);
+        else hipLaunchKernelGGL((amp_kernel<float, true>), grid, dim3(AMP_THREADS), 0, s, a);
+        return hipGetLastError();
+    }
     const size_t lds = (size_t)a.ants_per_wg * plane;
-    if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-    const void* k = a.in_f64 ? (const void*)amp_kernel<double> : (const void*)amp_kernel<float>;
+    const void* k = a.in_f64 ? (const void*)amp_kernel<double, false> : (const void*)amp_kernel<float, false>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     if (a.in_f64)
-        hipLaunchKernelGGL(amp_kernel<double>, grid, dim3(AMP_THREADS), lds, s, a);
+        hipLaunchKernelGGL((amp_kernel<double, false>), grid, dim3(AMP_THREADS), lds, s, a);
     else
-        hipLaunchKernelGGL(amp_kernel<float>, grid, dim3(AMP_THREADS), lds, s, a);
+        hipLaunchKernelGGL((amp_kernel<float, false>), grid, dim3(AMP_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
 // ----------------------------------------------------- standalone beam stage
 // Kernel-level beam entry (one workgroup per (source chunk, antenna)), same
 // device function as the fused phasor build.
-template <typename R>
+template <typename R, bool GB>
 __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    R* sb = reinterpret_cast<R*>(smem);
     const int ant = blockIdx.y;
     const int P = a.beam_px;
     const R* gb = reinterpret_cast<const R*>(a.beam) + (size_t)ant * P * P;
-    for (int i = threadIdx.x; i < P * P; i += 256) sb[i] = gb[i];
-    __syncthreads();
+    const R* sb = gb;   // GB: taps from global memory
+    if constexpr (!GB) {
+        R* st = reinterpret_cast<R*>(smem);
+        for (int i = threadIdx.x; i < P * P; i += 256) st[i] = gb[i];
+        __syncthreads();
+        sb = st;
+    }
     const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (s >= a.nsrc) return;
     const R* c = reinterpret_cast<const R*>(a.crd_top);
@@ -532,11 +559,19 @@ __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a) {
 
 hipError_t launch_beam(const BeamArgs& a, hipStream_t s) {
     dim3 grid((unsigned)((a.nsrc + 255) / 256), (unsigned)a.nant);
-    if (a.f64) {
-        hipLaunchKernelGGL(beam_kernel<double>, grid, dim3(256), (size_t)a.beam_px * a.beam_px * 8, s, a);
-    } else {
-        hipLaunchKernelGGL(beam_kernel<float>, grid, dim3(256), (size_t)a.beam_px * a.beam_px * 4, s, a);
+    const size_t lds = (size_t)a.beam_px * a.beam_px * (a.f64 ? 8 : 4);
+    if (lds > LDS_BYTES) {
+        if (a.f64) hipLaunchKernelGGL((beam_kernel<double, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((beam_kernel<float, true>), grid, dim3(256), 0, s, a);
+        return hipGetLastError();
     }
+    const void* k = a.f64 ? (const void*)beam_kernel<double, false> : (const void*)beam_kernel<float, false>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (a.f64) hipLaunchKernelGGL((beam_kernel<double, false>), grid, dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((beam_kernel<float, false>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

@@ -18,6 +18,10 @@
  *     real inputs, and complex64 / complex128 for complex outputs;
  *   - shapes follow the reference: antpos (N,3), eq2tops (T,3,3),
  *     crd_eq (3,K), I_sky / isqrt (K,), bm_cube (N,P,P) indexed [ant,y,x],
+ *     1 <= P <= 2048 (beam planes that fit in the 160 KB of LDS are staged
+ *     there, larger ones are read from device memory by the same bilinear
+ *     arithmetic, as the reference's texture path takes any P,
+ *     vis.py:113-138),
  *     visibilities (F,T,N,N) with V[a,b] = sum_s v_a(s) conj(v_b(s)).
  */
 #ifndef HERAVIS_H
@@ -212,8 +216,7 @@ int hv_vis_device_noalloc(hv_ctx* ctx, int mode, const double* antpos, int nant,
  * beam's rule; zero below the horizon).  vis_out: complex [nfreq][2][2]
  * [ntimes][nant][nant] (feed p, feed q, time, a, b).  J_a^{pk} = A_a
  * delta_pk gives the scalar result on p = q and zeros on p != q.  Runs the
- * phasor build + LDS-ring contraction on 2 nant rows and 2 nsrc columns;
- * beam_px <= 101 (71 for float64).
+ * phasor build + LDS-ring contraction on 2 nant rows and 2 nsrc columns.
  * ------------------------------------------------------------------------- */
 int hv_vis_pol(hv_ctx* ctx, int mode, int dtype, const void* antpos, int nant, const double* freqs, int nfreq,
                const void* eq2tops, int ntimes, const void* crd_eq, const void* isqrt, int64_t nsrc,

@@ -59,8 +59,11 @@ def test_polarized_scalar_freq_shape_and_errors():
     assert v.shape == (2, 2, 3, 37, 37) and v.dtype == np.complex64
     with pytest.raises(AssertionError):
         vis_gpu_polarized(antpos, 150e6, eq, crd, I, J[:, :1])
-    with pytest.raises(AssertionError):          # four planes of 128 x 128 do not fit in LDS
-        vis_gpu_polarized(antpos, 150e6, eq, crd, I, np.zeros((37, 2, 2, 128, 128), np.complex64))
+    # four planes of 128 x 128 do not fit in LDS: taps from device memory
+    z = vis_gpu_polarized(antpos, 150e6, eq, crd, I, np.zeros((37, 2, 2, 128, 128), np.complex64))
+    assert z.shape == (2, 2, 3, 37, 37) and np.all(z == 0)
+    with pytest.raises(AssertionError, match="beam_px"):   # one antenna: 134 MB of zeros
+        vis_gpu_polarized(antpos[:1], 150e6, eq, crd, I, np.zeros((1, 2, 2, 2049, 2049), np.complex64))
 
 
 @pytest.mark.parametrize("gemm", ["fp32", "split_bf16"])
