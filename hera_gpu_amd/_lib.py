@@ -9,6 +9,7 @@ import atexit
 import ctypes as C
 import os
 import threading
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -187,7 +188,13 @@ def contiguous(a, dtype) -> np.ndarray:
 
 
 class Context:
-    """One device, stream and workspace (wraps hv_ctx)."""
+    """One device, stream and workspace (wraps hv_ctx).
+
+    Objects that hold handles into the context (resident catalogs and beams,
+    servers) register with `adopt`; `close` closes the live ones, newest
+    first, before destroying the context, so no handle outlives the hv_ctx
+    its C object points to (at interpreter exit the atexit hook closes the
+    contexts before module globals are collected)."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         lib = load()
@@ -195,9 +202,28 @@ class Context:
         check(lib.hv_ctx_create(device, hbm_budget, C.byref(h)), "hv_ctx_create")
         self.handle = h
         self.device = device
+        self._deps: list[weakref.ref] = []
+        self._deps_lock = threading.Lock()
+
+    def adopt(self, obj) -> None:
+        """Close `obj` (anything with close()) before this context is destroyed."""
+        with self._deps_lock:
+            if len(self._deps) >= 64:
+                self._deps = [r for r in self._deps if r() is not None]
+            self._deps.append(weakref.ref(obj))
 
     def close(self):
         if self.handle:
+            with self._deps_lock:
+                deps, self._deps = self._deps, []
+            # newest first: a server closes before the catalogs and beams its jobs read
+            for r in reversed(deps):
+                o = r()
+                if o is not None:
+                    try:
+                        o.close()
+                    except Exception:
+                        pass
             load().hv_ctx_destroy(self.handle)
             self.handle = None
 

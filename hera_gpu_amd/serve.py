@@ -60,6 +60,7 @@ class Server:
                    "hv_server_create")
         self.handle = h
         self._keep = {}        # id -> (host arrays kept alive until the job finishes)
+        self.ctx.adopt(self)
 
     def submit(self, sky: stages.Sky, beam: stages.Beam, antpos, freqs, eq2tops, nsrc=None, gemm="split_bf16",
                flags=0) -> int:
@@ -112,9 +113,16 @@ class Server:
 
     def close(self):
         if self.handle:
-            self.lib.hv_server_destroy(self.handle)
+            if self.ctx.handle:
+                self.lib.hv_server_destroy(self.handle)
             self.handle = None
             self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

@@ -109,10 +109,13 @@ class Sky:
         _lib.check(_lib.load().hv_sky_upload(self.ctx.handle, _dtype_code(dtype), _lib.ptr(crd), _lib.ptr(isq),
                                              self.nsrc, C.byref(h)), "hv_sky_upload")
         self.handle = h
+        self.ctx.adopt(self)
 
     def close(self):
         if self.handle:
-            _lib.load().hv_sky_free(self.handle)
+            # a closed context has closed this catalog first (Context.adopt)
+            if self.ctx.handle:
+                _lib.load().hv_sky_free(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -138,10 +141,12 @@ class Beam:
         _lib.check(_lib.load().hv_beam_upload_freq(self.ctx.handle, _dtype_code(dtype), _lib.ptr(bm), self.nfreq,
                                                    self.nant, self.beam_px, C.byref(h)), "hv_beam_upload")
         self.handle = h
+        self.ctx.adopt(self)
 
     def close(self):
         if self.handle:
-            _lib.load().hv_beam_free(self.handle)
+            if self.ctx.handle:
+                _lib.load().hv_beam_free(self.handle)
             self.handle = None
 
     def __del__(self):
