@@ -182,7 +182,7 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
         return fail(HV_ERR_ARG, "bad argument");
     if (tile == 192) return fail(HV_ERR_ARG, "192-tiles build their operands in place: time them through hv_simulate");
     if (tile == 0) tile = herk_tile_operands(mode, nant);   // the HBM-operand kernel
-    if (pf == 5) return fail(HV_ERR_ARG, "pf 5 builds its operands in place: time it through hv_simulate");
+    if (pf == 5 || pf == 6) return fail(HV_ERR_ARG, "pf 5 and 6 build their operands in place: time them through hv_simulate");
     if (pf == 0) pf = herk_prefetch(mode, tile) == 5 ? 4 : herk_prefetch(mode, tile);   // same MFMA loop
     std::lock_guard<std::mutex> lk(c->mu);
     HV_HIP(hipSetDevice(c->device));
@@ -238,10 +238,12 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
 }
 
 extern "C" int hv_set_herk_variant(int tile, int pf) {
-    if (!(tile == 0 || tile == 64 || tile == 128 || tile == 192) || pf < 0 || pf > 5)
-        return fail(HV_ERR_ARG, "tile must be 0, 64, 128 or 192 and pf 0..5");
-    if (tile == 192 && pf != 0 && pf != 5)
-        return fail(HV_ERR_ARG, "192-tiles exist only as in-place operand builds (pf 5)");
+    if (!(tile == 0 || tile == 64 || tile == 128 || tile == 192) || pf < 0 || pf > 6)
+        return fail(HV_ERR_ARG, "tile must be 0, 64, 128 or 192 and pf 0..6");
+    if (tile == 192 && pf != 0 && pf != 5 && pf != 6)
+        return fail(HV_ERR_ARG, "192-tiles exist only as in-place operand builds (pf 5 or 6)");
+    if (pf == 6 && tile != 192 && tile != 0)
+        return fail(HV_ERR_ARG, "pf 6 (tile-pair lists) is a 192-tile variant");
     set_herk_variant(tile, pf);
     return HV_OK;
 }

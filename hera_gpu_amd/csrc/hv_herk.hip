@@ -66,8 +66,9 @@ static bool split_mode(int mode) { return mode == MODE_SPLIT_BF16 || mode == MOD
 // pf of the built-in choice or of the override where it exists for the mode
 // (pf 5, operands built in place, is a split-mode variant; FP64 has no pf 2).
 static int pf_choice(int mode) {
-    if (g_pf_override && (g_pf_override != 5 || split_mode(mode)) && !(mode == MODE_FP64 && g_pf_override == 2))
-        return g_pf_override;
+    // pf 6: in place (pf 5) with the 192-tile kernel's tile-pair lists only
+    const int pfo = g_pf_override == 6 ? 5 : g_pf_override;
+    if (pfo && (pfo != 5 || split_mode(mode)) && !(mode == MODE_FP64 && pfo == 2)) return pfo;
     // split modes build their operands in place (whole simulations, 2^18
     // sources: faster than the phasor build + LDS ring at every array size
     // measured, 1.2x at N = 37 and 350, 1.6x at N = 128, and no operand buffer)
@@ -1900,6 +1901,140 @@ __device__ __forceinline__ void t192_pair_mma(const lds_char* As, const lds_char
     }
 }
 
+// Arrays of 321..352 antennas (eleven 32-row blocks, two 192-blocks) as two
+// items per (unit, split) that split the upper triangle of blocks by column:
+// the triangle of blocks r <= c < 8 (36 blocks, antenna rows 0-255 built) and
+// the strip of columns 8-10 (30 blocks, rows 0-351 built).  Together they
+// build 608 operand rows per (unit, split), against 768 for the paired
+// diagonal item and the off-diagonal item (384 rows each).  Both items keep
+// the operand buffer's rows in antenna order (bi = 0, bj = 1), so block b
+// sits at row 32 b of the buffer, and their blocks are stored to the three
+// 192-tiles they belong to.  Each block runs the same MFMA chain as in the
+// other lists, so the partial sums are the same bits.
+
+// Blocks (r, c), r in R0..R1 and c in C0..C1 (all with r <= c), one step,
+// into acc[K0 + (r - R0) * NC + c - C0]: B fragments held, A streamed (Im:
+// the A fragments rotated); bhi/blo keep the B fragments for the caller.
+template <int MODE, bool IM, int R0, int R1, int C0, int C1, int K0>
+__device__ __forceinline__ void tri_rect(const lds_char* S, int fh, int fl, u32x4 (&bhi)[C1 - C0 + 1],
+                                         u32x4 (&blo)[C1 - C0 + 1], f32x16 (&acc)[9]) {
+    constexpr int NC = C1 - C0 + 1;
+    constexpr int RA = IM ? 2 : 0;
+    u32x4 ahi, alo;
+    t192_frag<RA>(S, R0, fh, fl, ahi, alo);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        t192_frag<0>(S, C0 + c, fh, fl, bhi[c], blo[c]);
+        t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[K0 + c]);
+    }
+#pragma unroll
+    for (int r = R0 + 1; r <= R1; ++r) {
+        t192_frag<RA>(S, r, fh, fl, ahi, alo);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[K0 + (r - R0) * NC + c]);
+    }
+}
+
+// Blocks (r, c), R0 <= r <= c <= R1 (a triangle on the diagonal), in row
+// order into acc[K0 ...], one step: each fragment read once, and rotated as
+// the A side of its own row for Im.
+template <int MODE, bool IM, int R0, int R1, int K0>
+__device__ __forceinline__ void tri_tri(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
+    constexpr int n = R1 - R0 + 1;
+    u32x4 hi[n], lo[n];
+#pragma unroll
+    for (int i = 0; i < n; ++i) t192_frag<0>(S, R0 + i, fh, fl, hi[i], lo[i]);
+#pragma unroll
+    for (int r = 0; r < n; ++r) {
+        const u32x4 ahi = IM ? t192_rot_a(hi[r]) : hi[r];
+        const u32x4 alo = IM ? t192_rot_a(lo[r]) : lo[r];
+#pragma unroll
+        for (int c = r; c < n; ++c) t192_mac<MODE>(ahi, alo, hi[c], lo[c], acc[K0 + r * n - r * (r - 1) / 2 + c - r]);
+    }
+}
+
+// One diagonal block (B, B) or off-diagonal (A, B) from fragments already
+// held (the strip's B fragments), A rotated for Im.
+template <int MODE, bool IM>
+__device__ __forceinline__ void tri_held(const u32x4& ahi0, const u32x4& alo0, const u32x4& bhi, const u32x4& blo,
+                                         f32x16& acc) {
+    const u32x4 ahi = IM ? t192_rot_a(ahi0) : ahi0;
+    const u32x4 alo = IM ? t192_rot_a(alo0) : alo0;
+    t192_mac<MODE>(ahi, alo, bhi, blo, acc);
+}
+
+// The two items, per SIMD q (9, 9, 9, 9 and 8, 8, 7, 7 blocks):
+//   triangle q0: rows 0-2 x cols 5-7; q1: rows 0-2 x cols 2-4;
+//            q2: (0,0) (0,1) (1,1) + rows 3-4 x cols 5-7;
+//            q3: (3,3) (3,4) (4,4) + the triangle of rows 5-7;
+//   strip    q: rows 2q, 2q+1 x cols 8-10, then q0: (8,8) (8,9);
+//            q1: (8,10) (9,9); q2: (9,10); q3: (10,10).
+template <int MODE, bool IM, int Q>
+__device__ __forceinline__ void tri_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int fh = fo.hi[s], fl = fo.lo[s];
+        u32x4 bhi[3], blo[3];
+        if constexpr (Q == 0) tri_rect<MODE, IM, 0, 2, 5, 7, 0>(S, fh, fl, bhi, blo, acc);
+        if constexpr (Q == 1) tri_rect<MODE, IM, 0, 2, 2, 4, 0>(S, fh, fl, bhi, blo, acc);
+        if constexpr (Q == 2) {
+            tri_tri<MODE, IM, 0, 1, 0>(S, fh, fl, acc);
+            tri_rect<MODE, IM, 3, 4, 5, 7, 3>(S, fh, fl, bhi, blo, acc);
+        }
+        if constexpr (Q == 3) {
+            tri_tri<MODE, IM, 3, 4, 0>(S, fh, fl, acc);
+            tri_tri<MODE, IM, 5, 7, 3>(S, fh, fl, acc);
+        }
+        if constexpr (Q >= 4) {
+            constexpr int q = Q - 4;
+            tri_rect<MODE, IM, 2 * q, 2 * q + 1, 8, 10, 0>(S, fh, fl, bhi, blo, acc);
+            if constexpr (q == 0) {
+                tri_held<MODE, IM>(bhi[0], blo[0], bhi[0], blo[0], acc[6]);
+                tri_held<MODE, IM>(bhi[0], blo[0], bhi[1], blo[1], acc[7]);
+            }
+            if constexpr (q == 1) {
+                tri_held<MODE, IM>(bhi[0], blo[0], bhi[2], blo[2], acc[6]);
+                tri_held<MODE, IM>(bhi[1], blo[1], bhi[1], blo[1], acc[7]);
+            }
+            if constexpr (q == 2) tri_held<MODE, IM>(bhi[1], blo[1], bhi[2], blo[2], acc[6]);
+            if constexpr (q == 3) tri_held<MODE, IM>(bhi[2], blo[2], bhi[2], blo[2], acc[6]);
+        }
+    }
+}
+
+// Block k of list Q of tri_mma -> absolute 32-row block (br, bc), or false.
+__device__ inline bool tri_block(int Q, int k, int& br, int& bc) {
+    switch (Q) {
+        case 0: br = k / 3; bc = 5 + k % 3; return k < 9;
+        case 1: br = k / 3; bc = 2 + k % 3; return k < 9;
+        case 2:
+            if (k < 3) { br = k < 2 ? 0 : 1; bc = k < 1 ? 0 : 1; return true; }
+            br = 3 + (k - 3) / 3; bc = 5 + (k - 3) % 3; return k < 9;
+        case 3:
+            if (k < 3) { br = k < 2 ? 3 : 4; bc = k < 1 ? 3 : 4; return true; }
+            br = k < 6 ? 5 : k < 8 ? 6 : 7;
+            bc = k < 6 ? 5 + (k - 3) : k < 8 ? 6 + (k - 6) : 7;
+            return k < 9;
+        default: {
+            const int q = Q - 4;
+            if (k < 6) { br = 2 * q + k / 3; bc = 8 + k % 3; return true; }
+            if (q == 0) { br = 8; bc = 8 + (k - 6); return k < 8; }
+            if (q == 1) { br = k == 6 ? 8 : 9; bc = k == 6 ? 10 : 9; return k < 8; }
+            br = q == 2 ? 9 : 10; bc = 10;
+            return k < 7;
+        }
+    }
+}
+
+// Build rows of the triangle/strip items: triangle (256 rows): wave w rows
+// 32 w .. 32 w + 31; strip (352 rows): waves 0-5 rows 48 w .. 48 w + 47,
+// waves 6-7 rows 288 + 32 (w - 6) .. + 31.
+template <bool STRIP, bool IM, int Q>
+struct TriRows {
+    static constexpr int NR = !STRIP ? 2 : (IM && Q >= 2) ? 2 : 3;
+    __device__ static int row0(int wave) { return !STRIP ? 32 * wave : wave < 6 ? 48 * wave : 288 + 32 * (wave - 6); }
+};
+
 // Block k of a wave's accumulator set -> (block row, block column) in the
 // tile (list DQ of t192_split; DQ < 0: quadrant q), or false past the set;
 // `second`: the block lies in the second tile of a paired item (DQ >= 20).
@@ -1966,7 +2101,8 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc, int& s
 // B column blocks inside (24 blocks); DQ = 20..23 / 24..27: the first and
 // the last diagonal tile as one item, at most five / four block rows of the
 // last inside (t192_pair_mma).  Paired items build both blocks' rows
-// as an off-diagonal tile does.
+// as an off-diagonal tile does.  DQ = 28..31 / 32..35: the triangle / strip
+// items of an array of 321..352 antennas (tri_mma lists 0..3 / 4..7).
 template <int MODE, bool IM, int DQ>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
                                            const double* antpos, int nant, int bi, int bj, const float* amp_t,
@@ -1976,12 +2112,14 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
     using L = T192;
     constexpr bool DIAG = (DQ >= 0 && DQ < 8) || (DQ >= 12 && DQ < 16);
     constexpr bool RECT5 = (DQ >= 8 && DQ < 12) || (DQ >= 16 && DQ < 20);
-    constexpr bool PAIR = DQ >= 20;
+    constexpr bool PAIR = DQ >= 20 && DQ < 28;
+    constexpr bool TRI = DQ >= 28, STRIP = DQ >= 32;
     constexpr int QP = DQ & 3, BCP = DQ < 24 ? 4 : 3;   // t192_pair_mma list
     constexpr int QD = DQ < 8 ? DQ : DQ - 4;     // t192_diag_mma list
     constexpr int QR = DQ < 12 ? DQ - 8 : DQ - 12;   // t192_rect5_mma list
-    constexpr int NROW = DIAG ? L::TILE : L::ROWS;
-    using RW = T192Rows<DIAG, IM>;
+    constexpr int QT = DQ - 28;                  // tri_mma list
+    constexpr int NROW = DIAG ? L::TILE : TRI ? (STRIP ? 352 : 256) : L::ROWS;
+    using RW = typename std::conditional<TRI, TriRows<STRIP, IM, (DQ & 3)>, T192Rows<DIAG, IM>>::type;
     constexpr int NR = RW::NR;
     const int row0 = RW::row0(wave);
     // antenna positions of the rows this lane builds, in wavelengths
@@ -2010,6 +2148,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? QD : 0)>(buf, fo, acc);          \
         else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, fo, acc); \
         else if constexpr (PAIR) t192_pair_mma<MODE, IM, QP, BCP>(buf, buf + L::OP, fo, acc); \
+        else if constexpr (TRI) tri_mma<MODE, IM, (TRI ? QT : 0)>(buf, fo, acc);               \
         else t192_quad_mma<MODE, IM>(buf + qa, buf + qb, fo, acc);                             \
     } while (0)
 
@@ -2076,6 +2215,11 @@ __host__ __device__ inline int t192_unit_group(int nunits, int64_t g) {
 // ntiles - 1 of the decode order); work tile w = 1 .. ntiles - 2 is tile w.
 // Two blocks: 2 items per (unit, split) instead of 3.
 __host__ __device__ inline bool t192_pairs(int nb, int nant) { return nb >= 2 && nant - (nb - 1) * 192 <= 160; }
+// Arrays of 321..352 antennas take the triangle and strip items (tri_mma)
+// in place of the paired diagonal and off-diagonal tiles unless `lists` is 1.
+__host__ __device__ inline bool t192_tri(int nb, int nant, int lists) {
+    return lists != 1 && nb == 2 && nant > 320 && nant <= 352;
+}
 __host__ __device__ inline int t192_work_tiles(int nb, int nant, int ntiles) {
     return t192_pairs(nb, nant) ? ntiles - 1 : ntiles;
 }
@@ -2109,6 +2253,7 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     constexpr int STAGES_SPLIT = KSPLIT / 16;
     const int ug = t192_unit_group(a.nunits, gridDim.x);
     const bool pairs = t192_pairs(a.nb, a.nant);
+    const bool tri = t192_tri(a.nb, a.nant, a.lists);   // work tile 0: triangle, 1: strip
     const int ntw = t192_work_tiles(a.nb, a.nant, a.ntiles);
     const int64_t nitems = t192_items(a.nunits, a.nsplit, ntw, ug);
 
@@ -2119,7 +2264,10 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         // last block; its blocks are stored to the diagonal tiles 0 and
         // ntiles - 1
         const bool pair = pairs && tile == 0;
-        if (pair) {
+        if (tri) {
+            bi = 0;   // rows in antenna order
+            bj = 1;
+        } else if (pair) {
             bi = 0;
             bj = a.nb - 1;
         } else {
@@ -2165,7 +2313,8 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         // block (off-diagonal tiles) hold antennas
         const int rem = a.nant - bj * TILE;
         const bool five = rem <= 160, four = rem <= 128;
-        const int dq = pair ? (four ? 24 : 20) + q
+        const int dq = tri ? (tile == 0 ? 28 : 32) + q
+                     : pair ? (four ? 24 : 20) + q
                             : diag ? (four ? 12 : five ? 4 : 0) + q : (four ? 16 + q : five ? 8 + q : -1);
         if (dq < 0) HV_T192BOTH(-1);
         else if (dq == 0) HV_T192BOTH(0);
@@ -2195,7 +2344,15 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         else if (dq == 24) HV_T192BOTH(24);
         else if (dq == 25) HV_T192BOTH(25);
         else if (dq == 26) HV_T192BOTH(26);
-        else HV_T192BOTH(27);
+        else if (dq == 27) HV_T192BOTH(27);
+        else if (dq == 28) HV_T192BOTH(28);
+        else if (dq == 29) HV_T192BOTH(29);
+        else if (dq == 30) HV_T192BOTH(30);
+        else if (dq == 31) HV_T192BOTH(31);
+        else if (dq == 32) HV_T192BOTH(32);
+        else if (dq == 33) HV_T192BOTH(33);
+        else if (dq == 34) HV_T192BOTH(34);
+        else HV_T192BOTH(35);
 #undef HV_T192BOTH
 #undef HV_T192SPLIT
 
@@ -2205,6 +2362,18 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             int br, bc, second;
+            if (tri) {
+                // absolute 32-row blocks -> their 192-tile (0, 0), (0, 1) or (1, 1)
+                if (!tri_block(dq - 28, k, br, bc) || br * 32 >= a.nant || bc * 32 >= a.nant) continue;
+                float* out = out0 + (size_t)(br < 6 ? (bc < 6 ? 0 : 1) : 2) * 2 * TSZ;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int row = (br % 6) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                    const int col = (bc % 6) * 32 + (lane & 31);
+                    out[(size_t)row * TILE + col] = acc[k][e];
+                }
+                continue;
+            }
             if (!t192_block(dq, q, k, br, bc, second)) continue;
             // a paired item's blocks: tile 0 = (0, 0) or tile ntiles - 1 = (nb - 1, nb - 1)
             const int ot = pair ? second * (a.ntiles - 1) : tile;
@@ -2276,9 +2445,11 @@ static int64_t t192_resident() {
 }
 
 template <int MODE>
-static hipError_t launch_t192_t(const FusedArgs& a, hipStream_t s) {
-    if ((int64_t)a.nunits * a.nsplit * a.ntiles <= 0) return hipSuccess;
-    if (a.kpad % 16) return hipErrorInvalidValue;
+static hipError_t launch_t192_t(const FusedArgs& a0, hipStream_t s) {
+    if ((int64_t)a0.nunits * a0.nsplit * a0.ntiles <= 0) return hipSuccess;
+    if (a0.kpad % 16) return hipErrorInvalidValue;
+    FusedArgs a = a0;
+    if (g_pf_override == 6) a.lists = 1;
     const int64_t res = t192_resident<MODE>();
     const int64_t g0 = res > 0 ? res : 256;
     const int64_t items = t192_items(a.nunits, a.nsplit, t192_work_tiles(a.nb, a.nant, a.ntiles),

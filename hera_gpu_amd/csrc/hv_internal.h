@@ -207,6 +207,7 @@ struct FusedArgs {
     int nant, nb, ntiles, nsplit, nunits, nfreq;
     const int64_t* dyn_kpad;
     int amp_per_unit;         // amp holds one block per unit (frequency-dependent beams)
+    int lists;                // 192-tile block lists: 0 built-in, 1 tile pairs only (no triangle/strip)
 };
 
 hipError_t launch_amp(const AmpArgs& a, hipStream_t s);

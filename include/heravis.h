@@ -296,8 +296,10 @@ int hv_bench_herk(hv_ctx* ctx, int mode, int nant, int64_t nsrc, int units, int 
  * ring, one workgroup per work item), 4 (LDS ring on a persistent grid) or 5
  * (split modes: operands built inside the contraction from A sqrt(I) per
  * time, no operand buffer; hv_k_herk and hv_bench_herk, which take operands,
- * use 4).  192-tiles exist only with pf 5 (or 0).  0 restores the built-in
- * choice.  Not safe while other calls run. */
+ * use 4) or 6 (as 5, with the 192-tile kernel's tile-pair work items also for
+ * arrays of 321..352 antennas, which otherwise take triangle and strip items;
+ * the same bits).  192-tiles exist only with pf 5, 6 (or 0).  0 restores the
+ * built-in choice.  Not safe while other calls run. */
 int hv_set_herk_variant(int tile, int pf);
 /* The contraction variant (tile edge, pf as above) that calls of this mode and
  * antenna count use now. */

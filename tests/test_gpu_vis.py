@@ -213,7 +213,7 @@ def test_batch_schedules_agree_bitwise(gemm, monkeypatch):
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("tile,nant", [(64, 300), (128, 300), (192, 300), (192, 160), (192, 129), (192, 400), (192, 97), (192, 128), (192, 230), (128, 128), (128, 100), (128, 70), (192, 350), (192, 520)])
+@pytest.mark.parametrize("tile,nant", [(64, 300), (128, 300), (192, 300), (192, 160), (192, 129), (192, 400), (192, 97), (192, 128), (192, 230), (128, 128), (128, 100), (128, 70), (192, 350), (192, 321), (192, 352), (192, 520)])
 def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile, nant):
     # pf 5 builds the operand stages inside the contraction from A sqrt(I);
     # pf 4 contracts the stages phasor_kernel wrote: same operand bits, same
@@ -244,6 +244,37 @@ def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile, na
     np.testing.assert_array_equal(outs[0], outs[1])
     ref = vis_oracle(ap, w.freqs[-1:], w.eq2tops[:1], w.crd_eq, w.I_sky, bm)[0, 0]
     assert rel_err(outs[1][-1, 0], ref) < VIS_TOL[gemm]
+
+
+@pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
+@pytest.mark.parametrize("nant", [321, 350, 352])
+def test_triangle_strip_items_equal_tile_pairs_bitwise(gemm, nant):
+    # arrays of 321..352 antennas (eleven 32-row blocks) cut the upper
+    # triangle of blocks into a triangle item (r <= c < 8) and a strip item
+    # (columns 8-10) per (unit, split); pf 6 runs the same array as the
+    # paired diagonal tiles and the off-diagonal tile.  Every block runs the
+    # same MFMA chain on the same operand bits, so the visibilities are equal
+    # bit for bit; N = 321 leaves one antenna in the last block, N = 352 none
+    # past it.  K spans several 8192-source splits; 2 times x 3 frequencies.
+    from hera_gpu_amd import _lib
+    lib = _lib.load()
+    w = configs.make_hera350(nsrc=2 * 8192 + 700, ntimes=2, nfreq=3)
+    ap, bm = w.antpos, w.bm_cube
+    if nant > len(ap):
+        extra = nant - len(ap)
+        ap = np.concatenate([ap, ap[:extra] + np.array([0.0, 400.0, 0.0], dtype=ap.dtype)])
+        bm = np.concatenate([bm, bm[:extra]])
+    ap, bm = ap[:nant], bm[:nant]
+    outs = []
+    try:
+        for pf in (5, 6):
+            _lib.check(lib.hv_set_herk_variant(192, pf))
+            outs.append(vis_gpu(ap, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, bm, gemm=gemm))
+    finally:
+        _lib.check(lib.hv_set_herk_variant(0, 0))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    ref = vis_oracle(ap, w.freqs[:1], w.eq2tops[-1:], w.crd_eq, w.I_sky, bm)[0, 0]
+    assert rel_err(outs[0][0, -1], ref) < VIS_TOL[gemm]
 
 
 @pytest.mark.parametrize("gemm", ["fp32", "split_bf16", "fp64"])
