@@ -30,8 +30,9 @@ HBM_PEAK_GBS = 8000.0
 # by what the same counters read for a 1 GiB device copy: FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0.
 MEASURED_TRAFFIC = {
-    # 192-tile in-place build (herk_t192_kernel), two times (128 units) per launch
-    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 6208934, "write_kib": 8660992, "fetch_scale": 2.0,
+    # 192-tile in-place build (herk_t192_kernel, triangle and strip items), two
+    # times (128 units) per launch
+    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5484638, "write_kib": 8666112, "fetch_scale": 2.0,
                                           "write_scale": 1.0},
     # 128-tile in-place build (--herk-variant 128,5), one time per launch
     ("hera350", "split_bf16", 64, 128): {"fetch_kib": 138366362, "write_kib": 4325376, "fetch_scale": 2.0,
@@ -48,7 +49,8 @@ PF_NAMES = {1: "register-staged, 1 stage ahead", 2: "register-staged, 2 stages a
 def variant_name(tile, pf):
     if tile == 192:
         return ("operands built in place from A sqrt(I), 8 waves (Re and Im wave pair per SIMD), "
-                "first and last diagonal tiles as one item, persistent XCD-grouped grid")
+                "at 321..352 antennas a triangle item (32-row blocks r <= c < 8) and a strip item "
+                "(block columns 8-10) per (unit, split), persistent XCD-grouped grid")
     return PF_NAMES[pf]
 
 
@@ -64,7 +66,8 @@ def t192_mfma_blocks(N):
     """32x32 MFMA blocks the 192-tile kernel runs per unit and 16-source stage
     for each component: per tile (bi <= bj) 21 (diagonal) or 36 (off-diagonal)
     blocks, 15 and 30 when at most five 32-row blocks of the last block row or
-    column hold antennas."""
+    column hold antennas (at 321..352 antennas the triangle and strip items
+    run the same 66 blocks: 36 + 30)."""
     nb = -(-N // 192)
     total = 0
     for bi in range(nb):
