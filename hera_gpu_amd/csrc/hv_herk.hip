@@ -75,16 +75,19 @@ static int pf_choice(int mode) {
     return split_mode(mode) ? 5 : 1;
 }
 
-// Tile edge: 64 (FP32, FP64, small arrays), 128 (split modes at 64 < N <= 128
-// and 192 < N <= 256), 192 (split modes at 128 < N <= 192 and above 256
-// antennas; in-place builds only).  Whole simulations on MI355X, split-BF16,
-// 2^20 sources: N = 160 258 TFLOP/s with 192-tiles against 120 with 128-tiles
-// (two 128-blocks, the second a quarter full); N = 96, 128, 200, 256:
-// 128-tiles 148 / 240 / 186 / 271 against 108 / 190 / 118 / 192; N = 300:
-// 261 against 184; N = 37 and 64: 64-tiles 45 / 130 against 22 / 67.
-// Overrides apply where the variant exists for the mode.
+// Tile edge: 64 (FP32, FP64, small arrays), 128 (split modes at 64 < N <=
+// 128), 192 (split modes above 128 antennas; in-place builds only: the
+// items of the triangle decomposition up to 352 antennas, 192-tiles above).
+// Whole simulations on MI355X, split-BF16, 2^20 sources, before the triangle
+// decomposition: N = 160 258 TFLOP/s with 192-tiles against 120 with
+// 128-tiles (two 128-blocks, the second a quarter full); N = 96, 128, 200,
+// 256: 128-tiles 148 / 240 / 186 / 271 against 108 / 190 / 118 / 192 with
+// 192-tiles; N = 300: 261 against 184; N = 37 and 64: 64-tiles 45 / 130
+// against 22 / 67.  Overrides apply where the variant exists for the mode.
 int herk_tile(int mode, int nant) {
-    const bool t192 = nant > 256 || (nant > 128 && nant <= 192);
+    // 129..352 antennas: the items of the triangle decomposition, which store
+    // into 192-tiles; pf 6 keeps 128-tiles at 193..256
+    const bool t192 = nant > 256 || (nant > 128 && (nant <= 192 || g_pf_override != 6));
     int t = split_mode(mode) ? (t192 ? 192 : nant > 64 ? 128 : 64) : 64;
     if (g_tile_override && (g_tile_override == 64 || split_mode(mode))) t = g_tile_override;
     if (t == 192 && pf_choice(mode) != 5) t = 128;
@@ -2003,7 +2006,7 @@ __device__ __forceinline__ void tri_mma(const lds_char* S, const T192Frag& fo, f
 }
 
 // Block k of list Q of tri_mma -> absolute 32-row block (br, bc), or false.
-__device__ inline bool tri_block(int Q, int k, int& br, int& bc) {
+__host__ __device__ constexpr bool tri_block(int Q, int k, int& br, int& bc) {
     switch (Q) {
         case 0: br = k / 3; bc = 5 + k % 3; return k < 9;
         case 1: br = k / 3; bc = 2 + k % 3; return k < 9;
@@ -2033,6 +2036,104 @@ template <bool STRIP, bool IM, int Q>
 struct TriRows {
     static constexpr int NR = !STRIP ? 2 : (IM && Q >= 2) ? 2 : 3;
     __device__ static int row0(int wave) { return !STRIP ? 32 * wave : wave < 6 ? 48 * wave : 288 + 32 * (wave - 6); }
+};
+
+// Arrays of 129..320 antennas (NBK = 5..10 blocks of 32 rows) take the same
+// decomposition with lists of row runs (t192_diag_row: one block row against
+// a range of block columns): up to eight blocks, one triangle item holding
+// every block (each row built once per (unit, split)); at nine and ten
+// blocks, the triangle of the first six or seven block columns and the strip
+// of the last three.  A SIMD's list is at most three runs of at most nine
+// blocks in all; rows are built in antenna order and spread over the eight
+// waves in 16-row groups.
+struct TRun {
+    int r, c0, c1;   // block row r, block columns c0..c1; r < 0: no run
+};
+__host__ __device__ constexpr int tri_n1(int nbk) { return nbk <= 8 ? nbk : nbk - 3; }
+__host__ __device__ constexpr int tri_items(int nbk) { return nbk <= 8 ? 1 : 2; }
+// Run i of SIMD q's list in item `part` (0: triangle, 1: strip) for NBK = 5..10.
+__host__ __device__ constexpr TRun tri_run(int nbk, int part, int q, int i) {
+    constexpr TRun X{-1, 0, -1};
+    if (part == 0) {
+        const int n = tri_n1(nbk);
+        if (n == 8) return i == 0 ? TRun{q, q, 7} : i == 1 ? TRun{7 - q, 7 - q, 7} : X;   // 9, 9, 9, 9
+        if (n == 7) {                                                                        // 7, 7, 7, 7
+            if (q == 0) return i == 0 ? TRun{0, 0, 6} : X;
+            return i == 0 ? TRun{q, q, 6} : i == 1 ? TRun{7 - q, 7 - q, 6} : X;
+        }
+        if (n == 6) {                                                                        // 6, 5, 5, 5
+            if (q < 2) return i == 0 ? TRun{q, q, 5} : X;
+            return i == 0 ? TRun{q, q, 5} : i == 1 ? TRun{7 - q, 7 - q, 5} : X;
+        }
+        // n == 5: 4, 4, 4, 3
+        if (q == 0) return i == 0 ? TRun{0, 0, 3} : X;
+        if (q == 1) return i == 0 ? TRun{0, 4, 4} : i == 1 ? TRun{1, 1, 3} : X;
+        if (q == 2) return i == 0 ? TRun{1, 4, 4} : i == 1 ? TRun{2, 2, 4} : X;
+        return i == 0 ? TRun{3, 3, 4} : i == 1 ? TRun{4, 4, 4} : X;
+    }
+    if (nbk == 10) {   // columns 7-9: 7, 7, 7, 6
+        if (q < 3) return i < 2 ? TRun{2 * q + i, 7, 9} : i == 2 ? TRun{7, 7 + q, 7 + q} : X;
+        return i == 0 ? TRun{6, 7, 9} : i == 1 ? TRun{8, 8, 9} : i == 2 ? TRun{9, 9, 9} : X;
+    }
+    // nbk == 9: columns 6-8: 6, 6, 6, 6
+    if (q < 3) return i < 2 ? TRun{2 * q + i, 6, 8} : X;
+    return i == 0 ? TRun{6, 6, 8} : i == 1 ? TRun{7, 7, 8} : i == 2 ? TRun{8, 8, 8} : X;
+}
+__host__ __device__ constexpr int tri_list_blocks(int nbk, int part, int q) {
+    int n = 0;
+    for (int i = 0; i < 3; ++i) {
+        const TRun r = tri_run(nbk, part, q, i);
+        if (r.r >= 0) n += r.c1 - r.c0 + 1;
+    }
+    return n;
+}
+
+// Block k of a run list -> (block row, block column), or false past it.
+__host__ __device__ constexpr bool tri_run_block(int nbk, int part, int q, int k, int& br, int& bc) {
+    for (int i = 0; i < 3; ++i) {
+        const TRun r = tri_run(nbk, part, q, i);
+        if (r.r < 0) return false;
+        const int n = r.c1 - r.c0 + 1;
+        if (k < n) {
+            br = r.r;
+            bc = r.c0 + k;
+            return true;
+        }
+        k -= n;
+    }
+    return false;
+}
+
+template <int MODE, bool IM, int NBK, int PART, int Q, int I, int K0>
+__device__ __forceinline__ void tri_runs_step(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
+    if constexpr (I < 3) {
+        constexpr TRun run = tri_run(NBK, PART, Q, I);
+        if constexpr (run.r >= 0) {
+            t192_diag_row<MODE, IM, run.r, run.c0, run.c1, K0>(S, fh, fl, acc);
+            tri_runs_step<MODE, IM, NBK, PART, Q, I + 1, K0 + run.c1 - run.c0 + 1>(S, fh, fl, acc);
+        }
+    }
+}
+template <int MODE, bool IM, int NBK, int PART, int Q>
+__device__ __forceinline__ void tri_runs_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9]) {
+    static_assert(tri_list_blocks(NBK, PART, Q) <= 9, "a SIMD list holds at most nine blocks");
+#pragma unroll
+    for (int s = 0; s < 2; ++s) tri_runs_step<MODE, IM, NBK, PART, Q, 0, 0>(S, fo.hi[s], fo.lo[s], acc);
+}
+
+// Build rows of the run-list items: the item's 16-row groups (rows 0 ..
+// 32 n - 1 of the triangle's n block columns, or all NBK blocks' rows for
+// the strip) split evenly over the waves, wave 4 IM + Q taking groups
+// G0 .. G1 - 1 (one to three).
+template <int NBK, int PART, bool IM, int Q>
+struct TriRowsG {
+    static constexpr int G = 2 * (PART == 0 ? tri_n1(NBK) : NBK);
+    static constexpr int W = (IM ? 4 : 0) + Q;
+    static constexpr int G0 = W * G / 8, G1 = (W + 1) * G / 8;
+    static constexpr int NR = G1 - G0;
+    static constexpr int ROWS = 16 * G;
+    static_assert(NR >= 1 && NR <= 3, "one to three 16-row groups per wave");
+    __device__ static int row0(int) { return 16 * G0; }
 };
 
 // Block k of a wave's accumulator set -> (block row, block column) in the
@@ -2102,8 +2203,9 @@ __device__ inline bool t192_block(int dq, int q, int k, int& br, int& bc, int& s
 // the last diagonal tile as one item, at most five / four block rows of the
 // last inside (t192_pair_mma).  Paired items build both blocks' rows
 // as an off-diagonal tile does.  DQ = 28..31 / 32..35: the triangle / strip
-// items of an array of 321..352 antennas (tri_mma lists 0..3 / 4..7).
-template <int MODE, bool IM, int DQ>
+// items of the triangle decomposition (NBK = 11: tri_mma lists 0..3 /
+// 4..7; NBK = 5..10: the run lists of item 0 / 1, SIMD DQ & 3).
+template <int MODE, bool IM, int DQ, int NBK>
 __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
                                            const double* antpos, int nant, int bi, int bj, const float* amp_t,
                                            const double* crd_t, size_t amp_stage, int s_begin, int n_st,
@@ -2118,8 +2220,12 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
     constexpr int QD = DQ < 8 ? DQ : DQ - 4;     // t192_diag_mma list
     constexpr int QR = DQ < 12 ? DQ - 8 : DQ - 12;   // t192_rect5_mma list
     constexpr int QT = DQ - 28;                  // tri_mma list
-    constexpr int NROW = DIAG ? L::TILE : TRI ? (STRIP ? 352 : 256) : L::ROWS;
-    using RW = typename std::conditional<TRI, TriRows<STRIP, IM, (DQ & 3)>, T192Rows<DIAG, IM>>::type;
+    constexpr bool RUNS = TRI && NBK < 11;       // run lists (tri_runs_mma)
+    using RW = typename std::conditional<
+        RUNS, TriRowsG<(RUNS ? NBK : 5), (STRIP ? 1 : 0), IM, (DQ & 3)>,
+        typename std::conditional<TRI, TriRows<STRIP, IM, (DQ & 3)>, T192Rows<DIAG, IM>>::type>::type;
+    constexpr int NROW = DIAG ? L::TILE : RUNS ? TriRowsG<(RUNS ? NBK : 5), (STRIP ? 1 : 0), IM, (DQ & 3)>::ROWS
+                                      : TRI ? (STRIP ? 352 : 256) : L::ROWS;
     constexpr int NR = RW::NR;
     const int row0 = RW::row0(wave);
     // antenna positions of the rows this lane builds, in wavelengths
@@ -2148,6 +2254,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? QD : 0)>(buf, fo, acc);          \
         else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, fo, acc); \
         else if constexpr (PAIR) t192_pair_mma<MODE, IM, QP, BCP>(buf, buf + L::OP, fo, acc); \
+        else if constexpr (RUNS) tri_runs_mma<MODE, IM, (RUNS ? NBK : 5), (STRIP ? 1 : 0), (DQ & 3)>(buf, fo, acc); \
         else if constexpr (TRI) tri_mma<MODE, IM, (TRI ? QT : 0)>(buf, fo, acc);               \
         else t192_quad_mma<MODE, IM>(buf + qa, buf + qb, fo, acc);                             \
     } while (0)
@@ -2215,10 +2322,12 @@ __host__ __device__ inline int t192_unit_group(int nunits, int64_t g) {
 // ntiles - 1 of the decode order); work tile w = 1 .. ntiles - 2 is tile w.
 // Two blocks: 2 items per (unit, split) instead of 3.
 __host__ __device__ inline bool t192_pairs(int nb, int nant) { return nb >= 2 && nant - (nb - 1) * 192 <= 160; }
-// Arrays of 321..352 antennas take the triangle and strip items (tri_mma)
-// in place of the paired diagonal and off-diagonal tiles unless `lists` is 1.
-__host__ __device__ inline bool t192_tri(int nb, int nant, int lists) {
-    return lists != 1 && nb == 2 && nant > 320 && nant <= 352;
+// Arrays of 129..352 antennas (5..11 blocks of 32 rows) take the items of
+// the triangle decomposition (tri_mma, tri_runs_mma) in place of 192-tiles
+// unless `lists` is 1; the block count, or 0 for the tile lists.
+__host__ __device__ inline int t192_tri_nbk(int nant, int lists) {
+    const int nbk = (nant + 31) / 32;
+    return (lists != 1 && nbk >= 5 && nbk <= 11) ? nbk : 0;
 }
 __host__ __device__ inline int t192_work_tiles(int nb, int nant, int ntiles) {
     return t192_pairs(nb, nant) ? ntiles - 1 : ntiles;
@@ -2230,7 +2339,10 @@ __host__ __device__ inline int64_t t192_items(int nunits, int nsplit, int ntiles
     return (ncombo + 7) / 8 * 8 * ug;
 }
 
-template <int MODE>
+// NBKT: 0 for the 192-tile lists, else the array's block count (5..11) for
+// the items of the triangle decomposition; each instantiation holds only
+// the lists it runs.
+template <int MODE, int NBKT>
 __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     using L = T192;
     constexpr int TILE = L::TILE;
@@ -2253,8 +2365,8 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     constexpr int STAGES_SPLIT = KSPLIT / 16;
     const int ug = t192_unit_group(a.nunits, gridDim.x);
     const bool pairs = t192_pairs(a.nb, a.nant);
-    const bool tri = t192_tri(a.nb, a.nant, a.lists);   // work tile 0: triangle, 1: strip
-    const int ntw = t192_work_tiles(a.nb, a.nant, a.ntiles);
+    constexpr bool tri = NBKT > 0;   // work tile 0: triangle, 1: strip
+    const int ntw = tri ? tri_items(NBKT) : t192_work_tiles(a.nb, a.nant, a.ntiles);
     const int64_t nitems = t192_items(a.nunits, a.nsplit, ntw, ug);
 
     for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
@@ -2297,7 +2409,7 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
 
         f32x16 acc[9];
 #define HV_T192SPLIT(IM, DQ)                                                                                 \
-    t192_split<MODE, IM, DQ>(in0, in1, op0, op1, a.antpos, a.nant, bi, bj, amp_t, crd_t, amp_stage, s_begin, n_st, \
+    t192_split<MODE, IM, DQ, NBKT>(in0, in1, op0, op1, a.antpos, a.nant, bi, bj, amp_t, crd_t, amp_stage, s_begin, n_st, \
                              ampoff, crdoff, lane, wave, il, acc)
 #define HV_T192BOTH(DQ)              \
     do {                             \
@@ -2316,7 +2428,18 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         const int dq = tri ? (tile == 0 ? 28 : 32) + q
                      : pair ? (four ? 24 : 20) + q
                             : diag ? (four ? 12 : five ? 4 : 0) + q : (four ? 16 + q : five ? 8 + q : -1);
-        if (dq < 0) HV_T192BOTH(-1);
+        if constexpr (tri) {
+            if (dq == 28) HV_T192BOTH(28);
+            else if (dq == 29) HV_T192BOTH(29);
+            else if (dq == 30) HV_T192BOTH(30);
+            else if (dq == 31) HV_T192BOTH(31);
+            else if constexpr (tri_items(NBKT) == 2) {
+                if (dq == 32) HV_T192BOTH(32);
+                else if (dq == 33) HV_T192BOTH(33);
+                else if (dq == 34) HV_T192BOTH(34);
+                else HV_T192BOTH(35);
+            }
+        } else if (dq < 0) HV_T192BOTH(-1);
         else if (dq == 0) HV_T192BOTH(0);
         else if (dq == 1) HV_T192BOTH(1);
         else if (dq == 2) HV_T192BOTH(2);
@@ -2344,27 +2467,26 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
         else if (dq == 24) HV_T192BOTH(24);
         else if (dq == 25) HV_T192BOTH(25);
         else if (dq == 26) HV_T192BOTH(26);
-        else if (dq == 27) HV_T192BOTH(27);
-        else if (dq == 28) HV_T192BOTH(28);
-        else if (dq == 29) HV_T192BOTH(29);
-        else if (dq == 30) HV_T192BOTH(30);
-        else if (dq == 31) HV_T192BOTH(31);
-        else if (dq == 32) HV_T192BOTH(32);
-        else if (dq == 33) HV_T192BOTH(33);
-        else if (dq == 34) HV_T192BOTH(34);
-        else HV_T192BOTH(35);
+        else HV_T192BOTH(27);
 #undef HV_T192BOTH
 #undef HV_T192SPLIT
 
         constexpr size_t TSZ = (size_t)TILE * TILE;
         float* const out0 = reinterpret_cast<float*>(a.partials) +
                             (((size_t)unit * a.nsplit + split) * a.ntiles) * 2 * TSZ + (im ? TSZ : 0);
+        // the list's SIMD slot, opaque to the compiler so that the block
+        // coordinates below are derived per item instead of being hoisted
+        // out of the item loop (144 store offsets held across it spill)
+        int qs = q;
+        asm volatile("" : "+v"(qs));
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             int br, bc, second;
-            if (tri) {
+            if constexpr (tri) {
                 // absolute 32-row blocks -> their 192-tile (0, 0), (0, 1) or (1, 1)
-                if (!tri_block(dq - 28, k, br, bc) || br * 32 >= a.nant || bc * 32 >= a.nant) continue;
+                const bool in = NBKT == 11 ? tri_block(dq - 28, k, br, bc)
+                                           : tri_run_block(NBKT, (dq - 28) >> 2, qs, k, br, bc);
+                if (!in || br * 32 >= a.nant || bc * 32 >= a.nant) continue;
                 float* out = out0 + (size_t)(br < 6 ? (bc < 6 ? 0 : 1) : 2) * 2 * TSZ;
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
@@ -2431,17 +2553,28 @@ static hipError_t launch_fused8_t(const FusedArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, int NBKT>
 static int64_t t192_resident() {
     static const int64_t cached = [] {
         int dev = 0, ncu = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_t192_kernel<MODE>, 512, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_t192_kernel<MODE, NBKT>, 512, 0) != hipSuccess)
             return (int64_t)0;
         return (int64_t)ncu * per_cu / 8 * 8;
     }();
     return cached;
+}
+
+template <int MODE, int NBKT>
+static hipError_t launch_t192_nbk(const FusedArgs& a, hipStream_t s) {
+    const int64_t res = t192_resident<MODE, NBKT>();
+    const int64_t g0 = res > 0 ? res : 256;
+    const int ntw = NBKT > 0 ? tri_items(NBKT) : t192_work_tiles(a.nb, a.nant, a.ntiles);
+    const int64_t items = t192_items(a.nunits, a.nsplit, ntw, t192_unit_group(a.nunits, g0));
+    const int64_t grid = g0 < items ? g0 : items;
+    hipLaunchKernelGGL((herk_t192_kernel<MODE, NBKT>), dim3((unsigned)grid), dim3(512), 0, s, a);
+    return hipGetLastError();
 }
 
 template <int MODE>
@@ -2450,13 +2583,18 @@ static hipError_t launch_t192_t(const FusedArgs& a0, hipStream_t s) {
     if (a0.kpad % 16) return hipErrorInvalidValue;
     FusedArgs a = a0;
     if (g_pf_override == 6) a.lists = 1;
-    const int64_t res = t192_resident<MODE>();
-    const int64_t g0 = res > 0 ? res : 256;
-    const int64_t items = t192_items(a.nunits, a.nsplit, t192_work_tiles(a.nb, a.nant, a.ntiles),
-                                     t192_unit_group(a.nunits, g0));
-    const int64_t grid = g0 < items ? g0 : items;
-    hipLaunchKernelGGL((herk_t192_kernel<MODE>), dim3((unsigned)grid), dim3(512), 0, s, a);
-    return hipGetLastError();
+    // the triangle decomposition stores into 192-tiles of one or two blocks
+    const int nbk = (a.nb <= 2) ? t192_tri_nbk(a.nant, a.lists) : 0;
+    switch (nbk) {
+        case 5: return launch_t192_nbk<MODE, 5>(a, s);
+        case 6: return launch_t192_nbk<MODE, 6>(a, s);
+        case 7: return launch_t192_nbk<MODE, 7>(a, s);
+        case 8: return launch_t192_nbk<MODE, 8>(a, s);
+        case 9: return launch_t192_nbk<MODE, 9>(a, s);
+        case 10: return launch_t192_nbk<MODE, 10>(a, s);
+        case 11: return launch_t192_nbk<MODE, 11>(a, s);
+        default: return launch_t192_nbk<MODE, 0>(a, s);
+    }
 }
 
 template <int MODE, int TILE>
@@ -2524,8 +2662,10 @@ void herk_init() {
     (void)fused8_resident<MODE_SPLIT_BF16>();
     (void)fused_resident<MODE_SPLIT_FP16, 64>();
     (void)fused8_resident<MODE_SPLIT_FP16>();
-    (void)t192_resident<MODE_SPLIT_BF16>();
-    (void)t192_resident<MODE_SPLIT_FP16>();
+    (void)t192_resident<MODE_SPLIT_BF16, 0>();
+    (void)t192_resident<MODE_SPLIT_FP16, 0>();
+    (void)t192_resident<MODE_SPLIT_BF16, 11>();
+    (void)t192_resident<MODE_SPLIT_FP16, 11>();
 }
 
 int herk_prefetch(int mode, int tile) {

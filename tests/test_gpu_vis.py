@@ -247,15 +247,18 @@ def test_operands_built_in_contraction_equal_phasor_build_bitwise(gemm, tile, na
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("nant", [321, 350, 352])
+@pytest.mark.parametrize("nant", [129, 160, 161, 192, 200, 230, 256, 257, 288, 289, 300, 320, 321, 350, 352])
 def test_triangle_strip_items_equal_tile_pairs_bitwise(gemm, nant):
-    # arrays of 321..352 antennas (eleven 32-row blocks) cut the upper
-    # triangle of blocks into a triangle item (r <= c < 8) and a strip item
-    # (columns 8-10) per (unit, split); pf 6 runs the same array as the
-    # paired diagonal tiles and the off-diagonal tile.  Every block runs the
-    # same MFMA chain on the same operand bits, so the visibilities are equal
-    # bit for bit; N = 321 leaves one antenna in the last block, N = 352 none
-    # past it.  K spans several 8192-source splits; 2 times x 3 frequencies.
+    # arrays of 129..352 antennas (5..11 blocks of 32 rows) run the items of
+    # the triangle decomposition: one item holding every block up to eight
+    # blocks, else a triangle item (the first 6, 7 or 8 block columns) and a
+    # strip item (the last three); pf 6 runs the same array as 192-tiles
+    # (the paired diagonal tiles and the off-diagonal tile).  Every block
+    # runs the same MFMA chain on the same operand bits, so the visibilities
+    # are equal bit for bit; the sizes cover each block count, one antenna
+    # past a block edge (161, 257, 289, 321) and full last blocks (160, 192,
+    # 256, 288, 320, 352).  K spans several 8192-source splits; 2 times x 3
+    # frequencies.
     from hera_gpu_amd import _lib
     lib = _lib.load()
     w = configs.make_hera350(nsrc=2 * 8192 + 700, ntimes=2, nfreq=3)
