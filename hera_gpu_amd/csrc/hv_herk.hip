@@ -549,6 +549,16 @@ __device__ __forceinline__ void wait_vmcnt() {
 
 #define HV_LDS(p) ((__attribute__((address_space(3))) void*)(p))
 
+// s_barrier without a memory fence (the waits before it are explicit, so
+// loads issued for later stages stay in flight), fenced for the compiler
+// only: no memory access moves across it, as LDS written by other waves
+// before the barrier is read after it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // Issue the direct-to-LDS loads of one stage (A and B tiles) into `slot`.
 template <int TILE>
 __device__ __forceinline__ void ring_issue(char* slot, const char* gA, const char* gB, const int* offA, const int* offB,
@@ -673,7 +683,7 @@ __device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s
         HV_ISSUE(s2, 2);
         HV_ISSUE(s3, 3);
         wait_vmcnt<(RING - 1) * PER_STAGE>();
-        __builtin_amdgcn_s_barrier();
+        lds_barrier();
         // 64-tiles: groups of FOLD_SRC sources accumulate from zero and are
         // added to the FP32 totals; 128-tiles (one wave per SIMD, no
         // registers for a second set of accumulators) run one chain per split
@@ -713,7 +723,7 @@ __device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s
                     }
                     __builtin_amdgcn_sched_barrier(0);
                     wait_vm_lgkm<(RING - 2) * PER_STAGE, 0>();
-                    __builtin_amdgcn_s_barrier();
+                    lds_barrier();
                     __builtin_amdgcn_sched_barrier(0);
                     HV_ISSUE(cur, st + k + RING);
                     read_frag<NS>(fa, nxt, nxt + OP, ra0, rb0, lh);   // past the end: a re-read, unused
@@ -735,7 +745,7 @@ __device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s
             }
         }
         wait_vm_lgkm<0, 0>();
-        __builtin_amdgcn_s_barrier();
+        lds_barrier();
         return;
     }
 
@@ -759,7 +769,7 @@ __device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s
 #pragma unroll
                 for (int k = 0; k < RING; ++k) {
                     wait_vmcnt<(RING - 2) * PER_STAGE>();  // stage st+k has landed (this wave)
-                    __builtin_amdgcn_s_barrier();          // ... for every wave; slot k-1 is free
+                    lds_barrier();          // ... for every wave; slot k-1 is free
                     HV_ISSUE(slots[(k + RING - 1) % RING], st + k + RING - 1);
                     if constexpr (KIND != 0)
                         stage_mma<MODE, TILE, KIND == 1, true>(slots[k], slots[k] + OP, wr, wc, lrow, lh, smask, re,
@@ -790,7 +800,7 @@ __device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s
 #pragma unroll
             for (int k = 0; k < RING; ++k) {
                 wait_vmcnt<(RING - 2) * PER_STAGE>();
-                __builtin_amdgcn_s_barrier();
+                lds_barrier();
                 HV_ISSUE(slots[(k + RING - 1) % RING], st + k + RING - 1);
                 if constexpr (KIND != 0)
                     stage_mma<MODE, TILE, KIND == 1, true>(slots[k], slots[k] + OP, wr, wc, lrow, lh, smask, tre, tim,
@@ -806,7 +816,7 @@ __device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s
     }
 #undef HV_ISSUE
     wait_vmcnt<0>();                 // the re-reads past the end have landed too
-    __builtin_amdgcn_s_barrier();    // every wave is done with the slots
+    lds_barrier();    // every wave is done with the slots
 }
 
 // Work item L (a (unit, split, tile) triple, numbered as decode_work reads
@@ -1071,7 +1081,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
     HV_FISSUE(in2, 2);
     HV_FISSUE(in3, 3);
     wait_vmcnt<(FRING - 1) * PER>();
-    __builtin_amdgcn_s_barrier();
+    lds_barrier();
     fused_build<MODE, TILE, RPT>(in0, op0, lane, wave, ax, ay, az);
 
     for (int st = 0; st < n_st;) {
@@ -1095,7 +1105,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
                 // inputs of stage st+k+1 have landed (this wave); every
                 // wave's LDS writes and reads of the previous stage are done
                 wait_vm_lgkm<(FRING - 2) * PER, 0>();
-                __builtin_amdgcn_s_barrier();
+                lds_barrier();
                 HV_FISSUE(ins[k], st + k + FRING);           // slot of stage st+k, consumed
                 SplitFrag<NS> fa, fb;
                 read_frag<NS>(fa, cur, cur + boff, ra0, rb0, lh);
@@ -1127,7 +1137,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
     }
 #undef HV_FISSUE
     wait_vm_lgkm<0, 0>();
-    __builtin_amdgcn_s_barrier();
+    lds_barrier();
 }
 
 template <int MODE, int TILE>
@@ -1413,7 +1423,7 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
     HV_F8ISSUE(in2, 2);
     HV_F8ISSUE(in3, 3);
     wait_vmcnt<(FRING - 1) * PER>();
-    __builtin_amdgcn_s_barrier();
+    lds_barrier();
     fused8_build<MODE, RPT>(in0, op0, rbase, g, ax, ay, az);
 
     for (int st = 0; st < n_st; st += FRING) {
@@ -1422,7 +1432,7 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
             char* cur = ops[k & 1];
             char* nxt = ops[(k + 1) & 1];
             wait_vm_lgkm<(FRING - 2) * PER, 0>();
-            __builtin_amdgcn_s_barrier();
+            lds_barrier();
             HV_F8ISSUE(ins[k], st + k + FRING);
             // the two waves of a SIMD (h = 0 and 1) take the build and the
             // MFMAs in opposite order, so one's matrix work runs while the
@@ -1440,7 +1450,7 @@ __device__ __forceinline__ void fused8_split(char* in0, char* in1, char* in2, ch
     }
 #undef HV_F8ISSUE
     wait_vm_lgkm<0, 0>();
-    __builtin_amdgcn_s_barrier();
+    lds_barrier();
 }
 
 template <int MODE>
@@ -2271,7 +2281,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         // is in LDS; slot st & 1 (read by that build) and the operand buffer
         // of stage st-1 are free
         wait_vm_lgkm<0, 0>();
-        __builtin_amdgcn_s_barrier();
+        lds_barrier();
         if (st + 2 < n_st) HV_T192ISSUE(odd ? in1 : in0, st + 2);
         // build stage st+1 (at the last stage: an earlier stage again, from a
         // slot nobody refills, into the buffer no MFMA reads)
@@ -2287,7 +2297,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
 #undef HV_T192MMA
 #undef HV_T192ISSUE
     wait_vm_lgkm<0, 0>();
-    __builtin_amdgcn_s_barrier();
+    lds_barrier();
 }
 
 // Work item L -> (unit, split, tile), or false for a slot with no work.
