@@ -1704,6 +1704,42 @@ __device__ __forceinline__ void t192_build(const lds_char* in, lds_char* op, int
     }
 }
 
+// The same for rows whose amplitude rows are not their operand rows (items
+// of several units: operand row v takes amplitude row v mod 32 NBK):
+// arow[k], the amplitude row of the lane's group k (multiple of 16).
+template <int MODE, int NR>
+__device__ __forceinline__ void t192_build_mu(const lds_char* in, lds_char* op, int row0, int lane,
+                                              const int (&arow)[3], const double (&ax)[3], const double (&ay)[3],
+                                              const double (&az)[3]) {
+    using L = T192;
+    const int g = lane >> 4, l16 = lane & 15;
+    const lds_char* cin = in + L::AMP + 32 * g;
+    const int aswz = (g ^ ((l16 >> 1) & 3)) << 4;
+    lds_char* wop = op + t192_off(row0 + l16, 2 * g);
+    const int wlo = (((2 * g + 1) ^ ((l16 >> 1) & 7)) - ((2 * g) ^ ((l16 >> 1) & 7))) << 4;
+    double x[4], y[4], z[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        x[j] = lds_ld<double>(cin + 8 * j);
+        y[j] = lds_ld<double>(cin + 128 + 8 * j);
+        z[j] = lds_ld<double>(cin + 256 + 8 * j);
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const f32x4 am = lds_ld<f32x4>(in + (arow[k] + l16) * 64 + aswz);
+        float sn[4], cs[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sincos_turns_hw(phase_turns(ax[k], ay[k], az[k], x[j], y[j], z[j]), &sn[j], &cs[j]);
+        uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
+        split_two<MODE>(am[0], cs[0], am[1], cs[1], h0, l0);
+        split_two<MODE>(am[2], cs[2], am[3], cs[3], h1, l1);
+        split_two<MODE>(am[0], sn[0], am[1], sn[1], h2, l2);
+        split_two<MODE>(am[2], sn[2], am[3], sn[3], h3, l3);
+        lds_st(wop + 2048 * k, u32x4{h0, h1, h2, h3});
+        lds_st(wop + 2048 * k + wlo, u32x4{l0, l1, l2, l3});
+    }
+}
+
 // acc += a . b (Re) or a . rot(b) (Im) as hi.hi + hi.lo + lo.hi.
 template <int MODE>
 __device__ __forceinline__ void t192_mac(const u32x4& ahi, const u32x4& alo, const u32x4& bhi, const u32x4& blo,
@@ -2131,6 +2167,67 @@ __device__ __forceinline__ void tri_runs_mma(const lds_char* S, const T192Frag& 
     for (int s = 0; s < 2; ++s) tri_runs_step<MODE, IM, NBK, PART, Q, 0, 0>(S, fo.hi[s], fo.lo[s], acc);
 }
 
+// Items of several units (arrays of 33..128 antennas, NBK = 2..4): the
+// U x NBK row runs (unit u, block row r: that unit's columns r .. NBK-1, as
+// operand rows u NBK + r against u NBK + r .. u NBK + NBK-1) dealt to the
+// four SIMDs longest first, each to the SIMD with the fewest blocks (the
+// lowest index on ties).  Run i of SIMD q, or r < 0 past the list.
+__host__ __device__ constexpr TRun mu_run(int nbk, int U, int q, int i) {
+    int nblk[4] = {0, 0, 0, 0}, nrun[4] = {0, 0, 0, 0};
+    for (int len = nbk; len >= 1; --len)
+        for (int u = 0; u < U; ++u) {
+            int b = 0;
+            for (int k = 1; k < 4; ++k)
+                if (nblk[k] < nblk[b]) b = k;
+            if (b == q && nrun[b] == i) {
+                const int r = u * nbk + nbk - len;
+                return TRun{r, r, u * nbk + nbk - 1};
+            }
+            nblk[b] += len;
+            ++nrun[b];
+        }
+    return TRun{-1, 0, -1};
+}
+__host__ __device__ constexpr int mu_list_blocks(int nbk, int U, int q) {
+    int n = 0;
+    for (int i = 0; i < 8; ++i) {
+        const TRun r = mu_run(nbk, U, q, i);
+        if (r.r >= 0) n += r.c1 - r.c0 + 1;
+    }
+    return n;
+}
+__host__ __device__ constexpr bool mu_run_block(int nbk, int U, int q, int k, int& br, int& bc) {
+    for (int i = 0; i < 3; ++i) {
+        const TRun r = mu_run(nbk, U, q, i);
+        if (r.r < 0) return false;
+        const int n = r.c1 - r.c0 + 1;
+        if (k < n) {
+            br = r.r;
+            bc = r.c0 + k;
+            return true;
+        }
+        k -= n;
+    }
+    return false;
+}
+template <int MODE, bool IM, int NBK, int U, int Q, int I, int K0>
+__device__ __forceinline__ void mu_runs_step(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
+    if constexpr (I < 3) {
+        constexpr TRun run = mu_run(NBK, U, Q, I);
+        if constexpr (run.r >= 0) {
+            t192_diag_row<MODE, IM, run.r, run.c0, run.c1, K0>(S, fh, fl, acc);
+            mu_runs_step<MODE, IM, NBK, U, Q, I + 1, K0 + run.c1 - run.c0 + 1>(S, fh, fl, acc);
+        }
+    }
+}
+template <int MODE, bool IM, int NBK, int U, int Q>
+__device__ __forceinline__ void mu_runs_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9]) {
+    static_assert(mu_list_blocks(NBK, U, Q) <= 9, "a SIMD list holds at most nine blocks");
+    static_assert(mu_run(NBK, U, Q, 3).r < 0, "a SIMD list holds at most three runs");
+#pragma unroll
+    for (int s = 0; s < 2; ++s) mu_runs_step<MODE, IM, NBK, U, Q, 0, 0>(S, fo.hi[s], fo.lo[s], acc);
+}
+
 // Build rows of the run-list items: the item's 16-row groups (rows 0 ..
 // 32 n - 1 of the triangle's n block columns, or all NBK blocks' rows for
 // the strip) split evenly over the waves, wave 4 IM + Q taking groups
@@ -2524,6 +2621,163 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
 
 #undef HV_LDS
 
+// ---------------------------------------------------------------------------
+// Items of U frequency units of one time for arrays of 33..128 antennas
+// (NBK = 2..4 blocks of 32 rows, U = 6, 4, 3: 384 operand rows per stage).
+// The units share the time's amplitude rows and source directions, so one
+// input DMA and one barrier per stage serve all of them, and a SIMD's list
+// holds up to nine blocks as at HERA-350 (a 128-antenna tile alone has 3,
+// 3, 2, 2).  Operand row v is antenna v mod 32 NBK of unit v / (32 NBK) of
+// the item; the MFMA chain of every block is that of the other kernels, so
+// the partial sums are theirs bit for bit.  Partials: the unit's TILE x
+// TILE tile (TILE = 64 for NBK = 2, 128 for 3 and 4).
+template <int MODE, bool IM, int Q, int NBK, int U>
+__device__ __forceinline__ void mu_split(lds_char* in0, lds_char* in1, lds_char* op0, lds_char* op1,
+                                         const double* antpos, int nant, const double* inv_lambda, int f0, int nu,
+                                         const float* amp_t, const double* crd_t, size_t amp_stage, int s_begin,
+                                         int n_st, const int* ampoff, int crdoff, int lane, int wave,
+                                         f32x16 (&acc)[9]) {
+    constexpr int R32 = NBK * 32;
+    constexpr int G = U * R32 / 16;                      // 16-row groups of the item
+    constexpr int W = (IM ? 4 : 0) + Q;
+    constexpr int G0 = W * G / 8, G1 = (W + 1) * G / 8;
+    constexpr int NR = G1 - G0;
+    static_assert(NR >= 1 && NR <= 3, "one to three 16-row groups per wave");
+    const int row0 = 16 * G0;
+    // antenna positions (wavelengths of the row's unit) and amplitude rows
+    double ax[3], ay[3], az[3];
+    int arow[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int kk = k < NR ? k : 0;
+        const int v = row0 + (lane & 15) + 16 * kk;
+        const int u = v / R32, r = v - u * R32;
+        const int ant = min(r, nant - 1);
+        const double il = inv_lambda[f0 + min(u, nu - 1)];
+        ax[k] = antpos[3 * ant] * il;
+        ay[k] = antpos[3 * ant + 1] * il;
+        az[k] = antpos[3 * ant + 2] * il;
+        arow[k] = (row0 + 16 * kk) % R32;
+    }
+    const T192Frag fo(lane);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[k][e] = 0;
+#define HV_MUISSUE(slot, st)                                                                                  \
+    t192_issue<R32>(slot, amp_t + (size_t)(s_begin + (st)) * amp_stage, crd_t + (size_t)(s_begin + (st)) * 16, \
+                    ampoff, crdoff, wave, lane)
+    HV_MUISSUE(in0, 0);
+    wait_vmcnt<0>();
+    __syncthreads();                       // stage 0's inputs are in LDS
+    HV_MUISSUE(in1, min(1, n_st - 1));
+    t192_build_mu<MODE, NR>(in0, op0, row0, lane, arow, ax, ay, az);
+    for (int st = 0; st < n_st; ++st) {
+        const bool odd = st & 1;
+        const lds_char* cur = odd ? op1 : op0;
+        wait_vm_lgkm<0, 0>();
+        lds_barrier();
+        if (st + 2 < n_st) HV_MUISSUE(odd ? in1 : in0, st + 2);
+        t192_build_mu<MODE, NR>(odd ? in0 : in1, odd ? op0 : op1, row0, lane, arow, ax, ay, az);
+        __builtin_amdgcn_sched_barrier(0);
+        mu_runs_mma<MODE, IM, NBK, U, Q>(cur, fo, acc);
+    }
+#undef HV_MUISSUE
+    wait_vm_lgkm<0, 0>();
+    lds_barrier();
+}
+
+template <int MODE, int NBK, int U>
+__global__ __launch_bounds__(512) void herk_mu_kernel(FusedArgs a) {
+    using L = T192;
+    constexpr int R32 = NBK * 32;
+    constexpr int PT = NBK <= 2 ? 64 : 128;
+    static_assert(U * R32 <= L::ROWS, "an item's operand rows fit the operand buffer");
+    __shared__ __attribute__((aligned(1024))) char s_in0[L::IN];
+    __shared__ __attribute__((aligned(1024))) char s_in1[L::IN];
+    __shared__ __attribute__((aligned(1024))) char s_op0[2 * L::OP];
+    __shared__ __attribute__((aligned(1024))) char s_op1[2 * L::OP];
+    lds_char* in0 = (lds_char*)s_in0;
+    lds_char* in1 = (lds_char*)s_in1;
+    lds_char* op0 = (lds_char*)s_op0;
+    lds_char* op1 = (lds_char*)s_op1;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int q = wave & 3;
+    const bool im = wave >= 4;
+    const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / 16);
+    const size_t amp_stage = (size_t)a.nant * 16;
+    const size_t amp_time = (size_t)a.kpad * a.nant;
+    constexpr int STAGES_SPLIT = KSPLIT / 16;
+    const int nt = a.nunits / a.nfreq;                   // times of the launch
+    const int gpt = (a.nfreq + U - 1) / U;               // unit groups per time
+    const int ngroups = nt * gpt;
+    const int ug = t192_unit_group(ngroups, gridDim.x);
+    const int64_t nitems = t192_items(ngroups, a.nsplit, 1, ug);
+    // amplitude chunk c = (i*8 + wave)*64 + lane: antenna row c/4 (< 32 NBK)
+    int ampoff[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int c = (i * 8 + wave) * 64 + lane;
+        const int rr = c >> 2;
+        ampoff[i] = min(rr, a.nant - 1) * 16 + ((c & 3) ^ amp_swz(rr)) * 4;
+    }
+    const int crdoff = min(wave, 2) * (int)a.kpad + 2 * lane;   // used by lanes 0-7 of waves 0-2
+
+    for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
+        int grp, split, tile;
+        if (!t192_decode(ngroups, a.nsplit, 1, ug, Lw, grp, split, tile)) continue;
+        const int s_begin = split * STAGES_SPLIT;
+        const int s_end = min(s_begin + STAGES_SPLIT, kst);
+        if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
+        const int n_st = s_end - s_begin;
+        const int tb = grp / gpt, f0 = (grp - tb * gpt) * U;
+        const int nu = min(U, a.nfreq - f0);
+        const float* amp_t = a.amp + (size_t)tb * amp_time;
+        const double* crd_t = a.crd_top + (size_t)tb * 3 * a.kpad;
+
+        f32x16 acc[9];
+#define HV_MUSPLIT(IM, Q)                                                                                      \
+    mu_split<MODE, IM, Q, NBK, U>(in0, in1, op0, op1, a.antpos, a.nant, a.inv_lambda, f0, nu, amp_t, crd_t,      \
+                                  amp_stage, s_begin, n_st, ampoff, crdoff, lane, wave, acc)
+        if (im) {
+            if (q == 0) HV_MUSPLIT(true, 0);
+            else if (q == 1) HV_MUSPLIT(true, 1);
+            else if (q == 2) HV_MUSPLIT(true, 2);
+            else HV_MUSPLIT(true, 3);
+        } else {
+            if (q == 0) HV_MUSPLIT(false, 0);
+            else if (q == 1) HV_MUSPLIT(false, 1);
+            else if (q == 2) HV_MUSPLIT(false, 2);
+            else HV_MUSPLIT(false, 3);
+        }
+#undef HV_MUSPLIT
+
+        constexpr size_t TSZ = (size_t)PT * PT;
+        // the SIMD slot, opaque to the compiler so that the block coordinates
+        // are derived per item instead of being hoisted out of the item loop
+        int qs = q;
+        asm volatile("" : "+v"(qs));
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            int br = 0, bc = 0;
+            if (!mu_run_block(NBK, U, qs, k, br, bc)) continue;
+            const int u = br / NBK, r = br - u * NBK, c = bc - u * NBK;
+            if (u >= nu || r * 32 >= a.nant || c * 32 >= a.nant) continue;
+            const int unit = tb * a.nfreq + f0 + u;
+            float* out = reinterpret_cast<float*>(a.partials) + ((size_t)unit * a.nsplit + split) * 2 * TSZ +
+                         (im ? TSZ : 0);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int row = r * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+                const int col = c * 32 + (lane & 31);
+                out[(size_t)row * PT + col] = acc[k][e];
+            }
+        }
+    }
+}
+
 // Resident workgroups of a fused kernel on the whole device (a multiple of 8).
 template <int MODE, int TILE>
 static int64_t fused_resident() {
@@ -2607,6 +2861,45 @@ static hipError_t launch_t192_t(const FusedArgs& a0, hipStream_t s) {
     }
 }
 
+template <int MODE, int NBK, int U>
+static int64_t mu_resident() {
+    static const int64_t cached = [] {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, herk_mu_kernel<MODE, NBK, U>, 512, 0) !=
+                hipSuccess)
+            return (int64_t)0;
+        return (int64_t)ncu * per_cu / 8 * 8;
+    }();
+    return cached;
+}
+
+template <int MODE, int NBK, int U>
+static hipError_t launch_mu_t(const FusedArgs& a, hipStream_t s) {
+    if (a.nfreq < 1 || a.nunits % a.nfreq) return hipErrorInvalidValue;
+    const int64_t res = mu_resident<MODE, NBK, U>();
+    const int64_t g0 = res > 0 ? res : 256;
+    const int ngroups = a.nunits / a.nfreq * ((a.nfreq + U - 1) / U);
+    const int64_t items = t192_items(ngroups, a.nsplit, 1, t192_unit_group(ngroups, g0));
+    const int64_t grid = g0 < items ? g0 : items;
+    hipLaunchKernelGGL((herk_mu_kernel<MODE, NBK, U>), dim3((unsigned)grid), dim3(512), 0, s, a);
+    return hipGetLastError();
+}
+
+// Items of several units for split modes at 33..128 antennas, when the plan's
+// tile is the one the items store into and every unit of a time shares one
+// A sqrt(I) (one beam); pf 6 keeps the 64- and 128-tile kernels.
+template <int MODE>
+static bool launch_mu(int tile, const FusedArgs& a, hipStream_t s, hipError_t& err) {
+    const int nbk = (a.nant + 31) / 32;
+    if (a.amp_per_unit || a.lists == 1 || g_pf_override == 6 || nbk < 2 || nbk > 4) return false;
+    if (tile != (nbk <= 2 ? 64 : 128) || a.kpad % 16) return false;
+    err = nbk == 2 ? launch_mu_t<MODE, 2, 6>(a, s) : nbk == 3 ? launch_mu_t<MODE, 3, 4>(a, s)
+                                                   : launch_mu_t<MODE, 4, 3>(a, s);
+    return true;
+}
+
 template <int MODE, int TILE>
 static hipError_t launch_fused_t(const FusedArgs& a, hipStream_t s) {
     const int64_t nblk = (int64_t)a.nunits * a.nsplit * a.ntiles;
@@ -2619,6 +2912,10 @@ static hipError_t launch_fused_t(const FusedArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s) {
+    if ((int64_t)a.nunits * a.nsplit * a.ntiles <= 0) return hipSuccess;
+    hipError_t err = hipSuccess;
+    if (mode == MODE_SPLIT_BF16 && launch_mu<MODE_SPLIT_BF16>(tile, a, s, err)) return err;
+    if (mode == MODE_SPLIT_FP16 && launch_mu<MODE_SPLIT_FP16>(tile, a, s, err)) return err;
     if (mode == MODE_SPLIT_BF16 && tile == 64) return launch_fused_t<MODE_SPLIT_BF16, 64>(a, s);
     if (mode == MODE_SPLIT_BF16 && tile == 128) return launch_fused8_t<MODE_SPLIT_BF16>(a, s);
     if (mode == MODE_SPLIT_FP16 && tile == 64) return launch_fused_t<MODE_SPLIT_FP16, 64>(a, s);

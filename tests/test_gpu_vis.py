@@ -280,6 +280,34 @@ def test_triangle_strip_items_equal_tile_pairs_bitwise(gemm, nant):
     assert rel_err(outs[0][0, -1], ref) < VIS_TOL[gemm]
 
 
+@pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
+@pytest.mark.parametrize("nant,nfreq", [(33, 7), (37, 5), (64, 13), (65, 5), (96, 9), (97, 3), (128, 7)])
+def test_multi_unit_items_equal_tile_kernels_bitwise(gemm, nant, nfreq):
+    # arrays of 33..128 antennas run items holding several frequency units
+    # of one time (6, 4 or 3 units at 2, 3 or 4 blocks of 32 rows) that share
+    # the time's amplitude rows.  Every block runs the MFMA chain of the
+    # 128-tile LDS ring on the same operand bits (tile 128, pf 4: one unit
+    # per work item, operands written by phasor_kernel), so the visibilities
+    # are equal bit for bit (the 64-tile kernels fold their accumulators
+    # every 512 sources and differ in the last bits).  The frequency counts
+    # leave the last group of each time short; K spans several 8192-source
+    # splits; 2 times.
+    from hera_gpu_amd import _lib
+    lib = _lib.load()
+    w = configs.make_hera350(nsrc=2 * 8192 + 700, ntimes=2, nfreq=nfreq)
+    ap, bm = w.antpos[:nant], w.bm_cube[:nant]
+    outs = []
+    try:
+        for tile, pf in ((0, 0), (128, 4)):
+            _lib.check(lib.hv_set_herk_variant(tile, pf))
+            outs.append(vis_gpu(ap, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, bm, gemm=gemm))
+    finally:
+        _lib.check(lib.hv_set_herk_variant(0, 0))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    ref = vis_oracle(ap, w.freqs[-1:], w.eq2tops[-1:], w.crd_eq, w.I_sky, bm)[0, 0]
+    assert rel_err(outs[0][-1, -1], ref) < VIS_TOL[gemm]
+
+
 @pytest.mark.parametrize("gemm", ["fp32", "split_bf16", "fp64"])
 def test_baseline_layouts_equal_full_matrix_bitwise(gemm):
     # the reduce writes only a <= b, in (T, Nbl, F) or (F, T, Nbl) order; the
