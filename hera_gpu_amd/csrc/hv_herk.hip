@@ -49,11 +49,16 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDS_STRIDE = 144;
 
-// Built-in variants (measured on MI355X): split modes take 128-tiles above
-// 64 antennas (fewer tiles share, and rebuild, each antenna block; for
-// operands in HBM the tiles of a split share their stages in L2: reads ~1.1x
-// the operand bytes against ~3.5x for 64-tiles) and one 64-tile below.  FP32
-// and FP64 keep the register-staged 64-tile kernel, their fastest.
+// Built-in variants (measured on MI355X): split modes build their operands
+// inside the contraction, in items of several frequency units at 33..128
+// antennas (herk_mu_kernel), in the one or two items of the triangle
+// decomposition at 129..352 (herk_t192_kernel with run or triangle/strip
+// lists) and in 192-tiles above; 64-tiles up to 32 antennas and with a beam
+// per frequency below 129.  Contractions of operands already in HBM take
+// the LDS ring with 128-tiles above 64 antennas (the tiles of a split share
+// their stages in L2: reads ~1.1x the operand bytes against ~3.5x for
+// 64-tiles).  FP32 and FP64 keep the register-staged 64-tile kernel, their
+// fastest.
 static int g_tile_override = 0, g_pf_override = 0;   // 0 = built-in choice
 
 void set_herk_variant(int tile, int pf) {
