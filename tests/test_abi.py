@@ -74,10 +74,12 @@ def test_bad_precision_raises_assertion():
 
 
 @pytest.mark.parametrize("nant,tile", [(1, 64), (37, 64), (64, 64), (65, 128), (128, 128), (129, 192), (160, 192),
-                                       (192, 192), (193, 128), (256, 128), (257, 192), (350, 192)])
+                                       (192, 192), (193, 192), (256, 192), (257, 192), (350, 192), (520, 192)])
 def test_contraction_tile_choice(nant, tile):
-    # host-side choice, no GPU call: split modes take 192-tiles where their
-    # last block is fuller than a 128-tile's would be
+    # host-side choice, no GPU call: split modes store their partials in
+    # 64-tiles up to 64 antennas, 128-tiles up to 128 (items of several
+    # frequency units), 192-tiles above (the triangle decomposition up to
+    # 352 antennas, 192-tiles beyond); FP32 keeps 64-tiles
     import ctypes as C
     lib = _lib.load()
     t, p = C.c_int(), C.c_int()
