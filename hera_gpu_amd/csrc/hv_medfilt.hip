@@ -30,14 +30,16 @@ namespace {
 
 constexpr int MF_TX = 64;        // output columns per workgroup, one per lane
 constexpr int MF_TY = 32;        // output rows per workgroup: 4 row pairs per wave
+constexpr int MF_TY_SMALL = 16;  // for batches of fewer than MF_SMALL_TILES 32-row tiles
+constexpr int MF_SMALL_TILES = 1024;
 constexpr int MF_THREADS = 256;
-constexpr int MF_PAIRS = MF_TY / 2 / (MF_THREADS / MF_TX);   // row pairs per thread
 constexpr int MF_MAX_SIDE = 127;
 
 struct MfArgs {
     const float* in;      // [nimg][nrows][ncols]
     float* out;           // [nimg][nrows][ncols]
     int nrows, ncols, wr, wc;
+    int ty;               // output rows per tile (MF_TY or MF_TY_SMALL)
     int tiles_x;
     int64_t tiles_per_img, ntiles, chunk;   // chunk = tiles handled by one XCD
 };
@@ -152,7 +154,7 @@ __device__ __forceinline__ bool tile_at(const MfArgs& a, TileAt& t) {
     const int64_t plane = (int64_t)a.nrows * a.ncols;
     t.img = a.in + img * plane;
     t.out = a.out + img * plane;
-    t.row0 = ty * MF_TY;
+    t.row0 = ty * a.ty;
     t.col0 = tx * MF_TX;
     return true;
 }
@@ -174,11 +176,13 @@ __device__ __forceinline__ void put(const MfArgs& a, const TileAt& t, int r, int
         t.out[(int64_t)(t.row0 + r) * a.ncols + t.col0 + c] = __int_as_float(total_order_key(key));
 }
 
-// Windows of WX rows x WY columns fixed at compile time.
-template <int WX, int WY>
+// Windows of WX rows x WY columns fixed at compile time, tiles of TY rows
+// (TY / 8 row pairs per thread).
+template <int WX, int WY, int TY>
 __global__ __launch_bounds__(MF_THREADS) void medfilt_kernel(MfArgs a) {
-    constexpr int LW = MF_TX + WY - 1, LH = MF_TY + WX - 1;
+    constexpr int LW = MF_TX + WY - 1, LH = TY + WX - 1;
     constexpr int W = WX * WY, K = W / 2;
+    constexpr int PAIRS = TY / 2 / (MF_THREADS / MF_TX);
     __shared__ int lds[LH * LW];
     TileAt t;
     if (!tile_at(a, t)) return;    // whole workgroup, before any barrier
@@ -187,8 +191,8 @@ __global__ __launch_bounds__(MF_THREADS) void medfilt_kernel(MfArgs a) {
     const int c = threadIdx.x & (MF_TX - 1);
     const int* col = lds + c;
 #pragma unroll 1
-    for (int p = 0; p < MF_PAIRS; ++p) {
-        const int r0 = 2 * (MF_PAIRS * (threadIdx.x / MF_TX) + p);   // window rows r0 .. r0 + WX: two outputs
+    for (int p = 0; p < PAIRS; ++p) {
+        const int r0 = 2 * (PAIRS * (threadIdx.x / MF_TX) + p);   // window rows r0 .. r0 + WX: two outputs
         int res0, res1;
         if constexpr (WX >= 3) {
             // rows r0 + 1 .. r0 + WX - 1 are in both windows: keep ranks K - WY .. K
@@ -234,15 +238,16 @@ __global__ __launch_bounds__(MF_THREADS) void medfilt_kernel(MfArgs a) {
 // its bits, counting the window keys below each candidate.
 __global__ __launch_bounds__(MF_THREADS) void medfilt_generic_kernel(MfArgs a) {
     extern __shared__ int lds_dyn[];
-    const int lw = MF_TX + a.wc - 1, lh = MF_TY + a.wr - 1;
+    const int lw = MF_TX + a.wc - 1, lh = a.ty + a.wr - 1;
     const int K = (a.wr * a.wc) / 2;
+    const int pairs = a.ty / 2 / (MF_THREADS / MF_TX);
     TileAt t;
     if (!tile_at(a, t)) return;
     stage(a, t, lds_dyn, lh, lw, a.wr / 2, a.wc / 2);
     __syncthreads();
     const int c = threadIdx.x & (MF_TX - 1);
-    const int r0 = 2 * MF_PAIRS * (threadIdx.x / MF_TX);
-    for (int o = 0; o < 2 * MF_PAIRS; ++o) {
+    const int r0 = 2 * pairs * (threadIdx.x / MF_TX);
+    for (int o = 0; o < 2 * pairs; ++o) {
         const int* win = lds_dyn + (r0 + o) * lw + c;
         unsigned p = 0;   // unsigned key (sign bit flipped) of the answer, built from the top bit
         for (int b = 31; b >= 0; --b) {
@@ -258,29 +263,34 @@ __global__ __launch_bounds__(MF_THREADS) void medfilt_generic_kernel(MfArgs a) {
 
 using KernelFn = void (*)(MfArgs);
 
-template <int WX>
+template <int WX, int TY>
 KernelFn pick_cols(int wc) {
     switch (wc) {
-        case 1: return medfilt_kernel<WX, 1>;
-        case 3: return medfilt_kernel<WX, 3>;
-        case 5: return medfilt_kernel<WX, 5>;
-        case 7: return medfilt_kernel<WX, 7>;
-        case 9: return medfilt_kernel<WX, 9>;
-        case 11: return medfilt_kernel<WX, 11>;
+        case 1: return medfilt_kernel<WX, 1, TY>;
+        case 3: return medfilt_kernel<WX, 3, TY>;
+        case 5: return medfilt_kernel<WX, 5, TY>;
+        case 7: return medfilt_kernel<WX, 7, TY>;
+        case 9: return medfilt_kernel<WX, 9, TY>;
+        case 11: return medfilt_kernel<WX, 11, TY>;
         default: return nullptr;
     }
 }
 
-KernelFn pick(int wr, int wc) {
+template <int TY>
+KernelFn pick_ty(int wr, int wc) {
     switch (wr) {
-        case 1: return pick_cols<1>(wc);
-        case 3: return pick_cols<3>(wc);
-        case 5: return pick_cols<5>(wc);
-        case 7: return pick_cols<7>(wc);
-        case 9: return pick_cols<9>(wc);
-        case 11: return pick_cols<11>(wc);
+        case 1: return pick_cols<1, TY>(wc);
+        case 3: return pick_cols<3, TY>(wc);
+        case 5: return pick_cols<5, TY>(wc);
+        case 7: return pick_cols<7, TY>(wc);
+        case 9: return pick_cols<9, TY>(wc);
+        case 11: return pick_cols<11, TY>(wc);
         default: return nullptr;
     }
+}
+
+KernelFn pick(int wr, int wc, int ty) {
+    return ty == MF_TY_SMALL ? pick_ty<MF_TY_SMALL>(wr, wc) : pick_ty<MF_TY>(wr, wc);
 }
 
 int check_args(hv_ctx* c, int nimg, int nrows, int ncols, int wr, int wc) {
@@ -303,14 +313,18 @@ int launch_medfilt(const float* in, float* out, int nimg, int nrows, int ncols, 
     a.wr = wr;
     a.wc = wc;
     a.tiles_x = (ncols + MF_TX - 1) / MF_TX;
-    a.tiles_per_img = (int64_t)a.tiles_x * ((nrows + MF_TY - 1) / MF_TY);
+    // batches of fewer than MF_SMALL_TILES 32-row tiles (4 workgroups per CU)
+    // take 16-row tiles: twice the workgroups for the same halo per column
+    const int64_t tiles32 = (int64_t)a.tiles_x * ((nrows + MF_TY - 1) / MF_TY) * nimg;
+    a.ty = tiles32 < MF_SMALL_TILES ? MF_TY_SMALL : MF_TY;
+    a.tiles_per_img = (int64_t)a.tiles_x * ((nrows + a.ty - 1) / a.ty);
     a.ntiles = a.tiles_per_img * nimg;
     a.chunk = (a.ntiles + 7) / 8;
     const dim3 grid((unsigned)(8 * a.chunk)), block(MF_THREADS);
-    if (KernelFn fn = pick(wr, wc)) {
+    if (KernelFn fn = pick(wr, wc, a.ty)) {
         hipLaunchKernelGGL(fn, grid, block, 0, s, a);
     } else {
-        const size_t lds = (size_t)(MF_TY + wr - 1) * (MF_TX + wc - 1) * sizeof(int);
+        const size_t lds = (size_t)(a.ty + wr - 1) * (MF_TX + wc - 1) * sizeof(int);
         if (lds > 64 * 1024)
             HV_HIP(hipFuncSetAttribute((const void*)medfilt_generic_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
