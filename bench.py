@@ -28,19 +28,27 @@ HBM_PEAK_GBS = 8000.0
 # arithmetic, units per launch, tile edge), from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE passes over `bench.py --kernels-only` (KiB per dispatch), scaled
 # by what the same counters read for a 1 GiB device copy: FETCH_SIZE x2.0,
-# WRITE_SIZE x1.0.
+# WRITE_SIZE x1.0.  `src` is the hash of the contraction's sources
+# (KERNEL_SOURCES) when the counters were read: once they change, the figure
+# is reported as null (stale) until the passes are run again.
 MEASURED_TRAFFIC = {
-    # 192-tile in-place build (herk_t192_kernel, triangle and strip items), two
-    # times (128 units) per launch
-    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5484638, "write_kib": 8666112, "fetch_scale": 2.0,
-                                          "write_scale": 1.0},
-    # 128-tile in-place build (--herk-variant 128,5), one time per launch
-    ("hera350", "split_bf16", 64, 128): {"fetch_kib": 138366362, "write_kib": 4325376, "fetch_scale": 2.0,
-                                         "write_scale": 1.0},
-    # phasor build + LDS ring (--herk-variant 128,4), 8 units per launch
-    ("hera350", "split_bf16", 8, 128): {"fetch_kib": 12448316, "write_kib": 540672, "fetch_scale": 2.0,
-                                        "write_scale": 1.0},
+    # in-place build (herk_t192_kernel, triangle and strip items), two times
+    # (128 units) per launch
+    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5117258, "write_kib": 8654336, "fetch_scale": 2.0,
+                                          "write_scale": 1.0, "src": "c1cf81a696e02430"},
 }
+KERNEL_SOURCES = ("hera_gpu_amd/csrc/hv_herk.hip", "hera_gpu_amd/csrc/hv_phasor_math.h",
+                  "hera_gpu_amd/csrc/hv_internal.h")
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    root = os.path.dirname(os.path.abspath(__file__))
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(root, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 PF_NAMES = {1: "register-staged, 1 stage ahead", 2: "register-staged, 2 stages ahead",
             3: "LDS ring, one workgroup per work item", 4: "LDS ring, persistent grid",
             5: "operands built in place from A sqrt(I), 8 waves, persistent grid"}
@@ -575,6 +583,9 @@ def main():
                     "executed_frac": herk_tflops * per_useful / peak}
     upl = units * args.steps / st["herk_n"]
     tr = MEASURED_TRAFFIC.get((args.workload, args.gemm, int(round(upl)), tile))
+    traffic_note = "FETCH_SIZE x2.0 + WRITE_SIZE x1.0 of one launch at this shape, from rocprofv3 --pmc passes"
+    if tr and tr["src"] != kernel_source_hash():
+        tr, traffic_note = None, "stale: the contraction's sources changed since the counter passes"
     traffic = ((tr["fetch_kib"] * tr["fetch_scale"] + tr["write_kib"] * tr["write_scale"]) * 1024
                if tr and abs(upl - round(upl)) < 1e-9 else None)
     out = {
@@ -587,7 +598,7 @@ def main():
                    "gemm": args.gemm, "flops_per_step": flops_job,
                    "flop_count": "useful Hermitian 4 N (N+1) K per unit"},
         "roofline": {"bound": "mfma", "achieved": herk_tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": herk_tflops / peak, "traffic": traffic,
+                     "frac": herk_tflops / peak, "traffic": traffic, "traffic_note": traffic_note,
                      "kernel": f"Hermitian contraction, {args.gemm}, {tile}x{tile} tiles, {variant_name(tile, pf)}",
                      "avg_launch_ms": herk_avg_ms, "launches": st["herk_n"], "units_per_launch": upl,
                      "flops_per_launch": flops_per_launch,
