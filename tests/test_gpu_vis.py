@@ -281,7 +281,8 @@ def test_triangle_strip_items_equal_tile_pairs_bitwise(gemm, nant):
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("nant,nfreq", [(33, 7), (37, 5), (64, 13), (65, 5), (96, 9), (97, 3), (128, 7)])
+@pytest.mark.parametrize("nant,nfreq", [(33, 7), (37, 5), (37, 1), (64, 13), (65, 5), (96, 9), (97, 3), (128, 7),
+                                        (128, 1)])
 def test_multi_unit_items_equal_tile_kernels_bitwise(gemm, nant, nfreq):
     # arrays of 33..128 antennas run items holding several frequency units
     # of one time (6, 4 or 3 units at 2, 3 or 4 blocks of 32 rows) that share
