@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # is reported as null (stale) until the passes are run again.
 MEASURED_TRAFFIC = {
     # in-place build (herk_t192_kernel, triangle and strip items), two times
-    # (128 units) per launch
-    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5117258, "write_kib": 8654336, "fetch_scale": 2.0,
-                                          "write_scale": 1.0, "src": "c1cf81a696e02430"},
+    # (128 units) per launch, 32768-source K-splits (a job of 2048 units)
+    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5076019, "write_kib": 2166272, "fetch_scale": 2.0,
+                                          "write_scale": 1.0, "src": "d1cc34d77f53ff4e"},
 }
 KERNEL_SOURCES = ("hera_gpu_amd/csrc/hv_herk.hip", "hera_gpu_amd/csrc/hv_phasor_math.h",
                   "hera_gpu_amd/csrc/hv_internal.h")

@@ -190,7 +190,8 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
     const size_t unit_bytes = (size_t)nant * operand_row_bytes(mode, kpad);
     const int nb = (nant + tile - 1) / tile;
     const int ntiles = nb * (nb + 1) / 2;
-    const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
+    const int ksplit = job_ksplit(mode, nant, units, kpad);
+    const int nsplit = (int)((kpad + ksplit - 1) / ksplit);
     const size_t part = (size_t)units * nsplit * ntiles * 2 * tile * tile * (mode == HV_MODE_FP64 ? 8 : 4);
     void *W = nullptr, *P = nullptr;
     HV_CHECK(dev_alloc(c, &W, unit_bytes * units));
@@ -208,6 +209,7 @@ extern "C" int hv_bench_herk(hv_ctx* c, int mode, int nant, int64_t nsrc, int un
     ha.ntiles = ntiles;
     ha.kpad = kpad;
     ha.nsplit = nsplit;
+    ha.ksplit = ksplit;
     ha.nunits = units;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const size_t nwords = unit_bytes * units / 4;
@@ -245,6 +247,13 @@ extern "C" int hv_set_herk_variant(int tile, int pf) {
     if (pf == 6 && tile != 192 && tile != 0)
         return fail(HV_ERR_ARG, "pf 6 (tile-pair lists) is a 192-tile variant");
     set_herk_variant(tile, pf);
+    return HV_OK;
+}
+
+extern "C" int hv_job_ksplit(int mode, int nant, int64_t units, int64_t nsrc, int* ksplit) {
+    if (!ksplit || nant < 1 || units < 1 || nsrc < 0) return fail(HV_ERR_ARG, "bad argument");
+    if (mode < MODE_FP32 || mode > MODE_FP64) return fail(HV_ERR_ARG, "unknown contraction mode");
+    *ksplit = job_ksplit(mode, nant, units, round_up(std::max<int64_t>(nsrc, 1), KALIGN));
     return HV_OK;
 }
 

@@ -22,10 +22,12 @@
 //               (hi.hi + hi.lo + lo.hi) on v_mfma_f32_32x32x16_bf16;
 //   SPLIT_FP16  as above with fp16 halves (rows pre-scaled by a power of 2);
 //   FP64        v_mfma_f64_16x16x4_f64 on complex128 rows.
-// FP32 accumulators are folded into a second FP32 register total every 512
-// sources (short MFMA accumulation chains); K is cut into fixed 8192-source
-// splits whose partial tiles are summed in FP64, in split order, by the
-// reduce stage, so results do not depend on how jobs are padded or batched.
+// 64-tiles fold their FP32 accumulators into a second FP32 register total
+// every 512 sources (short MFMA accumulation chains); K is cut into fixed
+// splits of job_ksplit(...) sources (8192, or 32768 for jobs that still
+// hold 2048 work items with them) whose partial tiles are summed in FP64, in
+// split order, by the reduce stage, so results do not depend on how jobs are
+// padded or batched.
 //
 // Workgroup: 4 waves (2 x 2) on a TILE x TILE complex tile; a K stage is
 // 128 bytes of every row (16 complex64 sources, 8 complex128), staged
@@ -162,12 +164,11 @@ struct Cfg {
     static constexpr int NS = WT / BLK;                   // subtiles per wave per dim
     static constexpr int NCH = TILE * (STAGE_BYTES / 16) / 256;   // 16-B chunks / thread / operand
     static constexpr int SRC_STAGE = stage_src(F64);
-    static constexpr int STAGES_SPLIT = KSPLIT / SRC_STAGE;
     // 64-tiles fold the MFMA accumulators into FP32 register totals every
     // FOLD_SRC sources; 128-tiles (split modes only) have no registers to
-    // spare for totals and accumulate each 8192-source split in one chain.
+    // spare for totals and accumulate each K-split in one chain.
     static constexpr bool FOLD_ON = !F64 && TILE == 64;
-    static constexpr int FOLD = FOLD_ON ? FOLD_SRC / SRC_STAGE : STAGES_SPLIT;
+    static constexpr int FOLD = FOLD_ON ? FOLD_SRC / SRC_STAGE : (1 << 30);
     // Im as P1 - P2 (FP32, FP64) or as one chain of w'.w (split modes)
     static constexpr bool PAIR_IM = (MODE == MODE_FP32 || MODE == MODE_FP64);
     using ACC = typename std::conditional<F64, f64x4, f32x16>::type;
@@ -423,8 +424,8 @@ __global__ __launch_bounds__(256) void herk_kernel(HerkArgs a) {
     const bool diag = (bi == bj);
 
     const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / C::SRC_STAGE);
-    const int s_begin = split * C::STAGES_SPLIT;
-    const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
+    const int s_begin = split * (a.ksplit / C::SRC_STAGE);
+    const int s_end = min(s_begin + a.ksplit / C::SRC_STAGE, kst);
     if (s_begin >= s_end) return;   // split beyond the job's sources (uniform per workgroup)
 
     // A subtile that lies strictly below the diagonal of a diagonal tile, or
@@ -693,7 +694,7 @@ __device__ __forceinline__ void ring_split(char* s0, char* s1, char* s2, char* s
         // added to the FP32 totals; 128-tiles (one wave per SIMD, no
         // registers for a second set of accumulators) run one chain per split
         // straight into the totals.
-        constexpr int RFOLD = (TILE == 64) ? FOLD_SRC / C::SRC_STAGE : C::STAGES_SPLIT;
+        constexpr int RFOLD = (TILE == 64) ? FOLD_SRC / C::SRC_STAGE : (1 << 30);
         constexpr int NMF = 6 * NS * NS;               // MFMAs per 8-source step
         constexpr int NRD = 4 * NS;                    // LDS reads per 8-source step
         SplitFrag<NS> fa, fb;
@@ -858,8 +859,8 @@ __global__ __launch_bounds__(256) void herk_ring_kernel(HerkArgs a) {
         decode_work(a, L, unit, split, tile);
         decode_tile(tile, a.nb, bi, bj);
         const bool diag = (bi == bj);
-        const int s_begin = split * C::STAGES_SPLIT;
-        const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
+        const int s_begin = split * (a.ksplit / C::SRC_STAGE);
+        const int s_end = min(s_begin + a.ksplit / C::SRC_STAGE, kst);
         if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
         const int n_st = s_end - s_begin; // a multiple of RING
 
@@ -1064,7 +1065,7 @@ __device__ __forceinline__ void fused_split(char* in0, char* in1, char* in2, cha
     constexpr int NMF = 6 * NS * NS;
     constexpr int PER = L::PER_STAGE;
     constexpr int VPM = (RPT * 4 * 36) / (2 * NMF);     // build VALU ops per MFMA (~36 per element)
-    constexpr int RFOLD = (TILE == 64) ? FOLD_SRC / C::SRC_STAGE : C::STAGES_SPLIT;
+    constexpr int RFOLD = (TILE == 64) ? FOLD_SRC / C::SRC_STAGE : (1 << 30);
     char* ins[FRING] = {in0, in1, in2, in3};
     char* ops[2] = {op0, op1};
     const int lrow = lane & 31, lh = lane >> 5;
@@ -1175,8 +1176,8 @@ __global__ __launch_bounds__(256) void herk_fused_kernel(FusedArgs a) {
         decode_work(a.nsplit, a.ntiles, Lw, unit, split, tile);
         decode_tile(tile, a.nb, bi, bj);
         const bool diag = (bi == bj);
-        const int s_begin = split * C::STAGES_SPLIT;
-        const int s_end = min(s_begin + C::STAGES_SPLIT, kst);
+        const int s_begin = split * (a.ksplit / C::SRC_STAGE);
+        const int s_end = min(s_begin + a.ksplit / C::SRC_STAGE, kst);
         if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
         const int n_st = s_end - s_begin;
         const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
@@ -1477,15 +1478,15 @@ __global__ __launch_bounds__(512) void herk_fused8_kernel(FusedArgs a) {
     const int64_t nitems = (int64_t)a.nunits * a.nsplit * a.ntiles;
     const size_t amp_stage = (size_t)a.nant * 16;
     const size_t amp_time = (size_t)a.kpad * a.nant;
-    constexpr int STAGES_SPLIT = KSPLIT / 16;
+    const int st_split = a.ksplit / 16;   // stages per K-split
 
     for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
         int unit, split, tile, bi, bj;
         decode_work(a.nsplit, a.ntiles, Lw, unit, split, tile);
         decode_tile(tile, a.nb, bi, bj);
         const bool diag = (bi == bj);
-        const int s_begin = split * STAGES_SPLIT;
-        const int s_end = min(s_begin + STAGES_SPLIT, kst);
+        const int s_begin = split * st_split;
+        const int s_end = min(s_begin + st_split, kst);
         if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
         const int n_st = s_end - s_begin;
         const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
@@ -2474,7 +2475,7 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
     const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / 16);
     const size_t amp_stage = (size_t)a.nant * 16;
     const size_t amp_time = (size_t)a.kpad * a.nant;
-    constexpr int STAGES_SPLIT = KSPLIT / 16;
+    const int st_split = a.ksplit / 16;   // stages per K-split
     const int ug = t192_unit_group(a.nunits, gridDim.x);
     const bool pairs = t192_pairs(a.nb, a.nant);
     constexpr bool tri = NBKT > 0;   // work tile 0: triangle, 1: strip
@@ -2498,8 +2499,8 @@ __global__ __launch_bounds__(512) void herk_t192_kernel(FusedArgs a) {
             decode_tile(tile, a.nb, bi, bj);
         }
         const bool diag = (bi == bj);
-        const int s_begin = split * STAGES_SPLIT;
-        const int s_end = min(s_begin + STAGES_SPLIT, kst);
+        const int s_begin = split * st_split;
+        const int s_end = min(s_begin + st_split, kst);
         if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
         const int n_st = s_end - s_begin;
         const int tb = unit / a.nfreq, fl = unit - tb * a.nfreq;
@@ -2714,7 +2715,7 @@ __global__ __launch_bounds__(512) void herk_mu_kernel(FusedArgs a) {
     const int kst = (int)(eff_kpad(a.kpad, a.dyn_kpad) / 16);
     const size_t amp_stage = (size_t)a.nant * 16;
     const size_t amp_time = (size_t)a.kpad * a.nant;
-    constexpr int STAGES_SPLIT = KSPLIT / 16;
+    const int st_split = a.ksplit / 16;   // stages per K-split
     const int nt = a.nunits / a.nfreq;                   // times of the launch
     const int gpt = (a.nfreq + U - 1) / U;               // unit groups per time
     const int ngroups = nt * gpt;
@@ -2733,8 +2734,8 @@ __global__ __launch_bounds__(512) void herk_mu_kernel(FusedArgs a) {
     for (int64_t Lw = blockIdx.x; Lw < nitems; Lw += gridDim.x) {
         int grp, split, tile;
         if (!t192_decode(ngroups, a.nsplit, 1, ug, Lw, grp, split, tile)) continue;
-        const int s_begin = split * STAGES_SPLIT;
-        const int s_end = min(s_begin + STAGES_SPLIT, kst);
+        const int s_begin = split * st_split;
+        const int s_end = min(s_begin + st_split, kst);
         if (s_begin >= s_end) continue;   // split beyond the job's sources (uniform per workgroup)
         const int n_st = s_end - s_begin;
         const int tb = grp / gpt, f0 = (grp - tb * gpt) * U;
@@ -2918,6 +2919,7 @@ static hipError_t launch_fused_t(const FusedArgs& a, hipStream_t s) {
 
 hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s) {
     if ((int64_t)a.nunits * a.nsplit * a.ntiles <= 0) return hipSuccess;
+    if (!ksplit_valid(a.ksplit)) return hipErrorInvalidValue;
     hipError_t err = hipSuccess;
     if (mode == MODE_SPLIT_BF16 && launch_mu<MODE_SPLIT_BF16>(tile, a, s, err)) return err;
     if (mode == MODE_SPLIT_FP16 && launch_mu<MODE_SPLIT_FP16>(tile, a, s, err)) return err;
@@ -2941,6 +2943,7 @@ static hipError_t launch_herk_t(const HerkArgs& a, hipStream_t s) {
 // Kernel variants: tile edge (64 / 128) x prefetch depth (1 / 2 stages).
 // variant 0 selects the default for the mode and antenna count.
 hipError_t launch_herk_variant(int mode, int tile, int pf, const HerkArgs& a, hipStream_t s) {
+    if (!ksplit_valid(a.ksplit)) return hipErrorInvalidValue;
 #define HV_V(M, T, P) if (mode == M && tile == T && pf == P) return launch_herk_t<M, T, P>(a, s)
     HV_V(MODE_FP32, 64, 1); HV_V(MODE_FP32, 64, 2);
     HV_V(MODE_SPLIT_BF16, 64, 1); HV_V(MODE_SPLIT_BF16, 64, 2);
@@ -3002,6 +3005,27 @@ hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {
 
 bool herk_fused(int mode, int nant) {
     return split_mode(mode) && herk_prefetch(mode, herk_tile(mode, nant)) == 5;
+}
+
+int job_ksplit(int mode, int nant, int64_t units, int64_t kpad) {
+    // work items per (unit, split) of the built-in kernels, times 12: a
+    // multi-unit item holds 6, 4 or 3 units at 2, 3 or 4 blocks of 32 rows;
+    // 5..8 blocks run one triangle item, 9..11 a triangle and a strip item;
+    // otherwise one item per tile (192-tiles for the split modes beyond 352
+    // antennas, 64-tiles else).  Variant overrides do not enter.
+    const int nbk = (nant + 31) / 32;
+    int64_t ipu12;
+    if (split_mode(mode) && nbk >= 2 && nbk <= 4) {
+        ipu12 = nbk == 2 ? 2 : nbk == 3 ? 3 : 4;
+    } else if (split_mode(mode) && nbk >= 5 && nbk <= 11) {
+        ipu12 = nbk <= 8 ? 12 : 24;
+    } else {
+        const int tile = (split_mode(mode) && nbk > 11) ? 192 : 64;
+        const int64_t nb = (nant + tile - 1) / tile;
+        ipu12 = 12 * (nb * (nb + 1) / 2);
+    }
+    const int64_t nlong = (kpad + KSPLIT_LONG - 1) / KSPLIT_LONG;
+    return units * ipu12 * nlong >= 12 * KSPLIT_LONG_ITEMS ? KSPLIT_LONG : KSPLIT;
 }
 
 }  // namespace hv

@@ -110,6 +110,7 @@ struct Plan {
     int mode = 0, nant = 0, ntimes = 0, nfreq = 0;
     int64_t kpad = 0;              // padded sources (buffers and grids)
     int tile = 0, nb = 0, ntiles = 0, nsplit = 0;
+    int ksplit = 0;                // sources per K-split (job_ksplit of the whole job)
     size_t row_bytes = 0, unit_bytes = 0, part_unit = 0;
     int fb = 1, tb = 1;            // frequencies / times per batch
     bool fused = false;            // operands built inside the contraction (B_W holds A sqrt(I))
@@ -122,8 +123,11 @@ struct Plan {
 
 // operands: build the operands in HBM (phasor build + LDS-ring contraction)
 // even where the built-in choice builds them inside the contraction.
+// ksplit: sources per K-split, job_ksplit of the whole job (0: of this
+// call's shape); passed where the plan covers a part of a job's times or a
+// serving bucket's padded size.
 Plan plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t operand_budget, int layout = 0,
-              bool beam_per_freq = false, bool operands = false);
+              bool beam_per_freq = false, bool operands = false, int ksplit = 0);
 
 struct JobDev {
     int mode;

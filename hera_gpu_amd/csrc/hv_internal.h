@@ -10,6 +10,7 @@
 //   crd_top    double[T][3][Kpad]      rotated source directions
 //   W          operand[units][stage][N][128 B]  phasors, stage-major (below)
 //   partials   acc[units][nsplit][ntiles][2][TILE][TILE]   (Re, Im) per split
+//              of job_ksplit(...) sources
 //   vis        complex[F][T][N][N]
 //
 // Phasor operands are stage-major: the sources are cut into stages of
@@ -29,8 +30,21 @@
 namespace hv {
 
 constexpr int KALIGN = 64;          // Kpad multiple (sources)
-constexpr int KSPLIT = 8192;        // sources per K-split (fixed: results do not
-                                    // depend on padding or batching)
+constexpr int KSPLIT = 8192;        // sources per K-split (short splits)
+constexpr int KSPLIT_LONG = 32768;  // sources per K-split of large jobs
+constexpr int64_t KSPLIT_LONG_ITEMS = 2048;
+// Sources per K-split of a job of `units` (time, frequency) units over kpad
+// sources.  A split is one FP32 accumulator chain: the contraction writes
+// one partial tile set per split and the reduce sums the splits in FP64, in
+// split order.  The split length is a function of the job alone (arithmetic,
+// antennas, units, padded sources), never of how its units are batched into
+// launches, of variant overrides or of the serving buckets, so a job's
+// results are the same bits however it runs.  A job that still has
+// KSPLIT_LONG_ITEMS contraction work items (8 per CU of a 256-CU device)
+// with 32768-source splits takes them: a quarter of the partial-tile writes
+// and reduce reads.  Smaller jobs keep 8192-source splits for parallelism.
+int job_ksplit(int mode, int nant, int64_t units, int64_t kpad);
+inline bool ksplit_valid(int ksplit) { return ksplit == KSPLIT || ksplit == KSPLIT_LONG; }
 constexpr int FOLD_SRC = 512;       // sources per FP32 accumulator chain
 constexpr double C_LIGHT = 299792458.0;
 
@@ -103,6 +117,7 @@ struct HerkArgs {
     int ntiles;              // nb (nb+1) / 2
     int64_t kpad;
     int nsplit;
+    int ksplit;              // sources per split (job_ksplit)
     int nunits;
     const int64_t* dyn_kpad;
 };
@@ -148,10 +163,11 @@ struct ReduceArgs {
     int nb;
     int ntiles;
     int nsplit;
+    int ksplit;              // sources per split (job_ksplit)
     int nunits;
     int tile;
     double scale;            // 1 / amp_scale^2
-    const int64_t* dyn_kpad; // splits summed = ceil(*dyn_kpad / KSPLIT)
+    const int64_t* dyn_kpad; // splits summed = ceil(*dyn_kpad / ksplit)
     int layout;              // Layout
     int nfreq_total;         // F of the output array
 };
@@ -205,6 +221,7 @@ struct FusedArgs {
     void* partials;           // as HerkArgs::partials
     int64_t kpad;
     int nant, nb, ntiles, nsplit, nunits, nfreq;
+    int ksplit;               // sources per split (job_ksplit)
     const int64_t* dyn_kpad;
     int amp_per_unit;         // amp holds one block per unit (frequency-dependent beams)
     int lists;                // 192-tile block lists: 0 built-in, 1 tile pairs only (no triangle/strip)

@@ -9,8 +9,8 @@
 // partial tiles and 1/lambda live in `scratch`, a file-scope device array of
 // HV_NOALLOC_SCRATCH_BYTES per GPU, carved per call.  Times are processed in
 // batches small enough for the scratch; results do not depend on batching
-// (fixed 8192-source splits summed in split order), so they equal hv_vis's
-// bit for bit.  Every byte a kernel reads from the scratch is written earlier
+// (K-splits fixed by the whole job, summed in split order), so they equal
+// hv_vis's bit for bit.  Every byte a kernel reads from the scratch is written earlier
 // in the same call, so nothing in it has to start at zero.  All calls on one
 // GPU share its array: calls that overlap on one GPU race.
 #include <mutex>
@@ -81,12 +81,13 @@ extern "C" int hv_vis_device_noalloc(hv_ctx* c, int mode, const double* antpos, 
     if (il_bytes >= SCRATCH_BYTES) return fail(HV_ERR_NOMEM, "too many frequencies for the scratch");
     const size_t avail = SCRATCH_BYTES - il_bytes;
     int nt = ntimes;
+    const int ksplit = job_ksplit(mode, nant, (int64_t)ntimes * nfreq, kpad);
     Plan p;
     size_t need = 0;
     for (;;) {
         const size_t crd = (size_t)nt * 3 * kpad * 8;
         const size_t wb = avail > crd ? avail - crd : 0;
-        p = plan_job(mode, nant, kpad, nt, nfreq, wb);
+        p = plan_job(mode, nant, kpad, nt, nfreq, wb, LAYOUT_FULL, false, false, ksplit);
         need = align256(p.crd_top_bytes) + align256(p.w_bytes) + align256(p.part_bytes);
         if (need <= avail) break;
         if (nt == 1)

@@ -317,7 +317,7 @@ static size_t operand_budget(hv_ctx* c) {
 }
 
 Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_t wbudget, int layout,
-                  bool beam_per_freq, bool operands) {
+                  bool beam_per_freq, bool operands, int ksplit) {
     Plan p;
     p.mode = mode;
     p.nant = nant;
@@ -328,7 +328,8 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.tile = p.fused ? herk_tile(mode, nant) : herk_tile_operands(mode, nant);
     p.nb = (nant + p.tile - 1) / p.tile;
     p.ntiles = p.nb * (p.nb + 1) / 2;
-    p.nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
+    p.ksplit = ksplit ? ksplit : job_ksplit(mode, nant, (int64_t)ntimes * nfreq, kpad);
+    p.nsplit = (int)((kpad + p.ksplit - 1) / p.ksplit);
     p.row_bytes = operand_row_bytes(mode, kpad);
     p.unit_bytes = (size_t)nant * p.row_bytes;
     const size_t acc_bytes = mode_is_f64(mode) ? 8 : 4;
@@ -410,6 +411,7 @@ static int enqueue_contract(const JobDev& j, const Plan& p, const void* W, void*
     ha.ntiles = p.ntiles;
     ha.kpad = p.kpad;
     ha.nsplit = p.nsplit;
+    ha.ksplit = p.ksplit;
     ha.nunits = nt * nf;
     ha.dyn_kpad = j.dyn_kpad;
     HV_CHECK(tm.begin(2, stream));
@@ -431,6 +433,7 @@ static int enqueue_reduce(const JobDev& j, const Plan& p, void* part, int t0, in
     ra.nb = p.nb;
     ra.ntiles = p.ntiles;
     ra.nsplit = p.nsplit;
+    ra.ksplit = p.ksplit;
     ra.nunits = nt * nf;
     ra.tile = p.tile;
     ra.scale = 1.0 / ((double)j.amp_scale * (double)j.amp_scale);
@@ -480,6 +483,7 @@ static int enqueue_fused(const JobDev& j, const Plan& p, Workspace& ws, int t0, 
     fa.nb = p.nb;
     fa.ntiles = p.ntiles;
     fa.nsplit = p.nsplit;
+    fa.ksplit = p.ksplit;
     fa.nunits = nt * nf;
     fa.nfreq = nf;
     fa.dyn_kpad = j.dyn_kpad;
@@ -1111,7 +1115,8 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     const int tile = herk_tile_operands(mode, nant);
     const int nb = (nant + tile - 1) / tile;
     const int ntiles = nb * (nb + 1) / 2;
-    const int nsplit = (int)((kpad + KSPLIT - 1) / KSPLIT);
+    const int ksplit = job_ksplit(mode, nant, 1, kpad);
+    const int nsplit = (int)((kpad + ksplit - 1) / ksplit);
     HV_CHECK(ensure(c, B_PART, (size_t)nsplit * ntiles * 2 * tile * tile * (f64 ? 8 : 4)));
     HV_CHECK(ensure(c, B_OUT, (size_t)nant * nant * cplx));
     HerkArgs ha{};
@@ -1123,6 +1128,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     ha.ntiles = ntiles;
     ha.kpad = kpad;
     ha.nsplit = nsplit;
+    ha.ksplit = ksplit;
     ha.nunits = 1;
     HV_HIP(launch_herk(mode, tile, ha, c->stream));
     ReduceArgs ra{};
@@ -1136,6 +1142,7 @@ extern "C" int hv_k_herk(hv_ctx* c, int mode, const void* v, int nant, int64_t n
     ra.nb = nb;
     ra.ntiles = ntiles;
     ra.nsplit = nsplit;
+    ra.ksplit = ksplit;
     ra.nunits = 1;
     ra.tile = tile;
     ra.scale = 1.0 / ((double)scale * scale);

@@ -59,10 +59,12 @@ struct BucketKey {
     int64_t kbucket = 0;
     int layout = 0;
     int beam_per_freq = 0;
+    int ksplit = 0;            // sources per K-split (fixed by the job's exact size)
     bool operator==(const BucketKey& o) const {
-        return std::tie(mode, nant, beam_px, ntimes, nfreq, beam_f64, scale, kbucket, layout, beam_per_freq) ==
+        return std::tie(mode, nant, beam_px, ntimes, nfreq, beam_f64, scale, kbucket, layout, beam_per_freq,
+                        ksplit) ==
                std::tie(o.mode, o.nant, o.beam_px, o.ntimes, o.nfreq, o.beam_f64, o.scale, o.kbucket, o.layout,
-                        o.beam_per_freq);
+                        o.beam_per_freq, o.ksplit);
     }
 };
 

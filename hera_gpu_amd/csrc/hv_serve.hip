@@ -413,12 +413,15 @@ extern "C" int hv_server_submit(hv_server* s, const hv_job* job, int64_t* id) {
     while (kb < hj->kpad) kb <<= 1;
     const float scale = (hj->mode == HV_MODE_SPLIT_FP16) ? fp16_scale(beam->max_abs * sky->max_isqrt) : 1.0f;
     hj->layout = (job->flags & HV_JOB_TBF) ? LAYOUT_TBF : (job->flags & HV_JOB_FTB) ? LAYOUT_FTB : LAYOUT_FULL;
+    // the K-split length follows the job's exact size (as a direct call's),
+    // not the bucket's padded one; it is part of the bucket key
+    const int ksplit = job_ksplit(hj->mode, hj->nant, (int64_t)hj->ntimes * hj->nfreq, hj->kpad);
     hj->plan = plan_job(hj->mode, hj->nant, kb, hj->ntimes, hj->nfreq, SERVE_OPERAND_BUDGET, hj->layout,
-                        beam->nfreq > 1);
+                        beam->nfreq > 1, false, ksplit);
     auto j = std::make_unique<sched::Job>();
     j->flags = job->flags;
     j->key = sched::BucketKey{hj->mode, hj->nant, beam->beam_px, hj->ntimes, hj->nfreq, beam->f64, scale, kb,
-                              hj->layout, beam->nfreq > 1 ? 1 : 0};
+                              hj->layout, beam->nfreq > 1 ? 1 : 0, ksplit};
     j->footprint = instance_bytes(*hj);
     j->out_bytes = hj->plan.out_bytes;
     j->out = job->vis_out;

@@ -632,7 +632,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceArgs r) {
     const size_t off = (size_t)(row % T) * T + (col % T);
     const ACC* p = reinterpret_cast<const ACC*>(r.partials) + ((size_t)u * r.nsplit * r.ntiles + tid) * 2 * tsz + off;
     const size_t split_stride = (size_t)r.ntiles * 2 * tsz;
-    const int nsplit = (int)((eff_kpad((int64_t)r.nsplit * KSPLIT, r.dyn_kpad) + KSPLIT - 1) / KSPLIT);
+    const int nsplit = (int)((eff_kpad((int64_t)r.nsplit * r.ksplit, r.dyn_kpad) + r.ksplit - 1) / r.ksplit);
     double re = 0.0, im = 0.0;
     for (int s = 0; s < nsplit; ++s) {
         re += (double)p[(size_t)s * split_stride];
@@ -687,7 +687,7 @@ __global__ __launch_bounds__(256) void reduce_sym_kernel(ReduceArgs r) {
     const size_t tsz = (size_t)T * T;
     const ACC* base = reinterpret_cast<const ACC*>(r.partials) + ((size_t)u * r.nsplit * r.ntiles + tid) * 2 * tsz;
     const size_t split_stride = (size_t)r.ntiles * 2 * tsz;
-    const int nsplit = (int)((eff_kpad((int64_t)r.nsplit * KSPLIT, r.dyn_kpad) + KSPLIT - 1) / KSPLIT);
+    const int nsplit = (int)((eff_kpad((int64_t)r.nsplit * r.ksplit, r.dyn_kpad) + r.ksplit - 1) / r.ksplit);
     const int64_t t = r.t0 + u / r.nfreq_launch, f = r.f0 + u % r.nfreq_launch;
     OUT* vis = reinterpret_cast<OUT*>(r.vis);
     const int b = j * 32 + tx;
@@ -728,6 +728,7 @@ __global__ __launch_bounds__(256) void reduce_sym_kernel(ReduceArgs r) {
 }
 
 hipError_t launch_reduce(int mode, const ReduceArgs& r, hipStream_t s) {
+    if (!ksplit_valid(r.ksplit)) return hipErrorInvalidValue;
     if (r.layout == LAYOUT_FULL || r.layout == LAYOUT_POL) {
         const int n32 = (r.nant + 31) / 32;
         dim3 grid((unsigned)(n32 * (n32 + 1) / 2), (unsigned)r.nunits);

@@ -184,6 +184,14 @@ def herk_choice(gemm: str, nant: int) -> tuple:
     return t.value, p.value
 
 
+def ksplit_choice(gemm: str, nant: int, units: int, nsrc: int) -> int:
+    """Sources per K-split (8192 or 32768) of a job of `units` (time,
+    frequency) units over nsrc sources: fixed by the job alone."""
+    k = C.c_int()
+    _lib.check(_lib.load().hv_job_ksplit(_mode(gemm), nant, units, nsrc, C.byref(k)), "hv_job_ksplit")
+    return k.value
+
+
 def bench_copy(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> dict:
     """Plain device copy of nbytes: ms per copy and GB/s (read + write)."""
     ms = C.c_double()

@@ -304,6 +304,12 @@ int hv_set_herk_variant(int tile, int pf);
 /* The contraction variant (tile edge, pf as above) that calls of this mode and
  * antenna count use now. */
 int hv_herk_choice(int mode, int nant, int* tile, int* pf);
+/* Sources per K-split (8192 or 32768) of a job of `units` (time, frequency)
+ * units over nsrc sources.  Each split is one FP32 accumulator chain whose
+ * partial tiles the reduce sums in FP64; the length is fixed by the job
+ * alone (never by batching, variants or serving buckets): 32768 where the
+ * job still has 2048 contraction work items with it, 8192 else. */
+int hv_job_ksplit(int mode, int nant, int64_t units, int64_t nsrc, int* ksplit);
 
 /* ---------------------------------------------------------------------------
  * Serving (no reference counterpart: the reference has no request loop; it
