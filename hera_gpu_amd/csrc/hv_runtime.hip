@@ -347,8 +347,13 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     if (p.fb == nfreq)
         p.tb = (int)std::max<size_t>(
             1, std::min<size_t>((size_t)ntimes, wbudget / std::max<size_t>(per_unit * nfreq + per_time, 1)));
-    // keep one contraction launch within ~64k work items
-    while (p.tb > 1 && (int64_t)p.tb * p.fb * p.nsplit * p.ntiles > 65536) --p.tb;
+    // keep one contraction launch within ~64k work items and 2^44 (unit,
+    // source, antenna pair) products, 128 units of 2^20 sources at 350
+    // antennas (HERA-350 with 32768-source splits: 2 times per launch
+    // 2580-2586 ms per step, 7 times 2597-2604)
+    while (p.tb > 1 && ((int64_t)p.tb * p.fb * p.nsplit * p.ntiles > 65536 ||
+                        (double)p.tb * p.fb * (double)kpad * nant * nant > 17592186044416.0))
+        --p.tb;
     const int units = p.fb * p.tb;
     p.crd_top_bytes = (size_t)ntimes * 3 * kpad * sizeof(double);
     p.w_bytes = p.fused ? p.amp_time * (p.amp_unit ? units : p.tb) : p.unit_bytes * units;
