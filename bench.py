@@ -14,6 +14,7 @@ is printed on stdout; progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -34,8 +35,8 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {
     # in-place build (herk_t192_kernel, triangle and strip items), two times
     # (128 units) per launch, 32768-source K-splits (a job of 2048 units)
-    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5076019, "write_kib": 2166272, "fetch_scale": 2.0,
-                                          "write_scale": 1.0, "src": "d1cc34d77f53ff4e"},
+    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5076017, "write_kib": 2166272, "fetch_scale": 2.0,
+                                          "write_scale": 1.0, "src": "4454a9e269760e68"},
 }
 KERNEL_SOURCES = ("hera_gpu_amd/csrc/hv_herk.hip", "hera_gpu_amd/csrc/hv_phasor_math.h",
                   "hera_gpu_amd/csrc/hv_internal.h")
@@ -83,6 +84,23 @@ def t192_mfma_blocks(N):
             five = N - bj * 192 <= 160
             total += (15 if five else 21) if bi == bj else (30 if five else 36)
     return total
+
+
+@contextlib.contextmanager
+def job_splits(gemm, N, units, K):
+    """Run a part of the workload (a sample of its times, one unit) with the
+    K-split length of the whole job, so its arithmetic is the headline's:
+    a part of a job that takes 32768-source splits would take 8192 alone."""
+    from hera_gpu_amd import _lib, stages
+    lib = _lib.load()
+    long = stages.ksplit_choice(gemm, N, units, K) == 32768
+    if long:
+        _lib.check(lib.hv_set_ksplit_items(1), "hv_set_ksplit_items")
+    try:
+        yield long
+    finally:
+        if long:
+            _lib.check(lib.hv_set_ksplit_items(0), "hv_set_ksplit_items")
 
 
 def rel_err(a, b):
@@ -466,8 +484,9 @@ def main():
         # the production batching of HERA-350) through the production kernels
         cp = stages.bench_copy(1 << 30, 3)
         nt = min(2, T)
-        _, st = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[:nt], gemm=args.gemm, timing=True,
-                                want_output=False, dtype=np.float32)
+        with job_splits(args.gemm, N, T * F, K):
+            _, st = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[:nt], gemm=args.gemm, timing=True,
+                                    want_output=False, dtype=np.float32)
         u_launch = nt * F / st["herk_n"]
         print(json.dumps({"copy": cp, "gemm": args.gemm, "units": nt * F, "herk_launches": st["herk_n"],
                           "units_per_launch": u_launch, "herk_avg_ms": st["herk_ms"] / st["herk_n"],
@@ -540,7 +559,8 @@ def main():
                               "baselines",
                     "reference": "FP64 GPU path (pinned to the FP64 oracle by tests)"}
         for gemm in ("split_bf16", "split_fp16", "fp32"):
-            v, _ = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[tsel], gemm=gemm, dtype=np.float32)
+            with job_splits(gemm, N, T * F, K):
+                v, _ = stages.simulate(sky, beam, w.antpos, w.freqs, w.eq2tops[tsel], gemm=gemm, dtype=np.float32)
             per_f = [rel_err(v[i], ref[i]) for i in range(F)]
             accuracy[gemm] = {"max_rel_err": max(per_f), "per_freq": per_f}
             log(f"[bench] rel err {gemm}: max {max(per_f):.2e} over {F} frequencies")
@@ -555,7 +575,9 @@ def main():
         sub = stages.Sky(w.crd_eq[:, :nsrc], w.I_sky[:nsrc])
         gpu_unit = {}
         for gemm in ("split_bf16", "fp32"):
-            v, _ = stages.simulate(sub, beam, w.antpos, w.freqs[-1:], w.eq2tops[:1], gemm=gemm, dtype=np.float32)
+            with job_splits(gemm, N, T * F, K):
+                v, _ = stages.simulate(sub, beam, w.antpos, w.freqs[-1:], w.eq2tops[:1], gemm=gemm,
+                                       dtype=np.float32)
             gpu_unit[gemm] = v[0, 0]
         sub.close()
         log(f"[bench] cpu baseline: 1 unit, {nsrc} sources, {workers} processes")

@@ -105,6 +105,7 @@ SIGNATURES = {
     "hv_set_herk_variant": (_i, [_i, _i]),
     "hv_herk_choice": (_i, [_i, _i, C.POINTER(_i), C.POINTER(_i)]),
     "hv_job_ksplit": (_i, [_i, _i, C.c_int64, C.c_int64, C.POINTER(_i)]),
+    "hv_set_ksplit_items": (_i, [C.c_int64]),
     "hv_server_create": (_i, [_vp, _i, _sz, _i, C.POINTER(_vp)]),
     "hv_server_submit": (_i, [_vp, C.POINTER(Job), C.POINTER(_i64)]),
     "hv_server_poll": (_i, [_vp, C.POINTER(JobInfo), _i, C.POINTER(_i)]),

@@ -257,6 +257,12 @@ extern "C" int hv_job_ksplit(int mode, int nant, int64_t units, int64_t nsrc, in
     return HV_OK;
 }
 
+extern "C" int hv_set_ksplit_items(int64_t items) {
+    if (items < 0) return fail(HV_ERR_ARG, "items must be >= 0 (0 restores the built-in threshold)");
+    set_ksplit_items(items);
+    return HV_OK;
+}
+
 extern "C" int hv_herk_choice(int mode, int nant, int* tile, int* pf) {
     if (!tile || !pf || nant < 1) return fail(HV_ERR_ARG, "bad argument");
     if (mode < MODE_FP32 || mode > MODE_FP64) return fail(HV_ERR_ARG, "unknown contraction mode");

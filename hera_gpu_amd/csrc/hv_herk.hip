@@ -24,8 +24,8 @@
 //   FP64        v_mfma_f64_16x16x4_f64 on complex128 rows.
 // 64-tiles fold their FP32 accumulators into a second FP32 register total
 // every 512 sources (short MFMA accumulation chains); K is cut into fixed
-// splits of job_ksplit(...) sources (8192, or 32768 for jobs that still
-// hold 2048 work items with them) whose partial tiles are summed in FP64, in
+// splits of job_ksplit(...) sources (8192, or 32768 for split-mode jobs
+// that still hold 32768 work items with them) whose partial tiles are summed in FP64, in
 // split order, by the reduce stage, so results do not depend on how jobs are
 // padded or batched.
 //
@@ -3007,25 +3007,29 @@ bool herk_fused(int mode, int nant) {
     return split_mode(mode) && herk_prefetch(mode, herk_tile(mode, nant)) == 5;
 }
 
+static int64_t g_ksplit_items = KSPLIT_LONG_ITEMS;
+void set_ksplit_items(int64_t items) { g_ksplit_items = items > 0 ? items : KSPLIT_LONG_ITEMS; }
+
 int job_ksplit(int mode, int nant, int64_t units, int64_t kpad) {
+    if (!split_mode(mode)) return KSPLIT;
     // work items per (unit, split) of the built-in kernels, times 12: a
     // multi-unit item holds 6, 4 or 3 units at 2, 3 or 4 blocks of 32 rows;
     // 5..8 blocks run one triangle item, 9..11 a triangle and a strip item;
-    // otherwise one item per tile (192-tiles for the split modes beyond 352
-    // antennas, 64-tiles else).  Variant overrides do not enter.
+    // otherwise one item per tile (192-tiles beyond 352 antennas, 64-tiles
+    // up to 32).  Variant overrides do not enter.
     const int nbk = (nant + 31) / 32;
     int64_t ipu12;
-    if (split_mode(mode) && nbk >= 2 && nbk <= 4) {
+    if (nbk >= 2 && nbk <= 4) {
         ipu12 = nbk == 2 ? 2 : nbk == 3 ? 3 : 4;
-    } else if (split_mode(mode) && nbk >= 5 && nbk <= 11) {
+    } else if (nbk >= 5 && nbk <= 11) {
         ipu12 = nbk <= 8 ? 12 : 24;
     } else {
-        const int tile = (split_mode(mode) && nbk > 11) ? 192 : 64;
+        const int tile = nbk > 11 ? 192 : 64;
         const int64_t nb = (nant + tile - 1) / tile;
         ipu12 = 12 * (nb * (nb + 1) / 2);
     }
     const int64_t nlong = (kpad + KSPLIT_LONG - 1) / KSPLIT_LONG;
-    return units * ipu12 * nlong >= 12 * KSPLIT_LONG_ITEMS ? KSPLIT_LONG : KSPLIT;
+    return units * ipu12 * nlong >= 12 * g_ksplit_items ? KSPLIT_LONG : KSPLIT;
 }
 
 }  // namespace hv

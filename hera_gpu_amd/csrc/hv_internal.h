@@ -32,18 +32,23 @@ namespace hv {
 constexpr int KALIGN = 64;          // Kpad multiple (sources)
 constexpr int KSPLIT = 8192;        // sources per K-split (short splits)
 constexpr int KSPLIT_LONG = 32768;  // sources per K-split of large jobs
-constexpr int64_t KSPLIT_LONG_ITEMS = 2048;
+constexpr int64_t KSPLIT_LONG_ITEMS = 32768;
 // Sources per K-split of a job of `units` (time, frequency) units over kpad
 // sources.  A split is one FP32 accumulator chain: the contraction writes
 // one partial tile set per split and the reduce sums the splits in FP64, in
 // split order.  The split length is a function of the job alone (arithmetic,
 // antennas, units, padded sources), never of how its units are batched into
 // launches, of variant overrides or of the serving buckets, so a job's
-// results are the same bits however it runs.  A job that still has
-// KSPLIT_LONG_ITEMS contraction work items (8 per CU of a 256-CU device)
-// with 32768-source splits takes them: a quarter of the partial-tile writes
-// and reduce reads.  Smaller jobs keep 8192-source splits for parallelism.
+// results are the same bits however it runs.  A split-BF16/FP16 job that
+// still has KSPLIT_LONG_ITEMS contraction work items (128 per CU of a
+// 256-CU device) with 32768-source splits takes them: a quarter of the
+// partial-tile writes and reduce reads.  Smaller jobs, and FP32 and FP64
+// jobs (whose launches hold few units: their operands go through HBM), keep
+// 8192-source splits for parallelism.
 int job_ksplit(int mode, int nant, int64_t units, int64_t kpad);
+// Work-item threshold of job_ksplit (0: KSPLIT_LONG_ITEMS); process-wide,
+// for tests and tuning.
+void set_ksplit_items(int64_t items);
 inline bool ksplit_valid(int ksplit) { return ksplit == KSPLIT || ksplit == KSPLIT_LONG; }
 constexpr int FOLD_SRC = 512;       // sources per FP32 accumulator chain
 constexpr double C_LIGHT = 299792458.0;

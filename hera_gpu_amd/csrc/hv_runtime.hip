@@ -328,7 +328,9 @@ Plan hv::plan_job(int mode, int nant, int64_t kpad, int ntimes, int nfreq, size_
     p.tile = p.fused ? herk_tile(mode, nant) : herk_tile_operands(mode, nant);
     p.nb = (nant + p.tile - 1) / p.tile;
     p.ntiles = p.nb * (p.nb + 1) / 2;
-    p.ksplit = ksplit ? ksplit : job_ksplit(mode, nant, (int64_t)ntimes * nfreq, kpad);
+    // polarised jobs contract operands built in HBM (few units per launch):
+    // short splits
+    p.ksplit = ksplit ? ksplit : operands ? KSPLIT : job_ksplit(mode, nant, (int64_t)ntimes * nfreq, kpad);
     p.nsplit = (int)((kpad + p.ksplit - 1) / p.ksplit);
     p.row_bytes = operand_row_bytes(mode, kpad);
     p.unit_bytes = (size_t)nant * p.row_bytes;

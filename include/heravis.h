@@ -307,9 +307,13 @@ int hv_herk_choice(int mode, int nant, int* tile, int* pf);
 /* Sources per K-split (8192 or 32768) of a job of `units` (time, frequency)
  * units over nsrc sources.  Each split is one FP32 accumulator chain whose
  * partial tiles the reduce sums in FP64; the length is fixed by the job
- * alone (never by batching, variants or serving buckets): 32768 where the
- * job still has 2048 contraction work items with it, 8192 else. */
+ * alone (never by batching, variants or serving buckets): 32768 for a
+ * split-BF16/FP16 job that still has 32768 contraction work items with it,
+ * 8192 else (FP32, FP64 and polarised jobs always). */
 int hv_job_ksplit(int mode, int nant, int64_t units, int64_t nsrc, int* ksplit);
+/* Process-wide work-item threshold of that rule for later calls (tests and
+ * tuning; 0 restores 32768).  Not safe while other calls run. */
+int hv_set_ksplit_items(int64_t items);
 
 /* ---------------------------------------------------------------------------
  * Serving (no reference counterpart: the reference has no request loop; it

@@ -91,25 +91,40 @@ def test_contraction_tile_choice(nant, tile):
 
 @pytest.mark.parametrize("gemm,nant,units,nsrc,ksplit", [
     ("split_bf16", 350, 2048, 1 << 20, 32768),   # the HERA-350 job: 2 items per (unit, split)
-    ("split_bf16", 350, 128, 1 << 20, 32768),    # its launches of 2 times x 64 frequencies
-    ("split_bf16", 350, 32, 1 << 20, 32768),     # 32 x 32 splits x 2 = 2048 items: just enough
-    ("split_bf16", 350, 31, 1 << 20, 8192),      # 1984 items
+    ("split_bf16", 350, 512, 1 << 20, 32768),    # 512 x 32 splits x 2 = 32768 items: just enough
+    ("split_bf16", 350, 511, 1 << 20, 8192),
+    ("split_bf16", 350, 128, 1 << 20, 8192),     # two times of it, as a job of their own
     ("split_bf16", 350, 1, 1 << 20, 8192),       # a lone unit keeps 128 splits x 2 items
-    ("split_bf16", 350, 192, 164540, 32768),     # 6 long splits: 2304 items
-    ("split_bf16", 256, 192, 164540, 8192),      # one triangle item per (unit, split): 1152
-    ("split_bf16", 256, 342, 164540, 32768),     # 2052
-    ("split_bf16", 37, 2048, 164540, 32768),     # items of 6 units: 2048 / 6 x 6 splits
-    ("split_bf16", 37, 2047, 164540, 8192),
-    ("split_fp16", 128, 1024, 164540, 32768),    # items of 3 units
-    ("split_fp16", 128, 1023, 164540, 8192),
-    ("fp32", 128, 256, 66236, 32768),            # 64-tiles: 3 per unit, 3 long splits
-    ("fp64", 128, 227, 66236, 8192),
-    ("split_bf16", 520, 11, 1 << 20, 32768),     # 192-tiles: 6 per unit
-    ("split_bf16", 520, 10, 1 << 20, 8192),
-    ("split_bf16", 37, 100000, 0, 32768),        # no sources: one split either way
+    ("split_bf16", 256, 1024, 1 << 20, 32768),   # one triangle item per (unit, split)
+    ("split_bf16", 256, 1023, 1 << 20, 8192),
+    ("split_fp16", 37, 6144, 1 << 20, 32768),    # items of 6 units
+    ("split_fp16", 37, 6143, 1 << 20, 8192),
+    ("split_bf16", 128, 3072, 1 << 20, 32768),   # items of 3 units
+    ("split_bf16", 520, 171, 1 << 20, 32768),    # 192-tiles: 6 per unit
+    ("split_bf16", 520, 170, 1 << 20, 8192),
+    ("fp32", 350, 2048, 1 << 20, 8192),          # FP32 and FP64 always keep short splits
+    ("fp64", 128, 100000, 1 << 20, 8192),
+    ("split_bf16", 37, 10 ** 6, 0, 32768),       # no sources: one split either way
 ])
 def test_ksplit_rule(gemm, nant, units, nsrc, ksplit):
-    # host-side rule, no GPU call: 32768-source K-splits where the job still
-    # holds 2048 contraction work items with them, else 8192
+    # host-side rule, no GPU call: 32768-source K-splits where a split-mode
+    # job still holds 32768 contraction work items with them, else 8192
     from hera_gpu_amd import stages
     assert stages.ksplit_choice(gemm, nant, units, nsrc) == ksplit
+
+
+def test_ksplit_threshold_knob():
+    # the rule's work-item threshold is process-wide (tests and tuning); 0
+    # restores 32768
+    from hera_gpu_amd import stages
+    lib = _lib.load()
+    assert stages.ksplit_choice("split_bf16", 350, 32, 1 << 20) == 8192
+    try:
+        assert lib.hv_set_ksplit_items(2048) == 0
+        assert stages.ksplit_choice("split_bf16", 350, 32, 1 << 20) == 32768    # 2048 items
+        assert stages.ksplit_choice("split_bf16", 350, 31, 1 << 20) == 8192
+        assert stages.ksplit_choice("fp32", 350, 2048, 1 << 20) == 8192
+    finally:
+        assert lib.hv_set_ksplit_items(0) == 0
+    assert stages.ksplit_choice("split_bf16", 350, 32, 1 << 20) == 8192
+    assert lib.hv_set_ksplit_items(-1) != 0

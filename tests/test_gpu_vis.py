@@ -312,15 +312,28 @@ def test_multi_unit_items_equal_tile_kernels_bitwise(gemm, nant, nfreq):
 LONG_K = 5 * 32768 + 700   # six 32768-source splits, the last one short
 
 
+@pytest.fixture
+def long_splits():
+    # the K-split rule's work-item threshold lowered from 32768 to 2048, so
+    # jobs of test size take 32768-source splits as the large ones do
+    from hera_gpu_amd import _lib
+    lib = _lib.load()
+    _lib.check(lib.hv_set_ksplit_items(2048))
+    try:
+        yield
+    finally:
+        _lib.check(lib.hv_set_ksplit_items(0))
+
+
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
 @pytest.mark.parametrize("nant,ntimes,variants", [(37, 32, ((0, 0), (128, 4))), (128, 16, ((0, 0), (128, 4))),
                                                   (300, 3, ((192, 5), (192, 6))), (350, 3, ((192, 5), (192, 6)))])
-def test_long_splits_equal_across_kernels_bitwise(gemm, nant, ntimes, variants):
-    # a job that still holds 2048 contraction work items with 32768-source
-    # K-splits takes them (one FP32 chain per split and block): the built-in
-    # items (multi-unit items at N = 37, 128; triangle/strip items at N =
-    # 300, 350) equal the 128-tile LDS ring / the 192-tile pairs bit for bit,
-    # as with 8192-source splits, and match the oracle.  64 frequencies.
+def test_long_splits_equal_across_kernels_bitwise(gemm, nant, ntimes, variants, long_splits):
+    # a job with 32768-source K-splits (one FP32 chain per split and block):
+    # the built-in items (multi-unit items at N = 37, 128; triangle/strip
+    # items at N = 300, 350) equal the 128-tile LDS ring / the 192-tile
+    # pairs bit for bit, as with 8192-source splits, and match the oracle.
+    # 64 frequencies.
     from hera_gpu_amd import _lib
     lib = _lib.load()
     assert stages.ksplit_choice(gemm, nant, ntimes * 64, LONG_K) == 32768
@@ -338,23 +351,7 @@ def test_long_splits_equal_across_kernels_bitwise(gemm, nant, ntimes, variants):
     assert rel_err(outs[0][::63, -1:], ref) < VIS_TOL[gemm]
 
 
-@pytest.mark.parametrize("gemm", ["fp32", "fp64"])
-def test_long_splits_fp32_fp64_vs_oracle(gemm):
-    # the FP32 (64-tiles, totals folded every 512 sources) and FP64 paths on
-    # a job of 4 x 64 units over three 32768-source splits
-    nsrc = 2 * 32768 + 500
-    assert stages.ksplit_choice(gemm, 128, 256, nsrc) == 32768
-    w = configs.make_disk128(nsrc=nsrc, ntimes=4)
-    freqs = np.linspace(110e6, 190e6, 64)
-    p = 2 if gemm == "fp64" else 1
-    I = w.I_sky.astype(np.float64) if p == 2 else w.I_sky
-    bm = w.bm_cube.astype(np.float64) if p == 2 else w.bm_cube
-    v = vis_gpu(w.antpos, freqs, w.eq2tops, w.crd_eq, I, bm, precision=p, gemm=gemm)
-    ref = vis_oracle(w.antpos, freqs[::63], w.eq2tops[:1], w.crd_eq, effective_intensity(I, p), w.bm_cube)
-    assert rel_err(v[::63, :1], ref) < VIS_TOL[gemm]
-
-
-def test_long_split_job_batches_agree_bitwise():
+def test_long_split_job_batches_agree_bitwise(long_splits):
     # the split length follows the job, not the launch: a job of 16 x 64
     # units cut into one launch per time by a small HBM budget gives the
     # bits of the job run in one launch

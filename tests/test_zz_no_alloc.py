@@ -77,18 +77,23 @@ def test_noalloc_refuses_beyond_scratch_and_bad_args():
 def test_noalloc_long_split_job_in_time_batches():
     # 16 times x 64 frequencies of 128 antennas over 164544 sources need more
     # than the scratch in one batch, so the call runs one time per batch; the
-    # K-splits follow the whole job (32768 sources: 2048 work items), so the
-    # results equal hv_vis bit for bit
+    # K-splits follow the whole job (32768 sources: 2048 work items, with the
+    # rule's threshold lowered to that), so the results equal hv_vis bit for
+    # bit
     from hera_gpu_amd import stages
     lib = _lib.load()
     ctx = _lib.context(0)
     nsrc = 164544
-    assert stages.ksplit_choice("split_bf16", 128, 1024, nsrc) == 32768
-    w = configs.make_disk128(nsrc=nsrc, ntimes=16)
-    w.freqs = np.linspace(110e6, 190e6, 64)
-    mode = _lib.MODES["split_bf16"]
-    _noalloc(lib, ctx, mode, w, nsrc)
-    got, grew = _noalloc(lib, ctx, mode, w, nsrc)
-    ref = vis_gpu(w.antpos, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, gemm="split_bf16")
+    _lib.check(lib.hv_set_ksplit_items(2048))
+    try:
+        assert stages.ksplit_choice("split_bf16", 128, 1024, nsrc) == 32768
+        w = configs.make_disk128(nsrc=nsrc, ntimes=16)
+        w.freqs = np.linspace(110e6, 190e6, 64)
+        mode = _lib.MODES["split_bf16"]
+        _noalloc(lib, ctx, mode, w, nsrc)
+        got, grew = _noalloc(lib, ctx, mode, w, nsrc)
+        ref = vis_gpu(w.antpos, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, w.bm_cube, gemm="split_bf16")
+    finally:
+        _lib.check(lib.hv_set_ksplit_items(0))
     np.testing.assert_array_equal(got, ref)
     assert grew == 0
