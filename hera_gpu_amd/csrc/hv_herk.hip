@@ -2900,6 +2900,9 @@ template <int MODE>
 static bool launch_mu(int tile, const FusedArgs& a, hipStream_t s, hipError_t& err) {
     const int nbk = (a.nant + 31) / 32;
     if (a.amp_per_unit || a.lists == 1 || g_pf_override == 6 || nbk < 2 || nbk > 4) return false;
+    // an item computes all U unit slots: a job with fewer frequencies per
+    // time than U takes the tile kernels (one unit per item) instead
+    if ((a.job_nfreq > 0 ? a.job_nfreq : a.nfreq) < herk_mu_units(a.nant)) return false;
     if (tile != (nbk <= 2 ? 64 : 128) || a.kpad % 16) return false;
     err = nbk == 2 ? launch_mu_t<MODE, 2, 6>(a, s) : nbk == 3 ? launch_mu_t<MODE, 3, 4>(a, s)
                                                    : launch_mu_t<MODE, 4, 3>(a, s);
@@ -3005,6 +3008,11 @@ hipError_t launch_herk(int mode, int tile, const HerkArgs& a, hipStream_t s) {
 
 bool herk_fused(int mode, int nant) {
     return split_mode(mode) && herk_prefetch(mode, herk_tile(mode, nant)) == 5;
+}
+
+int herk_mu_units(int nant) {
+    const int nbk = (nant + 31) / 32;
+    return nbk == 2 ? 6 : nbk == 3 ? 4 : nbk == 4 ? 3 : 0;
 }
 
 static int64_t g_ksplit_items = KSPLIT_LONG_ITEMS;

@@ -230,11 +230,16 @@ struct FusedArgs {
     const int64_t* dyn_kpad;
     int amp_per_unit;         // amp holds one block per unit (frequency-dependent beams)
     int lists;                // 192-tile block lists: 0 built-in, 1 tile pairs only (no triangle/strip)
+    int job_nfreq;            // frequencies per time of the whole job (items of several units need
+                              // herk_mu_units of them; 0: the launch's nfreq)
 };
 
 hipError_t launch_amp(const AmpArgs& a, hipStream_t s);
 hipError_t launch_fused(int mode, int tile, const FusedArgs& a, hipStream_t s);
 bool herk_fused(int mode, int nant);   // does the built-in choice build operands in place?
+// Units per multi-unit work item at this antenna count (6, 4, 3 at 2, 3, 4
+// blocks of 32 rows), 0 where there are none.
+int herk_mu_units(int nant);
 
 int herk_tile(int mode, int nant);
 int herk_tile_operands(int mode, int nant);

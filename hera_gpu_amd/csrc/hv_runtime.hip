@@ -495,6 +495,7 @@ static int enqueue_fused(const JobDev& j, const Plan& p, Workspace& ws, int t0, 
     fa.nfreq = nf;
     fa.dyn_kpad = j.dyn_kpad;
     fa.amp_per_unit = p.amp_unit ? 1 : 0;
+    fa.job_nfreq = j.nfreq;
     HV_CHECK(tm.begin(2, stream));
     HV_HIP(launch_fused(j.mode, p.tile, fa, stream));
     HV_CHECK(tm.end(2, stream));

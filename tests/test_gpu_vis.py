@@ -281,8 +281,7 @@ def test_triangle_strip_items_equal_tile_pairs_bitwise(gemm, nant):
 
 
 @pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
-@pytest.mark.parametrize("nant,nfreq", [(33, 7), (37, 5), (37, 1), (64, 13), (65, 5), (96, 9), (97, 3), (128, 7),
-                                        (128, 1)])
+@pytest.mark.parametrize("nant,nfreq", [(33, 7), (37, 13), (64, 13), (65, 5), (96, 9), (97, 3), (128, 7), (128, 4)])
 def test_multi_unit_items_equal_tile_kernels_bitwise(gemm, nant, nfreq):
     # arrays of 33..128 antennas run items holding several frequency units
     # of one time (6, 4 or 3 units at 2, 3 or 4 blocks of 32 rows) that share
@@ -305,6 +304,33 @@ def test_multi_unit_items_equal_tile_kernels_bitwise(gemm, nant, nfreq):
     finally:
         _lib.check(lib.hv_set_herk_variant(0, 0))
     np.testing.assert_array_equal(outs[0], outs[1])
+    ref = vis_oracle(ap, w.freqs[-1:], w.eq2tops[-1:], w.crd_eq, w.I_sky, bm)[0, 0]
+    assert rel_err(outs[0][-1, -1], ref) < VIS_TOL[gemm]
+
+
+@pytest.mark.parametrize("gemm", ["split_bf16", "split_fp16"])
+@pytest.mark.parametrize("nant,nfreq,tile", [(37, 1, 64), (37, 5, 64), (64, 5, 64), (65, 3, 128), (100, 2, 128),
+                                             (128, 1, 128)])
+def test_few_frequency_jobs_take_tile_kernels_bitwise(gemm, nant, nfreq, tile):
+    # an item of several units computes all its 6, 4 or 3 unit slots, so a
+    # job with fewer frequencies per time than that runs one unit per item
+    # (the 64- or 128-tile in-place kernels): the bits of the LDS ring of
+    # that tile (pf 4), and at N <= 64 not those of the 128-tile chains the
+    # multi-unit items share (the 64-tiles fold every 512 sources)
+    from hera_gpu_amd import _lib
+    lib = _lib.load()
+    w = configs.make_hera350(nsrc=2 * 8192 + 700, ntimes=2, nfreq=nfreq)
+    ap, bm = w.antpos[:nant], w.bm_cube[:nant]
+    outs = []
+    try:
+        for t, pf in ((0, 0), (tile, 4), (128, 4)):
+            _lib.check(lib.hv_set_herk_variant(t, pf))
+            outs.append(vis_gpu(ap, w.freqs, w.eq2tops, w.crd_eq, w.I_sky, bm, gemm=gemm))
+    finally:
+        _lib.check(lib.hv_set_herk_variant(0, 0))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    if tile == 64:
+        assert not np.array_equal(outs[0], outs[2])
     ref = vis_oracle(ap, w.freqs[-1:], w.eq2tops[-1:], w.crd_eq, w.I_sky, bm)[0, 0]
     assert rel_err(outs[0][-1, -1], ref) < VIS_TOL[gemm]
 
