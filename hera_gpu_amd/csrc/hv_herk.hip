@@ -1970,16 +1970,33 @@ __device__ __forceinline__ void t192_pair_mma(const lds_char* As, const lds_char
 // Blocks (r, c), r in R0..R1 and c in C0..C1 (all with r <= c), one step,
 // into acc[K0 + (r - R0) * NC + c - C0]: B fragments held, A streamed (Im:
 // the A fragments rotated); bhi/blo keep the B fragments for the caller.
-template <int MODE, bool IM, int R0, int R1, int C0, int C1, int K0>
+// The first two fragments of a step read ahead (tri_pre): for a rectangle
+// its first A row and first B column, for a triangle its first two rows.
+struct TriPre {
+    u32x4 h1, l1, h2, l2;
+};
+
+template <int MODE, bool IM, int R0, int R1, int C0, int C1, int K0, bool PRE = false>
 __device__ __forceinline__ void tri_rect(const lds_char* S, int fh, int fl, u32x4 (&bhi)[C1 - C0 + 1],
-                                         u32x4 (&blo)[C1 - C0 + 1], f32x16 (&acc)[9]) {
+                                         u32x4 (&blo)[C1 - C0 + 1], f32x16 (&acc)[9],
+                                         const TriPre* pre = nullptr) {
     constexpr int NC = C1 - C0 + 1;
     constexpr int RA = IM ? 2 : 0;
     u32x4 ahi, alo;
-    t192_frag<RA>(S, R0, fh, fl, ahi, alo);
+    if constexpr (PRE) {
+        ahi = pre->h1;
+        alo = pre->l1;
+    } else {
+        t192_frag<RA>(S, R0, fh, fl, ahi, alo);
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        t192_frag<0>(S, C0 + c, fh, fl, bhi[c], blo[c]);
+        if (PRE && c == 0) {
+            bhi[0] = pre->h2;
+            blo[0] = pre->l2;
+        } else {
+            t192_frag<0>(S, C0 + c, fh, fl, bhi[c], blo[c]);
+        }
         t192_mac<MODE>(ahi, alo, bhi[c], blo[c], acc[K0 + c]);
     }
 #pragma unroll
@@ -1993,12 +2010,23 @@ __device__ __forceinline__ void tri_rect(const lds_char* S, int fh, int fl, u32x
 // Blocks (r, c), R0 <= r <= c <= R1 (a triangle on the diagonal), in row
 // order into acc[K0 ...], one step: each fragment read once, and rotated as
 // the A side of its own row for Im.
-template <int MODE, bool IM, int R0, int R1, int K0>
-__device__ __forceinline__ void tri_tri(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
+template <int MODE, bool IM, int R0, int R1, int K0, bool PRE = false>
+__device__ __forceinline__ void tri_tri(const lds_char* S, int fh, int fl, f32x16 (&acc)[9],
+                                        const TriPre* pre = nullptr) {
     constexpr int n = R1 - R0 + 1;
     u32x4 hi[n], lo[n];
 #pragma unroll
-    for (int i = 0; i < n; ++i) t192_frag<0>(S, R0 + i, fh, fl, hi[i], lo[i]);
+    for (int i = 0; i < n; ++i) {
+        if (PRE && i == 0) {
+            hi[0] = pre->h1;
+            lo[0] = pre->l1;
+        } else if (PRE && i == 1) {
+            hi[1] = pre->h2;
+            lo[1] = pre->l2;
+        } else {
+            t192_frag<0>(S, R0 + i, fh, fl, hi[i], lo[i]);
+        }
+    }
 #pragma unroll
     for (int r = 0; r < n; ++r) {
         const u32x4 ahi = IM ? t192_rot_a(hi[r]) : hi[r];
@@ -2024,25 +2052,43 @@ __device__ __forceinline__ void tri_held(const u32x4& ahi0, const u32x4& alo0, c
 //            q3: (3,3) (3,4) (4,4) + the triangle of rows 5-7;
 //   strip    q: rows 2q, 2q+1 x cols 8-10, then q0: (8,8) (8,9);
 //            q1: (8,10) (9,9); q2: (9,10); q3: (10,10).
+// The step's first two fragments of list Q, read ahead of the build (the
+// stage's operands are complete once the stage's barrier has passed).
 template <int MODE, bool IM, int Q>
-__device__ __forceinline__ void tri_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9]) {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const int fh = fo.hi[s], fl = fo.lo[s];
+__device__ __forceinline__ TriPre tri_pre(const lds_char* S, const T192Frag& fo) {
+    TriPre p;
+    const int fh = fo.hi[0], fl = fo.lo[0];
+    if constexpr (Q == 2 || Q == 3) {
+        constexpr int R0 = Q == 2 ? 0 : 3;
+        t192_frag<0>(S, R0, fh, fl, p.h1, p.l1);
+        t192_frag<0>(S, R0 + 1, fh, fl, p.h2, p.l2);
+    } else {
+        constexpr int R0 = Q < 2 ? 0 : 2 * (Q - 4);
+        constexpr int C0 = Q == 0 ? 5 : Q == 1 ? 2 : 8;
+        t192_frag<IM ? 2 : 0>(S, R0, fh, fl, p.h1, p.l1);
+        t192_frag<0>(S, C0, fh, fl, p.h2, p.l2);
+    }
+    return p;
+}
+
+template <int MODE, bool IM, int Q, bool P>
+__device__ __forceinline__ void tri_mma_step(const lds_char* S, int fh, int fl, f32x16 (&acc)[9],
+                                             const TriPre* pre) {
+    {
         u32x4 bhi[3], blo[3];
-        if constexpr (Q == 0) tri_rect<MODE, IM, 0, 2, 5, 7, 0>(S, fh, fl, bhi, blo, acc);
-        if constexpr (Q == 1) tri_rect<MODE, IM, 0, 2, 2, 4, 0>(S, fh, fl, bhi, blo, acc);
+        if constexpr (Q == 0) tri_rect<MODE, IM, 0, 2, 5, 7, 0, P>(S, fh, fl, bhi, blo, acc, pre);
+        if constexpr (Q == 1) tri_rect<MODE, IM, 0, 2, 2, 4, 0, P>(S, fh, fl, bhi, blo, acc, pre);
         if constexpr (Q == 2) {
-            tri_tri<MODE, IM, 0, 1, 0>(S, fh, fl, acc);
+            tri_tri<MODE, IM, 0, 1, 0, P>(S, fh, fl, acc, pre);
             tri_rect<MODE, IM, 3, 4, 5, 7, 3>(S, fh, fl, bhi, blo, acc);
         }
         if constexpr (Q == 3) {
-            tri_tri<MODE, IM, 3, 4, 0>(S, fh, fl, acc);
+            tri_tri<MODE, IM, 3, 4, 0, P>(S, fh, fl, acc, pre);
             tri_tri<MODE, IM, 5, 7, 3>(S, fh, fl, acc);
         }
         if constexpr (Q >= 4) {
             constexpr int q = Q - 4;
-            tri_rect<MODE, IM, 2 * q, 2 * q + 1, 8, 10, 0>(S, fh, fl, bhi, blo, acc);
+            tri_rect<MODE, IM, 2 * q, 2 * q + 1, 8, 10, 0, P>(S, fh, fl, bhi, blo, acc, pre);
             if constexpr (q == 0) {
                 tri_held<MODE, IM>(bhi[0], blo[0], bhi[0], blo[0], acc[6]);
                 tri_held<MODE, IM>(bhi[0], blo[0], bhi[1], blo[1], acc[7]);
@@ -2055,6 +2101,13 @@ __device__ __forceinline__ void tri_mma(const lds_char* S, const T192Frag& fo, f
             if constexpr (q == 3) tri_held<MODE, IM>(bhi[2], blo[2], bhi[2], blo[2], acc[6]);
         }
     }
+}
+
+template <int MODE, bool IM, int Q, bool PRE = false>
+__device__ __forceinline__ void tri_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9],
+                                        const TriPre* pre = nullptr) {
+    tri_mma_step<MODE, IM, Q, PRE>(S, fo.hi[0], fo.lo[0], acc, pre);
+    tri_mma_step<MODE, IM, Q, false>(S, fo.hi[1], fo.lo[1], acc, pre);
 }
 
 // Block k of list Q of tri_mma -> absolute 32-row block (br, bc), or false.
@@ -2368,7 +2421,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, fo, acc); \
         else if constexpr (PAIR) t192_pair_mma<MODE, IM, QP, BCP>(buf, buf + L::OP, fo, acc); \
         else if constexpr (RUNS) tri_runs_mma<MODE, IM, (RUNS ? NBK : 5), (STRIP ? 1 : 0), (DQ & 3)>(buf, fo, acc); \
-        else if constexpr (TRI) tri_mma<MODE, IM, (TRI ? QT : 0)>(buf, fo, acc);               \
+        else if constexpr (TRI) tri_mma<MODE, IM, (TRI ? QT : 0), true>(buf, fo, acc, &pre);   \
         else t192_quad_mma<MODE, IM>(buf + qa, buf + qb, fo, acc);                             \
     } while (0)
 
@@ -2390,6 +2443,15 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         // slot nobody refills, into the buffer no MFMA reads)
         const lds_char* bin = odd ? in0 : in1;
         lds_char* bop = odd ? op0 : op1;
+        // NBK = 11 lists: the first two fragments of this stage's MFMAs go
+        // out before the build, so the MFMAs start without waiting on LDS
+        // (measured on MI355X: 157.0 against 159.2 ms per 128-unit launch;
+        // reading the whole first group ahead spilled: 196 ms)
+        TriPre pre;
+        if constexpr (TRI && !RUNS) {
+            pre = tri_pre<MODE, IM, (TRI ? QT : 0)>(cur, fo);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         // both waves of a SIMD build, then both run their MFMAs (measured on
         // MI355X: 7% faster than staggering the build and the MFMAs of the
         // pair, whose LDS traffic then competes)
