@@ -35,8 +35,8 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {
     # in-place build (herk_t192_kernel, triangle and strip items), two times
     # (128 units) per launch, 32768-source K-splits (a job of 2048 units)
-    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5124977, "write_kib": 2173440, "fetch_scale": 2.0,
-                                          "write_scale": 1.0, "src": "c191481e206b48a0"},
+    ("hera350", "split_bf16", 128, 192): {"fetch_kib": 5124967, "write_kib": 2173440, "fetch_scale": 2.0,
+                                          "write_scale": 1.0, "src": "f025ed689749092f"},
 }
 KERNEL_SOURCES = ("hera_gpu_amd/csrc/hv_herk.hip", "hera_gpu_amd/csrc/hv_phasor_math.h",
                   "hera_gpu_amd/csrc/hv_internal.h")

@@ -1818,19 +1818,47 @@ __device__ __forceinline__ void t192_quad_mma(const lds_char* As, const lds_char
     }
 }
 
-// Diagonal tiles: block row R, columns C0..C1 into acc[K0 ...], one step.
-template <int MODE, bool IM, int R, int C0, int C1, int K0>
-__device__ __forceinline__ void t192_diag_row(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
+// The first two fragments of a stage's first step read ahead of the build
+// (tri_pre, runs_pre): for a rectangle or a row run its A row (rotated for
+// Im) and first B column, for a triangle its first two rows.
+struct TriPre {
+    u32x4 h1, l1, h2, l2;
+};
+
+// Diagonal tiles: block row R, columns C0..C1 into acc[K0 ...], one step
+// (PRE: its A fragment and first B fragment already read, in pre).
+template <int MODE, bool IM, int R, int C0, int C1, int K0, bool PRE = false>
+__device__ __forceinline__ void t192_diag_row(const lds_char* S, int fh, int fl, f32x16 (&acc)[9],
+                                              const TriPre* pre = nullptr) {
     // Im: the one A fragment of the row is rotated instead of its B fragments
     constexpr int RA = IM ? 2 : 0;
     u32x4 ahi, alo;
-    t192_frag<RA>(S, R, fh, fl, ahi, alo);
+    if constexpr (PRE) {
+        ahi = pre->h1;
+        alo = pre->l1;
+    } else {
+        t192_frag<RA>(S, R, fh, fl, ahi, alo);
+    }
 #pragma unroll
     for (int c = C0; c <= C1; ++c) {
         u32x4 bhi, blo;
-        t192_frag<0>(S, c, fh, fl, bhi, blo);
+        if (PRE && c == C0) {
+            bhi = pre->h2;
+            blo = pre->l2;
+        } else {
+            t192_frag<0>(S, c, fh, fl, bhi, blo);
+        }
         t192_mac<MODE>(ahi, alo, bhi, blo, acc[K0 + c - C0]);
     }
+}
+
+// The A row R and B column C of a run list's first run, step 0.
+template <bool IM>
+__device__ __forceinline__ TriPre runs_pre(const lds_char* S, const T192Frag& fo, int R, int C) {
+    TriPre p;
+    t192_frag<IM ? 2 : 0>(S, R, fo.hi[0], fo.lo[0], p.h1, p.l1);
+    t192_frag<0>(S, C, fo.hi[0], fo.lo[0], p.h2, p.l2);
+    return p;
 }
 
 template <int MODE, bool IM, int Q>
@@ -1970,12 +1998,6 @@ __device__ __forceinline__ void t192_pair_mma(const lds_char* As, const lds_char
 // Blocks (r, c), r in R0..R1 and c in C0..C1 (all with r <= c), one step,
 // into acc[K0 + (r - R0) * NC + c - C0]: B fragments held, A streamed (Im:
 // the A fragments rotated); bhi/blo keep the B fragments for the caller.
-// The first two fragments of a step read ahead (tri_pre): for a rectangle
-// its first A row and first B column, for a triangle its first two rows.
-struct TriPre {
-    u32x4 h1, l1, h2, l2;
-};
-
 template <int MODE, bool IM, int R0, int R1, int C0, int C1, int K0, bool PRE = false>
 __device__ __forceinline__ void tri_rect(const lds_char* S, int fh, int fl, u32x4 (&bhi)[C1 - C0 + 1],
                                          u32x4 (&blo)[C1 - C0 + 1], f32x16 (&acc)[9],
@@ -2209,21 +2231,23 @@ __host__ __device__ constexpr bool tri_run_block(int nbk, int part, int q, int k
     return false;
 }
 
-template <int MODE, bool IM, int NBK, int PART, int Q, int I, int K0>
-__device__ __forceinline__ void tri_runs_step(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
+template <int MODE, bool IM, int NBK, int PART, int Q, int I, int K0, bool PRE = false>
+__device__ __forceinline__ void tri_runs_step(const lds_char* S, int fh, int fl, f32x16 (&acc)[9],
+                                              const TriPre* pre = nullptr) {
     if constexpr (I < 3) {
         constexpr TRun run = tri_run(NBK, PART, Q, I);
         if constexpr (run.r >= 0) {
-            t192_diag_row<MODE, IM, run.r, run.c0, run.c1, K0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, run.r, run.c0, run.c1, K0, PRE && I == 0>(S, fh, fl, acc, pre);
             tri_runs_step<MODE, IM, NBK, PART, Q, I + 1, K0 + run.c1 - run.c0 + 1>(S, fh, fl, acc);
         }
     }
 }
-template <int MODE, bool IM, int NBK, int PART, int Q>
-__device__ __forceinline__ void tri_runs_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9]) {
+template <int MODE, bool IM, int NBK, int PART, int Q, bool PRE = false>
+__device__ __forceinline__ void tri_runs_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9],
+                                             const TriPre* pre = nullptr) {
     static_assert(tri_list_blocks(NBK, PART, Q) <= 9, "a SIMD list holds at most nine blocks");
-#pragma unroll
-    for (int s = 0; s < 2; ++s) tri_runs_step<MODE, IM, NBK, PART, Q, 0, 0>(S, fo.hi[s], fo.lo[s], acc);
+    tri_runs_step<MODE, IM, NBK, PART, Q, 0, 0, PRE>(S, fo.hi[0], fo.lo[0], acc, pre);
+    tri_runs_step<MODE, IM, NBK, PART, Q, 0, 0>(S, fo.hi[1], fo.lo[1], acc);
 }
 
 // Items of several units (arrays of 33..128 antennas, NBK = 2..4): the
@@ -2269,22 +2293,24 @@ __host__ __device__ constexpr bool mu_run_block(int nbk, int U, int q, int k, in
     }
     return false;
 }
-template <int MODE, bool IM, int NBK, int U, int Q, int I, int K0>
-__device__ __forceinline__ void mu_runs_step(const lds_char* S, int fh, int fl, f32x16 (&acc)[9]) {
+template <int MODE, bool IM, int NBK, int U, int Q, int I, int K0, bool PRE = false>
+__device__ __forceinline__ void mu_runs_step(const lds_char* S, int fh, int fl, f32x16 (&acc)[9],
+                                             const TriPre* pre = nullptr) {
     if constexpr (I < 3) {
         constexpr TRun run = mu_run(NBK, U, Q, I);
         if constexpr (run.r >= 0) {
-            t192_diag_row<MODE, IM, run.r, run.c0, run.c1, K0>(S, fh, fl, acc);
+            t192_diag_row<MODE, IM, run.r, run.c0, run.c1, K0, PRE && I == 0>(S, fh, fl, acc, pre);
             mu_runs_step<MODE, IM, NBK, U, Q, I + 1, K0 + run.c1 - run.c0 + 1>(S, fh, fl, acc);
         }
     }
 }
-template <int MODE, bool IM, int NBK, int U, int Q>
-__device__ __forceinline__ void mu_runs_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9]) {
+template <int MODE, bool IM, int NBK, int U, int Q, bool PRE = false>
+__device__ __forceinline__ void mu_runs_mma(const lds_char* S, const T192Frag& fo, f32x16 (&acc)[9],
+                                            const TriPre* pre = nullptr) {
     static_assert(mu_list_blocks(NBK, U, Q) <= 9, "a SIMD list holds at most nine blocks");
     static_assert(mu_run(NBK, U, Q, 3).r < 0, "a SIMD list holds at most three runs");
-#pragma unroll
-    for (int s = 0; s < 2; ++s) mu_runs_step<MODE, IM, NBK, U, Q, 0, 0>(S, fo.hi[s], fo.lo[s], acc);
+    mu_runs_step<MODE, IM, NBK, U, Q, 0, 0, PRE>(S, fo.hi[0], fo.lo[0], acc, pre);
+    mu_runs_step<MODE, IM, NBK, U, Q, 0, 0>(S, fo.hi[1], fo.lo[1], acc);
 }
 
 // Build rows of the run-list items: the item's 16-row groups (rows 0 ..
@@ -2420,7 +2446,7 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         if constexpr (DIAG) t192_diag_mma<MODE, IM, (DIAG ? QD : 0)>(buf, fo, acc);          \
         else if constexpr (RECT5) t192_rect5_mma<MODE, IM, (RECT5 ? QR : 0)>(buf, buf + L::OP, fo, acc); \
         else if constexpr (PAIR) t192_pair_mma<MODE, IM, QP, BCP>(buf, buf + L::OP, fo, acc); \
-        else if constexpr (RUNS) tri_runs_mma<MODE, IM, (RUNS ? NBK : 5), (STRIP ? 1 : 0), (DQ & 3)>(buf, fo, acc); \
+        else if constexpr (RUNS) tri_runs_mma<MODE, IM, (RUNS ? NBK : 5), (STRIP ? 1 : 0), (DQ & 3), (NBK <= 8)>(buf, fo, acc, &pre); \
         else if constexpr (TRI) tri_mma<MODE, IM, (TRI ? QT : 0), true>(buf, fo, acc, &pre);   \
         else t192_quad_mma<MODE, IM>(buf + qa, buf + qb, fo, acc);                             \
     } while (0)
@@ -2443,13 +2469,21 @@ __device__ __forceinline__ void t192_split(lds_char* in0, lds_char* in1, lds_cha
         // slot nobody refills, into the buffer no MFMA reads)
         const lds_char* bin = odd ? in0 : in1;
         lds_char* bop = odd ? op0 : op1;
-        // NBK = 11 lists: the first two fragments of this stage's MFMAs go
-        // out before the build, so the MFMAs start without waiting on LDS
-        // (measured on MI355X: 157.0 against 159.2 ms per 128-unit launch;
-        // reading the whole first group ahead spilled: 196 ms)
+        // the first two fragments of this stage's MFMAs go out before the
+        // build, so the MFMAs start without waiting on LDS (measured on
+        // MI355X, 128 units, in one run each: N = 350 157.0 against 159.2 ms,
+        // the whole first group read ahead spilled: 196 ms; N = 160 40.8
+        // against 41.8, N = 256 81.1-82.2 against 81.1-83.1)
         TriPre pre;
         if constexpr (TRI && !RUNS) {
             pre = tri_pre<MODE, IM, (TRI ? QT : 0)>(cur, fo);
+            __builtin_amdgcn_sched_barrier(0);
+        } else if constexpr (RUNS && NBK <= 8) {
+            // one item of every block (NBK <= 8); with a strip item too
+            // (NBK 9, 10) the read-ahead spilled (N = 300: 153.0 against
+            // 145.8 ms)
+            constexpr TRun run0 = tri_run((RUNS ? NBK : 5), (STRIP ? 1 : 0), (DQ & 3), 0);
+            if constexpr (run0.r >= 0) pre = runs_pre<IM>(cur, fo, run0.r, run0.c0);
             __builtin_amdgcn_sched_barrier(0);
         }
         // both waves of a SIMD build, then both run their MFMAs (measured on
@@ -2746,9 +2780,15 @@ __device__ __forceinline__ void mu_split(lds_char* in0, lds_char* in1, lds_char*
         wait_vm_lgkm<0, 0>();
         lds_barrier();
         if (st + 2 < n_st) HV_MUISSUE(odd ? in1 : in0, st + 2);
+        // the first two fragments of this stage's MFMAs go out before the
+        // build (N = 128, 128 units: 27.0-27.5 against 27.2-28.0 ms)
+        constexpr TRun run0 = mu_run(NBK, U, Q, 0);
+        TriPre pre;
+        if constexpr (run0.r >= 0) pre = runs_pre<IM>(cur, fo, run0.r, run0.c0);
+        __builtin_amdgcn_sched_barrier(0);
         t192_build_mu<MODE, NR>(odd ? in0 : in1, odd ? op0 : op1, row0, lane, arow, ax, ay, az);
         __builtin_amdgcn_sched_barrier(0);
-        mu_runs_mma<MODE, IM, NBK, U, Q>(cur, fo, acc);
+        mu_runs_mma<MODE, IM, NBK, U, Q, true>(cur, fo, acc, &pre);
     }
 #undef HV_MUISSUE
     wait_vm_lgkm<0, 0>();
