@@ -288,3 +288,52 @@ def test_config5_mix_equals_alone_and_oracle(full_catalogs):
             err = rel_err(d.vis[f, t][np.ix_(S, S)], ref[k])
             assert err < SERVE_TOL["split_bf16"], f"job {j.job_id} unit ({f}, {t}): rel err {err:.2e}"
         del d.vis, alone
+
+
+@pytest.fixture
+def ksplit_items_1000():
+    # the K-split rule's work-item threshold lowered from 32768 to 1000, so
+    # HERA-350 jobs of two times take 32768-source splits from 98304 sources
+    from hera_gpu_amd import _lib
+    lib = _lib.load()
+    _lib.check(lib.hv_set_ksplit_items(1000))
+    try:
+        yield
+    finally:
+        _lib.check(lib.hv_set_ksplit_items(0))
+
+
+def test_long_split_jobs_served_equal_alone(ksplit_items_1000):
+    # HERA-350 jobs of 2 times x 64 frequencies: 70000 sources keep
+    # 8192-source splits (768 work items with long ones), 100000 and 120000
+    # take 32768-source splits (1024).  All three share the bucket of padded
+    # sizes up to 131072, so the split length in the bucket key keeps the
+    # first apart; the third replays the second's graph.  Served results
+    # equal the stand-alone runs bit for bit and match the oracle.
+    from oracle.vis_oracle import rel_err, vis_oracle_units
+    antpos, bm, crd, flux = configs.serving_catalog("hera350", nsrc_max=131072)
+    beam, sky = stages.Beam(bm), stages.Sky(crd, flux)
+    freqs = np.linspace(100e6, 200e6, 64)
+    eq = configs.eq2tops_lst(2, lst0_deg=40.0).astype(np.float64)
+    eq2 = configs.eq2tops_lst(2, lst0_deg=95.0).astype(np.float64)
+    assert stages.ksplit_choice("split_bf16", 350, 128, 70000) == 8192
+    assert stages.ksplit_choice("split_bf16", 350, 128, 100000) == 32768
+    try:
+        with Server(nslots=2) as srv:
+            a = srv.wait(srv.submit(sky, beam, antpos, freqs, eq, nsrc=70000), 300)
+            b = srv.wait(srv.submit(sky, beam, antpos, freqs, eq, nsrc=100000), 300)
+            c = srv.wait(srv.submit(sky, beam, antpos, freqs, eq2, nsrc=120000), 300)
+            st = srv.stats()
+        assert a.status == 0 and b.status == 0 and c.status == 0
+        assert not b.graph_hit and c.graph_hit
+        assert st["instances"] >= 2
+        for d, e, n in ((a, eq, 70000), (b, eq, 100000), (c, eq2, 120000)):
+            alone, _ = stages.simulate(sky, beam, antpos, freqs, e, nsrc=n, gemm="split_bf16")
+            np.testing.assert_array_equal(d.vis, alone, err_msg=f"{n} sources")
+            del alone
+        S = SAMPLE_ANTS[350]
+        ref = vis_oracle_units(antpos[S], freqs, eq2, crd[:, :120000], flux[:120000], bm[S], [(63, 1)])
+        assert rel_err(c.vis[63, 1][np.ix_(S, S)], ref[0]) < SERVE_TOL["split_bf16"]
+    finally:
+        beam.close()
+        sky.close()
